@@ -1,0 +1,188 @@
+"""Benchmark: incoherent ray casting on one MI355X.
+
+Workload (one step): 16,777,216 closest-hit rays plus 16,777,216 shadow
+segments against a displaced icosphere of 1,310,720 triangles (20 * 4^8, the
+icosphere level closest to 2^20), SAH BVH built with the reference's builder.
+Rays: origins uniform in the mesh bbox scaled 1.2x, directions uniform on S^2;
+segment endpoints uniform in the same box.  Inputs are resident in HBM before
+timing starts.
+
+Prints one JSON line: Grays/s over both ray kinds, the closest-hit kernel's
+roofline entry (algorithmic bytes from the reference-order traversal's own
+node/primitive test counts, over its event-timed duration), and a CPU baseline
+(the oracle's restatement of Bvh::intersect / Bvh::visibility on host cores,
+timed on a bounded sample of the same rays).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
+NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n, counts, out_bytes):
+    c = counts.sum(axis=0, dtype=np.float64)
+    return n * (RAY_BYTES + out_bytes) + NODE_BYTES * c[0] + TRI_BYTES * c[1] + SPHERE_BYTES * c[2]
+
+
+def cpu_baseline(spec, rays8, segs8, sample, threads):
+    from gammaray_amd import scenes
+    from oracle.oracle import OracleScene
+    t0 = time.time()
+    o = OracleScene(spec)
+    log("oracle BVH built in %.1f s" % (time.time() - t0))
+    r6 = scenes.rays6(rays8[:sample])
+    s6 = scenes.rays6(segs8[:sample])
+    o.intersect(r6[:4096], threads=threads)  # warm
+    t0 = time.perf_counter()
+    o.intersect(r6, threads=threads)
+    t1 = time.perf_counter()
+    o.visibility(s6, threads=threads)
+    t2 = time.perf_counter()
+    return dict(value=2 * sample / (t2 - t0) / 1e9, unit="Grays/s", cores=threads, kind="port",
+                sample="%d closest-hit rays + %d shadow segments of the same batches "
+                       "(oracle Bvh::intersect / Bvh::visibility restatement, %d threads; "
+                       "closest %.2f s, shadow %.2f s)" % (sample, sample, threads, t1 - t0, t2 - t1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--occlusion", choices=["exact", "anyhit"], default="exact")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the film/"
+              "ray batches would shard by tiles; see the design notes). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gammaray_amd import scenes
+    from gammaray_amd.api import GR_OCCLUSION_ANYHIT, GR_OCCLUSION_EXACT, Context, Stage
+
+    mode = GR_OCCLUSION_EXACT if args.occlusion == "exact" else GR_OCCLUSION_ANYHIT
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    t0 = time.time()
+    spec = scenes.icosphere_scene(level=args.level, seed=2)
+    log("icosphere L%d: %d triangles (%.1f s)" % (args.level, spec.num_components, time.time() - t0))
+    ctx = Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    stage = Stage(spec, ctx=ctx)
+    info = stage.info()
+    log("BVH: %d nodes, depth %d, built in %.2f s" % (info["num_nodes"], info["max_depth"],
+                                                       info["build_seconds"]))
+    n = args.rays
+    t0 = time.time()
+    rays8 = scenes.ray_batch(spec, n, seed=20)
+    segs8 = scenes.segment_batch(spec, n, seed=21)
+    log("ray batches generated (%.1f s)" % (time.time() - t0))
+    rays = torch.from_numpy(rays8).to(dev)
+    segs = torch.from_numpy(segs8).to(dev)
+    hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    vis = torch.empty((n,), dtype=torch.int32, device=dev)
+
+    # Work counts of the reference-order traversal (untimed pass).
+    _, cnt_c = stage.intersect(rays, counts=True)
+    _, cnt_s = stage.visibility(segs, mode=mode, counts=True)
+    torch.cuda.synchronize()
+    cnt_c = cnt_c.cpu().numpy().view(np.uint32)
+    cnt_s = cnt_s.cpu().numpy().view(np.uint32)
+    bytes_closest = algorithmic_bytes(n, cnt_c, HIT_BYTES)
+    bytes_shadow = algorithmic_bytes(n, cnt_s, VIS_BYTES)
+    log("per ray: closest %.1f box %.2f tri; shadow %.1f box %.2f tri" % (
+        cnt_c[:, 0].mean(), cnt_c[:, 1].mean(), cnt_s[:, 0].mean(), cnt_s[:, 1].mean()))
+    del cnt_c, cnt_s
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        stage.intersect_ptr(rays.data_ptr(), n, hits.data_ptr())
+        if ev is not None:
+            ev[1].record(stream)
+        stage.occluded_ptr(segs.data_ptr(), n, vis.data_ptr(), mode)
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    if stage.take_overflow_count() != 0:
+        log("warning: traversal stack overflow reported")
+    ms_closest = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    ms_shadow = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    ms_step = wall * 1e3 / args.steps
+    value = 2.0 * n / (ms_step * 1e-3) / 1e9
+    log("closest %.3f ms, shadow %.3f ms, step %.3f ms" % (ms_closest, ms_shadow, ms_step))
+
+    dominant = "closest" if ms_closest >= ms_shadow else "shadow"
+    ms_dom = ms_closest if dominant == "closest" else ms_shadow
+    bytes_dom = bytes_closest if dominant == "closest" else bytes_shadow
+    achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_intersect" if dominant == "closest" else "k_occluded",
+                    algorithmic_bytes_per_launch=int(bytes_dom),
+                    kernel_ms=round(ms_dom, 4),
+                    shadow_kernel_gbs=round(bytes_shadow / (ms_shadow * 1e-3) / 1e9, 1),
+                    shadow_kernel_ms=round(ms_shadow, 4))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(spec, rays8, segs8, min(args.cpu_sample, n), threads)
+        log("cpu baseline %.4f Grays/s on %d threads" % (cpu["value"], threads))
+
+    out = {
+        "metric": "Grays/s (closest-hit + shadow rays, incoherent, 1 GPU)",
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "incoherent ray cast, displaced icosphere L%d" % args.level,
+                   "triangles": spec.num_components, "closest_rays": n, "shadow_segments": n,
+                   "occlusion_mode": args.occlusion, "bvh_nodes": info["num_nodes"],
+                   "bvh_depth": info["max_depth"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

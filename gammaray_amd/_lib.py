@@ -1,0 +1,82 @@
+"""ctypes binding of libgammaray_amd.so (the C ABI in include/gammaray.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgammaray_amd.so")
+
+GR_OK = 0
+GR_HOST_ONLY = -1
+GR_NONE = 0xFFFFFFFF
+GR_OCCLUSION_EXACT = 0
+GR_OCCLUSION_ANYHIT = 1
+ERRORS = {-1: "GR_ERR_INVALID", -2: "GR_ERR_HIP", -3: "GR_ERR_NOMEM", -4: "GR_ERR_STATE",
+          -5: "GR_ERR_OVERFLOW"}
+
+_vp = ctypes.c_void_p
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i = ctypes.c_int
+_u32 = ctypes.c_uint32
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("num_prims", _u32), ("num_components", _u32), ("num_nodes", _u32),
+                ("num_interior", _u32), ("num_lights", _u32), ("max_depth", _u32),
+                ("node_bytes", ctypes.c_uint64), ("component_bytes", ctypes.c_uint64),
+                ("build_seconds", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/gammaray.h.
+SIGNATURES = {
+    "gr_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "gr_ctx_destroy": (None, [_vp]),
+    "gr_ctx_set_stream": (_i, [_vp, _vp]),
+    "gr_ctx_stream": (_vp, [_vp]),
+    "gr_ctx_synchronize": (_i, [_vp]),
+    "gr_ctx_last_error": (ctypes.c_char_p, [_vp]),
+    "gr_scene_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "gr_scene_destroy": (None, [_vp]),
+    "gr_scene_add_mesh": (_i, [_vp, _f32p, _u32, _f32p, _u32, _f32p, _u32, _u32p, _u32, _f32p, _u32p]),
+    "gr_scene_add_sphere": (_i, [_vp, _f32p, ctypes.c_float, _f32p, _u32p]),
+    "gr_scene_commit": (_i, [_vp]),
+    "gr_scene_info_get": (_i, [_vp, ctypes.POINTER(SceneInfo)]),
+    "gr_scene_export_bvh": (_i, [_vp, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p]),
+    "gr_intersect_closest": (_i, [_vp, _vp, _u32, _vp, _vp]),
+    "gr_occluded": (_i, [_vp, _vp, _u32, _vp, _i, _vp]),
+    "gr_scene_take_overflow_count": (_i, [_vp, _u64p]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgammaray_amd.so is not built (run `python -m gammaray_amd.build`); "
+                               "there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class GrError(RuntimeError):
+    pass
+
+
+def check(code, ctx=None, what=""):
+    if code != GR_OK:
+        msg = ""
+        if ctx is not None:
+            raw = lib().gr_ctx_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise GrError("%s failed: %s %s" % (what, ERRORS.get(code, code), msg))

@@ -1,0 +1,162 @@
+"""Python host API over the C ABI, named after the reference's types.
+
+``Stage`` is the counterpart of gammaray's ``Stage`` (src/render/stage.rs): it
+takes the scene's prims, builds the BVH with the reference's SAH builder and
+owns the GPU copy.  ``intersect`` / ``visibility`` are the batched forms of
+``Bvh::intersect`` / ``Bvh::visibility`` (src/geom/bvh.rs:406-477) over rays
+held in device memory (torch tensors on the GPU are used as plain buffers).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import GR_HOST_ONLY, GR_NONE, GR_OCCLUSION_ANYHIT, GR_OCCLUSION_EXACT, check, lib
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """A device context (one HIP stream).  device=GR_HOST_ONLY builds scenes
+    without touching a GPU."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        code = lib().gr_ctx_create(device, ctypes.byref(self._h))
+        if code != 0:
+            raise _lib.GrError("gr_ctx_create(%d) failed: %s" % (device, _lib.ERRORS.get(code, code)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        check(lib().gr_ctx_set_stream(self._h, ctypes.c_void_p(stream_ptr)), self._h, "set_stream")
+
+    def stream(self):
+        return lib().gr_ctx_stream(self._h)
+
+    def synchronize(self):
+        check(lib().gr_ctx_synchronize(self._h), self._h, "synchronize")
+
+    def close(self):
+        if self._h:
+            lib().gr_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stage:
+    """Scene + BVH on one device: Stage::new(prims) (src/render/stage.rs:16-21)."""
+
+    def __init__(self, spec, ctx=None, device=0):
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.spec = spec
+        L = lib()
+        self._h = ctypes.c_void_p()
+        check(L.gr_scene_create(self.ctx.handle, ctypes.byref(self._h)), self.ctx.handle, "scene_create")
+        for prim in spec.prims:
+            mat = np.ascontiguousarray(prim.material, np.float32)
+            out = ctypes.c_uint32()
+            if prim.kind == "mesh":
+                v = np.ascontiguousarray(prim.vertices, np.float32)
+                n = None if prim.normals is None else np.ascontiguousarray(prim.normals, np.float32)
+                uv = None if prim.uvs is None else np.ascontiguousarray(prim.uvs, np.float32)
+                t = np.ascontiguousarray(prim.tris, np.uint32)
+                code = L.gr_scene_add_mesh(self._h, _p(v, _f32p), len(v), _p(n, _f32p),
+                                           0 if n is None else len(n), _p(uv, _f32p),
+                                           0 if uv is None else len(uv), _p(t, _u32p), len(t),
+                                           _p(mat, _f32p), ctypes.byref(out))
+            else:
+                c = np.ascontiguousarray(prim.center, np.float32)
+                code = L.gr_scene_add_sphere(self._h, _p(c, _f32p), float(prim.radius),
+                                             _p(mat, _f32p), ctypes.byref(out))
+            check(code, self.ctx.handle, "scene_add")
+        check(L.gr_scene_commit(self._h), self.ctx.handle, "scene_commit")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = _lib.SceneInfo()
+        check(lib().gr_scene_info_get(self._h, ctypes.byref(inf)), self.ctx.handle, "scene_info")
+        return {k: getattr(inf, k) for k, _ in _lib.SceneInfo._fields_}
+
+    def export_bvh(self):
+        inf = self.info()
+        nn, nc = inf["num_nodes"], inf["num_components"]
+        bbox = np.zeros((nn, 6), np.float32)
+        off = np.zeros(nn, np.uint32)
+        cnt = np.zeros(nn, np.uint32)
+        axis = np.zeros(nn, np.uint32)
+        cp = np.zeros(nc, np.uint32)
+        ci = np.zeros(nc, np.uint32)
+        check(lib().gr_scene_export_bvh(self._h, _p(bbox, _f32p), _p(off, _u32p), _p(cnt, _u32p),
+                                        _p(axis, _u32p), _p(cp, _u32p), _p(ci, _u32p)),
+              self.ctx.handle, "export_bvh")
+        return dict(bbox=bbox, offset=off, count=cnt, axis=axis, comp_prim=cp, comp_index=ci)
+
+    # -- batched traversal on device buffers --------------------------------
+    def intersect_ptr(self, rays_ptr, n, hits_ptr, counts_ptr=None):
+        check(lib().gr_intersect_closest(self._h, ctypes.c_void_p(rays_ptr), n,
+                                         ctypes.c_void_p(hits_ptr),
+                                         ctypes.c_void_p(counts_ptr) if counts_ptr else None),
+              self.ctx.handle, "intersect_closest")
+
+    def occluded_ptr(self, segs_ptr, n, vis_ptr, mode=GR_OCCLUSION_EXACT, counts_ptr=None):
+        check(lib().gr_occluded(self._h, ctypes.c_void_p(segs_ptr), n, ctypes.c_void_p(vis_ptr), mode,
+                                ctypes.c_void_p(counts_ptr) if counts_ptr else None),
+              self.ctx.handle, "occluded")
+
+    def intersect(self, rays, counts=False):
+        """rays: (n, 8) float32 torch tensor on the context's device.  Returns
+        (hits (n, 4) int32 view of gr_hit, counts or None)."""
+        import torch
+        n = rays.shape[0]
+        hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        cnt = torch.empty((n, 3), dtype=torch.int32, device=rays.device) if counts else None
+        self.intersect_ptr(rays.data_ptr(), n, hits.data_ptr(), cnt.data_ptr() if counts else None)
+        return hits, cnt
+
+    def visibility(self, segs, mode=GR_OCCLUSION_EXACT, counts=False):
+        import torch
+        n = segs.shape[0]
+        vis = torch.empty((n,), dtype=torch.int32, device=segs.device)
+        cnt = torch.empty((n, 3), dtype=torch.int32, device=segs.device) if counts else None
+        self.occluded_ptr(segs.data_ptr(), n, vis.data_ptr(), mode, cnt.data_ptr() if counts else None)
+        return vis, cnt
+
+    def take_overflow_count(self):
+        v = ctypes.c_uint64()
+        check(lib().gr_scene_take_overflow_count(self._h, ctypes.byref(v)), self.ctx.handle, "overflow")
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gr_scene_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hits_to_numpy(hits):
+    """(n, 4) int32 tensor -> dict(prim, comp, t, slot) numpy arrays."""
+    h = hits.cpu().numpy().view(np.uint32)
+    return dict(prim=h[:, 0].copy(), comp=h[:, 1].copy(), dist=h[:, 2].view(np.float32).copy(),
+                slot=h[:, 3].copy())

@@ -1,0 +1,323 @@
+// gr_api.cpp -- the extern "C" boundary (include/gammaray.h).
+//
+// Ownership mirrors the reference: a scene takes copies of its prims and is
+// immutable after commit (Stage owns Bvh owns prims, src/render/stage.rs:10-21);
+// the film is a separate object borrowed by each trace call.  No call aborts
+// or throws across the boundary: failures return a negative code and leave a
+// message in gr_ctx_last_error().
+#pragma clang fp contract(off)
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "gr_internal.h"
+
+using namespace gr;
+
+namespace {
+
+int fail(gr_ctx* ctx, int code, const char* fmt, const char* detail = nullptr) {
+    if (ctx) {
+        char buf[512];
+        std::snprintf(buf, sizeof(buf), fmt, detail ? detail : "");
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_fail(gr_ctx* ctx, hipError_t e, const char* where) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+    if (ctx) ctx->err = buf;
+    return GR_ERR_HIP;
+}
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& src, hipStream_t s) {
+    *dst = nullptr;
+    size_t bytes = src.size() * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), bytes);
+    if (e != hipSuccess) return e;
+    if (!src.empty()) e = hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+void free_device(gr_scene* s) {
+    (void)hipFree(s->d_nodes);
+    (void)hipFree(s->d_comps);
+    (void)hipFree(s->d_comp_ids);
+    (void)hipFree(s->d_overflow);
+    s->d_nodes = nullptr;
+    s->d_comps = nullptr;
+    s->d_comp_ids = nullptr;
+    s->d_overflow = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gr_ctx_create(int device, gr_ctx** out) {
+    if (!out) return GR_ERR_INVALID;
+    *out = nullptr;
+    if (device == GR_HOST_ONLY) {
+        gr_ctx* c = new (std::nothrow) gr_ctx();
+        if (!c) return GR_ERR_NOMEM;
+        c->device = GR_HOST_ONLY;
+        *out = c;
+        return GR_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return GR_ERR_INVALID;
+    gr_ctx* c = new (std::nothrow) gr_ctx();
+    if (!c) return GR_ERR_NOMEM;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return GR_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return GR_OK;
+}
+
+void gr_ctx_destroy(gr_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->device == GR_HOST_ONLY) {
+        delete ctx;
+        return;
+    }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) {
+        (void)hipStreamSynchronize(ctx->own_stream);
+        (void)hipStreamDestroy(ctx->own_stream);
+    }
+    delete ctx;
+}
+
+int gr_ctx_set_stream(gr_ctx* ctx, void* stream) {
+    if (!ctx) return GR_ERR_INVALID;
+    ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own_stream;
+    return GR_OK;
+}
+
+void* gr_ctx_stream(gr_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
+
+int gr_ctx_synchronize(gr_ctx* ctx) {
+    if (!ctx) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return GR_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(ctx, e, "hipStreamSynchronize");
+}
+
+const char* gr_ctx_last_error(gr_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gr_scene_create(gr_ctx* ctx, gr_scene** out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    gr_scene* s = new (std::nothrow) gr_scene();
+    if (!s) return fail(ctx, GR_ERR_NOMEM, "out of host memory%s");
+    s->ctx = ctx;
+    *out = s;
+    return GR_OK;
+}
+
+void gr_scene_destroy(gr_scene* s) {
+    if (!s) return;
+    if (s->ctx->device != GR_HOST_ONLY) {
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        free_device(s);
+    }
+    delete s;
+}
+
+int gr_scene_add_mesh(gr_scene* s, const float* verts, uint32_t nverts, const float* normals,
+                      uint32_t nnormals, const float* uvs, uint32_t nuvs, const uint32_t* tris,
+                      uint32_t ntris, const float* material, uint32_t* prim_out) {
+    if (!s) return GR_ERR_INVALID;
+    gr_ctx* ctx = s->ctx;
+    if (s->committed) return fail(ctx, GR_ERR_STATE, "scene already committed%s");
+    if (!verts || !tris || !material || nverts == 0 || ntris == 0)
+        return fail(ctx, GR_ERR_INVALID, "mesh needs vertices, triangles and a material%s");
+    if ((nnormals && !normals) || (nuvs && !uvs)) return fail(ctx, GR_ERR_INVALID, "null normal/uv array%s");
+    for (uint64_t i = 0; i < (uint64_t)ntris * 9; ++i) {
+        const uint32_t k = (uint32_t)(i % 9), v = tris[i];
+        if ((k < 3 && v >= nverts) || (k >= 3 && k < 6 && v != kNone && v >= nnormals) ||
+            (k >= 6 && v != kNone && v >= nuvs))
+            return fail(ctx, GR_ERR_INVALID, "triangle index out of range%s");
+    }
+    Prim p;
+    p.type = kPrimMesh;
+    std::memcpy(p.material, material, sizeof(p.material));
+    p.verts.resize(nverts);
+    for (uint32_t i = 0; i < nverts; ++i) p.verts[i] = mk(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    p.normals.resize(nnormals);
+    for (uint32_t i = 0; i < nnormals; ++i)
+        p.normals[i] = mk(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    p.uvs.resize(nuvs + 1);
+    for (uint32_t i = 0; i < nuvs; ++i) p.uvs[i] = mk(uvs[2 * i], uvs[2 * i + 1], 0.0f);
+    p.uvs[nuvs] = mk(0.0f, 0.0f, 0.0f);  // the zero uv of src/geom/mesh.rs:133-136
+    p.ntris = ntris;
+    p.tris.assign(tris, tris + 9 * (size_t)ntris);
+    for (uint32_t t = 0; t < ntris; ++t) {
+        uint32_t* tr = &p.tris[9 * (size_t)t];
+        // Missing or zero-length shading normals -> geometric normal
+        // (src/geom/mesh.rs:138-153); missing uvs -> the zero uv (:129-137).
+        bool has_n = tr[3] != kNone && tr[4] != kNone && tr[5] != kNone;
+        if (has_n) {
+            for (int k = 3; k < 6; ++k)
+                if (nearly_zero(dot(p.normals[tr[k]], p.normals[tr[k]]))) has_n = false;
+        }
+        if (!has_n) tr[3] = tr[4] = tr[5] = kNone;
+        if (tr[6] == kNone || tr[7] == kNone || tr[8] == kNone) tr[6] = tr[7] = tr[8] = nuvs;
+    }
+    s->prims.push_back(std::move(p));
+    if (prim_out) *prim_out = (uint32_t)(s->prims.size() - 1);
+    return GR_OK;
+}
+
+int gr_scene_add_sphere(gr_scene* s, const float center[3], float radius, const float* material,
+                        uint32_t* prim_out) {
+    if (!s) return GR_ERR_INVALID;
+    if (s->committed) return fail(s->ctx, GR_ERR_STATE, "scene already committed%s");
+    if (!center || !material) return fail(s->ctx, GR_ERR_INVALID, "sphere needs a centre and a material%s");
+    Prim p;
+    p.type = kPrimSphere;
+    std::memcpy(p.material, material, sizeof(p.material));
+    p.center = mk(center[0], center[1], center[2]);
+    p.radius = radius;
+    s->prims.push_back(std::move(p));
+    if (prim_out) *prim_out = (uint32_t)(s->prims.size() - 1);
+    return GR_OK;
+}
+
+int gr_scene_commit(gr_scene* s) {
+    if (!s) return GR_ERR_INVALID;
+    gr_ctx* ctx = s->ctx;
+    if (s->committed) return fail(ctx, GR_ERR_STATE, "scene already committed%s");
+    if (s->prims.empty()) return fail(ctx, GR_ERR_INVALID, "scene has no prims%s");
+    size_t total = 0;
+    for (const Prim& p : s->prims) total += p.num_components();
+    if (total >= (size_t)kLeafBit) return fail(ctx, GR_ERR_INVALID, "too many components%s");
+    auto t0 = std::chrono::steady_clock::now();
+    try {
+        s->tree = build_reference_bvh(s->prims);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, GR_ERR_NOMEM, "out of host memory building the BVH%s");
+    }
+    GpuLayout lay = make_gpu_layout(s->tree, s->prims);
+    s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    s->num_interior = (uint32_t)lay.nodes.size();
+    s->num_lights = 0;
+    for (const Prim& p : s->prims)
+        if ((int)p.material[0] == 1) s->num_lights++;
+    if (ctx->device == GR_HOST_ONLY) {
+        s->committed = true;
+        return GR_OK;
+    }
+
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = upload(&s->d_nodes, lay.nodes, ctx->stream);
+    if (e == hipSuccess) e = upload(&s->d_comps, lay.comps, ctx->stream);
+    if (e == hipSuccess) e = upload(&s->d_comp_ids, lay.comp_ids, ctx->stream);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_overflow), sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_overflow, 0, sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        free_device(s);
+        return hip_fail(ctx, e, "scene upload");
+    }
+    SceneView& v = s->view;
+    v.nodes = s->d_nodes;
+    v.comps = s->d_comps;
+    v.comp_ids = s->d_comp_ids;
+    v.root_ref = lay.root_ref;
+    for (int k = 0; k < 3; ++k) {
+        v.root_lo[k] = lay.root_lo[k];
+        v.root_hi[k] = lay.root_hi[k];
+    }
+    v.pad = slab_pad();
+    v.overflow = s->d_overflow;
+    s->committed = true;
+    return GR_OK;
+}
+
+int gr_scene_info_get(gr_scene* s, gr_scene_info* out) {
+    if (!s || !out) return GR_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->num_prims = (uint32_t)s->prims.size();
+    out->num_components = (uint32_t)s->tree.comp_prim.size();
+    out->num_nodes = (uint32_t)s->tree.nodes.size();
+    out->num_interior = s->num_interior;
+    out->num_lights = s->num_lights;
+    out->max_depth = s->tree.max_depth;
+    out->node_bytes = (uint64_t)s->num_interior * sizeof(GpuNode);
+    out->component_bytes = (uint64_t)out->num_components * (sizeof(GpuComp) + sizeof(uint2));
+    out->build_seconds = s->build_seconds;
+    return s->committed ? GR_OK : fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+}
+
+int gr_scene_export_bvh(gr_scene* s, float* bbox6, uint32_t* offset, uint32_t* count,
+                        uint32_t* axis, uint32_t* comp_prim, uint32_t* comp_index) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    const auto& nodes = s->tree.nodes;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if (bbox6) {
+            for (int k = 0; k < 3; ++k) {
+                bbox6[6 * i + k] = nodes[i].lo[k];
+                bbox6[6 * i + 3 + k] = nodes[i].hi[k];
+            }
+        }
+        if (offset) offset[i] = nodes[i].offset;
+        if (count) count[i] = nodes[i].count;
+        if (axis) axis[i] = nodes[i].axis;
+    }
+    for (size_t i = 0; i < s->tree.comp_prim.size(); ++i) {
+        if (comp_prim) comp_prim[i] = s->tree.comp_prim[i];
+        if (comp_index) comp_index[i] = s->tree.comp_index[i];
+    }
+    return GR_OK;
+}
+
+int gr_intersect_closest(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits,
+                         uint32_t* d_counts) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    if (n && (!d_rays || !d_hits)) return fail(s->ctx, GR_ERR_INVALID, "null ray/hit buffer%s");
+    hipError_t e = launch_intersect(s->view, d_rays, n, d_hits, d_counts, s->ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "intersect launch");
+}
+
+int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_visible, int mode,
+                uint32_t* d_counts) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (mode != GR_OCCLUSION_EXACT && mode != GR_OCCLUSION_ANYHIT)
+        return fail(s->ctx, GR_ERR_INVALID, "unknown occlusion mode%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    if (n && (!d_segs || !d_visible)) return fail(s->ctx, GR_ERR_INVALID, "null segment/output buffer%s");
+    hipError_t e = launch_occluded(s->view, d_segs, n, d_visible, mode, d_counts, s->ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "occlusion launch");
+}
+
+int gr_scene_take_overflow_count(gr_scene* s, uint64_t* out) {
+    if (!s || !out) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    unsigned long long v = 0;
+    hipError_t e = hipMemcpyAsync(&v, s->d_overflow, sizeof(v), hipMemcpyDeviceToHost, s->ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_overflow, 0, sizeof(v), s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "overflow readback");
+    *out = v;
+    return GR_OK;
+}
+
+}  // extern "C"

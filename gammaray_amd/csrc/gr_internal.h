@@ -1,0 +1,125 @@
+// gr_internal.h -- library-internal types: context, scene, host BVH and the
+// GPU layout of the tree.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gammaray.h"
+#include "gr_math.h"
+
+namespace gr {
+
+// ---- host scene -----------------------------------------------------------
+enum PrimType : uint32_t { kPrimMesh = 0, kPrimSphere = 1 };
+
+struct Prim {
+    PrimType type;
+    float material[GR_MAT_FLOATS];
+    // mesh
+    std::vector<vec3> verts, normals, uvs;  // uvs: (u, v, 0); the last entry is the zero uv
+    std::vector<uint32_t> tris;             // 9 per triangle, sentinels resolved
+    uint32_t ntris = 0;
+    // sphere
+    vec3 center{0, 0, 0};
+    float radius = 0;
+    uint32_t num_components() const { return type == kPrimMesh ? ntris : 1u; }
+};
+
+// Reference-order flattened tree (BvhLinearNode, src/geom/bvh.rs:87-92).
+struct FlatNode {
+    float lo[3], hi[3];
+    uint32_t offset;  // leaf: first ordered component; interior: second child
+    uint32_t count;   // 0 = interior
+    uint32_t axis;
+};
+
+struct FlatTree {
+    std::vector<FlatNode> nodes;
+    std::vector<uint32_t> comp_prim, comp_index;  // ordered components
+    uint32_t max_depth = 0;
+};
+
+// Build with the reference's SAH algorithm (src/geom/bvh.rs:182-382).
+FlatTree build_reference_bvh(const std::vector<Prim>& prims);
+
+// ---- GPU layout -------------------------------------------------------------
+// One 64-byte node per interior node of the reference tree, holding the boxes
+// of both children so one fetch serves both child tests.  Child refs: interior
+// -> node index; leaf -> kLeafBit | first component slot.  Visiting order and
+// every box decision stay those of the reference traversal.
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+struct alignas(64) GpuNode {
+    float4 q0;  // lo0.x lo0.y lo0.z hi0.x     child 0 = first child (DFS i+1)
+    float4 q1;  // hi0.y hi0.z lo1.x lo1.y     child 1 = second child (offset)
+    float4 q2;  // lo1.z hi1.x hi1.y hi1.z
+    uint4 q3;   // ref0 ref1 axis 0
+};
+static_assert(sizeof(GpuNode) == 64, "node layout");
+
+// One 48-byte record per ordered component (leaf order).  Triangles store
+// c, a - c, b - c (the reference's edge1/edge2, computed with the same f32
+// subtraction); spheres store centre and radius.  q2.y holds the tag.
+constexpr uint32_t kTagSphere = 1u;
+constexpr uint32_t kTagLast = 2u;  // last component of its leaf
+
+struct alignas(16) GpuComp {
+    float4 q0;  // tri: c.x c.y c.z e1.x       sphere: c.x c.y c.z r
+    float4 q1;  // tri: e1.y e1.z e2.x e2.y
+    float4 q2;  // tri: e2.z tag 0 0           sphere: 0 tag 0 0
+};
+static_assert(sizeof(GpuComp) == 48, "component layout");
+
+struct SceneView {
+    const GpuNode* nodes;
+    const GpuComp* comps;
+    const uint2* comp_ids;  // (prim, comp) per slot
+    uint32_t root_ref;
+    float root_lo[3], root_hi[3];
+    float pad;  // 1 + 2 gamma(3)
+    unsigned long long* overflow;
+};
+
+struct GpuLayout {
+    std::vector<GpuNode> nodes;
+    std::vector<GpuComp> comps;
+    std::vector<uint2> comp_ids;
+    uint32_t root_ref;
+    float root_lo[3], root_hi[3];
+};
+GpuLayout make_gpu_layout(const FlatTree& t, const std::vector<Prim>& prims);
+
+// ---- launchers (gr_traverse.hip) -------------------------------------------
+hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n, gr_hit* hits,
+                            uint32_t* counts, hipStream_t s);
+hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
+                           uint32_t* visible, int mode, uint32_t* counts, hipStream_t s);
+
+}  // namespace gr
+
+// ---- ABI objects ----------------------------------------------------------
+struct gr_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct gr_scene {
+    gr_ctx* ctx = nullptr;
+    std::vector<gr::Prim> prims;
+    bool committed = false;
+    gr::FlatTree tree;
+    uint32_t num_interior = 0, num_lights = 0;
+    double build_seconds = 0;
+    gr::GpuNode* d_nodes = nullptr;
+    gr::GpuComp* d_comps = nullptr;
+    uint2* d_comp_ids = nullptr;
+    unsigned long long* d_overflow = nullptr;
+    gr::SceneView view{};
+};

@@ -1,0 +1,126 @@
+/*
+ * gammaray.h -- C ABI of the MI355X-native gammaray hot path.
+ *
+ * The reference (sdao/gammaray, Rust) has no FFI; its per-ray, per-sample
+ * interfaces are too fine-grained for a GPU.  This ABI sits at the granularity
+ * a Rust (or any) host would bind through `extern "C"`:
+ *   - scene construction  <->  Stage::new(Vec<Box<Prim>>)          src/render/stage.rs:16-21
+ *   - batched traversal   <->  Bvh::intersect / Bvh::visibility    src/geom/bvh.rs:406-477
+ *   - one render pass     <->  Stage::trace(&Camera, &Integrator, &mut Film)
+ *                                                                  src/render/stage.rs:23-36
+ *   - film readback       <->  Film::pixels / ExrWriter::update    src/render/film.rs:38-115,
+ *                                                                  src/render/exr.rs:168-199
+ * Plain pointers and sizes only.  Every entry point returns GR_OK (0) or a
+ * negative GR_ERR_* code and never aborts; gr_ctx_last_error() gives the text.
+ * A context owns one HIP stream; calls on one context are stream-ordered and a
+ * context must be used by one host thread at a time.
+ */
+#ifndef GAMMARAY_H
+#define GAMMARAY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_OK 0
+#define GR_ERR_INVALID (-1)   /* bad argument */
+#define GR_ERR_HIP (-2)       /* HIP runtime failure */
+#define GR_ERR_NOMEM (-3)     /* allocation failure */
+#define GR_ERR_STATE (-4)     /* call out of order (e.g. scene not committed) */
+#define GR_ERR_OVERFLOW (-5)  /* a traversal stack overflowed (reference: panic, bvh.rs:156) */
+
+#define GR_NONE 0xffffffffu
+
+typedef struct gr_ctx gr_ctx;
+typedef struct gr_scene gr_scene;
+typedef struct gr_film gr_film;
+
+/* Ray: origin, pad, unit direction, pad (32 bytes; src/core/ray.rs:11-14). */
+typedef struct { float ox, oy, oz, pad0, dx, dy, dz, pad1; } gr_ray;
+/* Closest hit.  prim/comp = (prim index, component index) or GR_NONE on a miss;
+ * t = distance along the ray (Intersection::Hit.dist, src/geom/bvh.rs:108-115
+ * plus the component index the reference drops at bvh.rs:422). */
+typedef struct { uint32_t prim, comp; float t; uint32_t slot; } gr_hit;
+/* Shadow segment: start, pad, target, pad (arguments of Bvh::visibility). */
+typedef struct { float sx, sy, sz, pad0, tx, ty, tz, pad1; } gr_segment;
+
+/* ---- context ---------------------------------------------------------- */
+/* device >= 0: a HIP device.  device == GR_HOST_ONLY: no GPU is touched; scenes
+ * can be built and their tree exported, but nothing can be traced. */
+#define GR_HOST_ONLY (-1)
+int gr_ctx_create(int device, gr_ctx **out);
+void gr_ctx_destroy(gr_ctx *ctx);
+/* Run subsequent work on an existing HIP stream (hipStream_t); NULL restores
+ * the context's own stream. */
+int gr_ctx_set_stream(gr_ctx *ctx, void *hip_stream);
+void *gr_ctx_stream(gr_ctx *ctx);
+int gr_ctx_synchronize(gr_ctx *ctx);
+const char *gr_ctx_last_error(gr_ctx *ctx);
+
+/* ---- scene (Stage::new / Bvh::build) ------------------------------------ */
+/* Material record: 16 floats, rec[0] = kind:
+ *   0 Disney: [1..3] base colour, [4] roughness, [5] anisotropic, [6] ior,
+ *     [7] metallic, [8] specular_trans, [9] specular_tint, [10] sheen,
+ *     [11] sheen_tint, [12] clearcoat, [13] clearcoat_gloss
+ *     (DisneyMaterialBuilder, src/material/material.rs:256-389)
+ *   1 diffuse_light: [1..3] incandescence (material.rs:27-33)
+ *   2 diffuse (material.rs:35-43)       3 mirror (material.rs:45-53)        */
+#define GR_MAT_FLOATS 16
+
+int gr_scene_create(gr_ctx *ctx, gr_scene **out);
+void gr_scene_destroy(gr_scene *scene);
+/* Triangle mesh in world space (the state Mesh::from_obj builds,
+ * src/geom/mesh.rs:93-188).  tris: 9 uint32 per triangle (a,b,c vertex;
+ * an,bn,cn normal or GR_NONE = geometric normal; at,bt,ct uv or GR_NONE = zero
+ * uv).  Writes the prim index to *prim_out (may be NULL). */
+int gr_scene_add_mesh(gr_scene *scene, const float *verts, uint32_t nverts,
+                      const float *normals, uint32_t nnormals, const float *uvs, uint32_t nuvs,
+                      const uint32_t *tris, uint32_t ntris, const float *material,
+                      uint32_t *prim_out);
+/* Sphere::new(material, translation, radius), src/geom/sphere.rs:17-24. */
+int gr_scene_add_sphere(gr_scene *scene, const float center[3], float radius,
+                        const float *material, uint32_t *prim_out);
+/* Bvh::build (src/geom/bvh.rs:339-382) on the host -- the reference's SAH
+ * builder, so the tree is identical -- then upload of the GPU layout. */
+int gr_scene_commit(gr_scene *scene);
+
+typedef struct {
+    uint32_t num_prims, num_components, num_nodes, num_interior, num_lights, max_depth;
+    uint64_t node_bytes, component_bytes;
+    double build_seconds;
+} gr_scene_info;
+int gr_scene_info_get(gr_scene *scene, gr_scene_info *out);
+/* Copy of the flattened tree in the reference's BvhLinearNode order
+ * (src/geom/bvh.rs:87-106,310-337): bbox6 (6 floats per node), offset, count,
+ * axis per node; (prim, comp) per ordered component.  Any pointer may be NULL. */
+int gr_scene_export_bvh(gr_scene *scene, float *bbox6, uint32_t *offset, uint32_t *count,
+                        uint32_t *axis, uint32_t *comp_prim, uint32_t *comp_index);
+
+/* ---- batched traversal (device pointers) -------------------------------- */
+/* Bvh::intersect for n rays.  d_rays, d_hits: device memory.  If d_counts is
+ * not NULL it receives 3 uint32 per ray: bbox tests, triangle tests, sphere
+ * tests (the reference-order traversal's work, for the roofline's bytes). */
+int gr_intersect_closest(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
+                         uint32_t *d_counts);
+/* Bvh::visibility for n segments: d_visible[i] = 1 if target is visible from
+ * start, else 0.  mode GR_OCCLUSION_EXACT runs the reference's closest-hit
+ * traversal order and stops at the first hit that proves occlusion (result
+ * identical to the reference on every input); GR_OCCLUSION_ANYHIT also prunes
+ * nodes beyond the target (identical except documented ties within rounding of
+ * the target distance). */
+#define GR_OCCLUSION_EXACT 0
+#define GR_OCCLUSION_ANYHIT 1
+int gr_occluded(gr_scene *scene, const gr_segment *d_segs, uint32_t n, uint32_t *d_visible,
+                int mode, uint32_t *d_counts);
+/* Number of rays whose traversal stack overflowed since the last call (the
+ * reference panics on these, src/geom/bvh.rs:156-157).  Such rays report
+ * prim = GR_NONE with t = -1 (closest hit) or not-visible (visibility). */
+int gr_scene_take_overflow_count(gr_scene *scene, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+/*
+ * gr_oracle.h -- CPU oracle for the gammaray ray-traversal + integrator path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * renderer (sdao/gammaray, Rust) that the GPU library is checked against.  Only
+ * the test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it, and only as the checker / the timed CPU baseline.  The product
+ * library never links, loads or calls anything in this directory.
+ *
+ * Every function cites the reference file:line it restates.  The reference is
+ * f32 throughout and Rust never contracts a*b+c into an FMA, so this file is
+ * compiled with -ffp-contract=off -fno-fast-math and keeps the reference's
+ * evaluation order.  f32::min/max map to fminf/fmaxf (same NaN rule).
+ *
+ * Parity pins (what checks each part; see tests/):
+ *   - BVH build/flatten: structural invariants + hand-derived small trees.
+ *   - intersect: the reference's own brute force (intersect_naive) restated,
+ *     closed-form ray/triangle and ray/sphere hits, axis-aligned edge cases.
+ *   - visibility: equivalence with a brute-force segment test.
+ *   - RNG: Philox4x32-10 published known-answer vectors.
+ *   - samplers / lobes / integrators: closed forms, normalisation, chi-square
+ *     and analytic scenes (listed in the test files).
+ */
+#ifndef GR_ORACLE_H
+#define GR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene orc_scene;
+
+/* Material record: 16 floats.  rec[0] = kind:
+ *   0 = Disney (rec[1..3] base colour, [4] roughness, [5] anisotropic, [6] ior,
+ *       [7] metallic, [8] specular_trans, [9] specular_tint, [10] sheen,
+ *       [11] sheen_tint, [12] clearcoat, [13] clearcoat_gloss)
+ *   1 = diffuse_light (rec[1..3] incandescence)
+ *   2 = diffuse (PerfectDiffuse)      3 = mirror (PerfectMirror)          */
+#define ORC_MAT_FLOATS 16
+
+int  orc_scene_create(orc_scene **out);
+void orc_scene_destroy(orc_scene *s);
+/* Mesh from arrays (the reference only loads OBJ; this is the array form of
+ * Mesh::from_obj's result, src/geom/mesh.rs:93-188).  tris holds 9 uint32 per
+ * triangle: a,b,c (vertex), an,bn,cn (normal; 0xffffffff = use geometric
+ * normal), at,bt,ct (uv; 0xffffffff = zero uv).  Returns the prim index. */
+int  orc_scene_add_mesh(orc_scene *s, const float *verts, uint32_t nverts,
+                        const float *normals, uint32_t nnormals,
+                        const float *uvs, uint32_t nuvs,
+                        const uint32_t *tris, uint32_t ntris, const float *mat);
+int  orc_scene_add_sphere(orc_scene *s, const float *center, float radius, const float *mat);
+int  orc_scene_build(orc_scene *s);
+
+/* Flattened tree export (BvhLinearNode, src/geom/bvh.rs:87-92). */
+uint32_t orc_bvh_num_nodes(const orc_scene *s);
+uint32_t orc_bvh_num_components(const orc_scene *s);
+void orc_bvh_export(const orc_scene *s, float *bbox6, uint32_t *offset, uint32_t *ncomp,
+                    uint32_t *axis, uint32_t *comp_prim, uint32_t *comp_index);
+uint32_t orc_num_lights(const orc_scene *s);
+void orc_light_indices(const orc_scene *s, uint32_t *out);
+
+/* Batched closest hit (Bvh::intersect, src/geom/bvh.rs:406-456).  rays: 6 floats
+ * (origin, direction).  Outputs per ray: prim (0xffffffff = miss), component,
+ * dist.  counts (optional, 3 per ray): bbox tests, triangle tests, sphere tests.
+ * frame (optional, 12 per ray): normal, tangent, binormal, geom_normal.
+ * Returns the number of rays whose traversal overflowed the 64-entry stack
+ * (the reference panics there); those rays report prim = 0xffffffff and
+ * dist = -1. */
+int64_t orc_intersect(const orc_scene *s, const float *rays, int64_t n, uint32_t *prim,
+                      uint32_t *comp, float *dist, uint32_t *counts, float *frame, int nthreads);
+/* Brute force (Bvh::intersect_naive, src/geom/bvh.rs:385-400). */
+void orc_intersect_naive(const orc_scene *s, const float *rays, int64_t n, uint32_t *prim,
+                         uint32_t *comp, float *dist, int nthreads);
+/* Bvh::visibility (src/geom/bvh.rs:460-477).  segs: 6 floats (start, target).
+ * out: 1 = visible.  counts optional as for orc_intersect. */
+int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t *out,
+                       uint32_t *counts, int nthreads);
+
+/* Philox4x32-10 block function (counter-based RNG shared by the GPU path). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

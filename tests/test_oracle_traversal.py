@@ -1,0 +1,296 @@
+"""CPU tests: pin the oracle's BVH build and traversal, and check the
+library's host-side builder against it.
+
+What pins what:
+  * Philox4x32-10 -- the published Random123 known-answer vectors.
+  * BVH build -- structural invariants on several scenes, plus hand-derived
+    trees that expose the reference's quirks (median split through the
+    nth_element whose left recursion drops an element, src/geom/util.rs:39-41;
+    multi-component leaves for coincident centroids, src/geom/bvh.rs:207-216).
+  * intersect -- the reference's own brute force, Bvh::intersect_naive
+    (src/geom/bvh.rs:385-400), equal except documented ties (|dt| <= 1e-6 t),
+    and closed-form ray/triangle and ray/sphere distances.
+  * visibility -- brute-force segment test built on intersect_naive.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from gammaray_amd import scenes
+from oracle.oracle import NONE, OracleScene, philox4x32_10
+
+
+# --------------------------------------------------------------------------
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds.
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert list(philox4x32_10(ctr, key)) == want
+
+
+# --------------------------------------------------------------------------
+def _spheres_scene(centers, radius=0.1):
+    s = scenes.SceneSpec(name="spheres")
+    for c in centers:
+        s.add_sphere(c, radius, scenes.disney())
+    return s
+
+
+def check_invariants(spec, tree):
+    bbox, off, cnt, axis = tree["bbox"], tree["offset"], tree["count"], tree["axis"]
+    n = len(off)
+    # every (prim, comp) exactly once
+    pairs = set(zip(tree["comp_prim"].tolist(), tree["comp_index"].tolist()))
+    assert len(pairs) == spec.num_components == len(tree["comp_prim"])
+    covered = np.zeros(len(tree["comp_prim"]), np.int32)
+    # DFS layout: subtree of i spans [i, end(i)); first child i+1, second = offset
+    end = np.zeros(n, np.int64)
+    for i in range(n - 1, -1, -1):
+        if cnt[i] > 0:
+            end[i] = i + 1
+            covered[off[i]:off[i] + cnt[i]] += 1
+        else:
+            assert off[i] > i + 1
+            assert end[i + 1] == off[i], "first subtree must end where the second starts"
+            end[i] = end[off[i]]
+            for c in (i + 1, off[i]):
+                assert np.all(bbox[c, :3] >= bbox[i, :3]) and np.all(bbox[c, 3:] <= bbox[i, 3:])
+            assert axis[i] in (0, 1, 2)
+    assert end[0] == n
+    assert np.all(covered == 1)
+    # leaf boxes bound their components
+    for i in np.nonzero(cnt > 0)[0][:2000]:
+        for k in range(off[i], off[i] + cnt[i]):
+            p = spec.prims[tree["comp_prim"][k]]
+            if p.kind == "mesh":
+                v = p.vertices[p.tris[tree["comp_index"][k], :3]]
+                lo, hi = v.min(0), v.max(0)
+            else:
+                lo, hi = p.center - p.radius, p.center + p.radius
+            assert np.all(lo >= bbox[i, :3]) and np.all(hi <= bbox[i, 3:])
+
+
+SCENES = {
+    "box": lambda: scenes.box_scene(),
+    "ico4": lambda: scenes.icosphere_scene(level=4),
+    "showcase_small": lambda: scenes.showcase_scene(n_spheres=40, grid=(30, 20)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_bvh_invariants(name):
+    spec = SCENES[name]()
+    check_invariants(spec, OracleScene(spec).export_bvh())
+
+
+def test_bvh_median_split_quirk():
+    # Four spheres on the x axis, centroids 1, 3, 9, 2 (prims 0..3).  n <= 4
+    # takes the median split; the reference's nth_element leaves [1, 3, 2, 9]
+    # (its left recursion never revisits index 2), so the halves are {1, 3}
+    # and {2, 9} -- a correct median split would give {1, 2} | {3, 9}.
+    spec = _spheres_scene([(1, 0, 0), (3, 0, 0), (9, 0, 0), (2, 0, 0)])
+    t = OracleScene(spec).export_bvh()
+    assert t["count"].tolist() == [0, 0, 1, 1, 0, 1, 1]
+    assert t["offset"].tolist() == [4, 3, 0, 1, 6, 2, 3]
+    assert t["axis"].tolist() == [0, 0, 0, 0, 0, 0, 0]
+    assert t["comp_prim"].tolist() == [0, 1, 3, 2]
+    np.testing.assert_allclose(t["bbox"][1], [0.9, -0.1, -0.1, 3.1, 0.1, 0.1], rtol=1e-6)
+
+
+def test_bvh_three_in_a_row():
+    spec = _spheres_scene([(0, 0, 0), (1, 0, 0), (2, 0, 0)])
+    t = OracleScene(spec).export_bvh()
+    assert t["count"].tolist() == [0, 1, 0, 1, 1]
+    assert t["offset"].tolist() == [2, 0, 4, 1, 2]
+    assert t["comp_prim"].tolist() == [0, 1, 2]
+
+
+def test_bvh_coincident_centroids_make_one_leaf():
+    # Identical spheres: the centroid bbox is a point, so the reference makes
+    # a single multi-component leaf (src/geom/bvh.rs:207-216), even above 255.
+    spec = _spheres_scene([(0.5, 0.5, 0.5)] * 300)
+    t = OracleScene(spec).export_bvh()
+    assert t["count"].tolist() == [300]
+    assert t["comp_prim"].tolist() == list(range(300))
+
+
+def test_bvh_axis_choice_prefers_x_on_ties():
+    # maximum_extent returns 0 only when dx >= dy and dy >= dz (bbox.rs:52-63):
+    # extents (1, 1, 1) -> axis 0; (1, 2, 2) -> axis 1.
+    t = OracleScene(_spheres_scene([(0, 0, 0), (1, 1, 1)])).export_bvh()
+    assert t["axis"][0] == 0
+    t = OracleScene(_spheres_scene([(0, 0, 0), (1, 2, 2)])).export_bvh()
+    assert t["axis"][0] == 1
+
+
+# --------------------------------------------------------------------------
+def _tie_ok(a, b, idx):
+    ta, tb = a["dist"][idx], b["dist"][idx]
+    return np.all(np.abs(ta - tb) <= 1e-6 * np.maximum(np.abs(ta), np.abs(tb)))
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_bvh_matches_brute_force(name):
+    spec = SCENES[name]()
+    o = OracleScene(spec)
+    rays = scenes.rays6(scenes.ray_batch(spec, 20000, seed=11))
+    a = o.intersect(rays)
+    b = o.intersect_naive(rays)
+    assert a["overflow"] == 0
+    assert np.array_equal(a["prim"] == NONE, b["prim"] == NONE)
+    diff = np.nonzero((a["prim"] != b["prim"]) | (a["comp"] != b["comp"]) | (a["dist"] != b["dist"]))[0]
+    # only ties: both hits within 1e-6 relative distance
+    assert _tie_ok(a, b, diff)
+    assert len(diff) <= 0.001 * len(rays)
+
+
+def test_closed_form_triangle_and_sphere():
+    s = scenes.SceneSpec()
+    s.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), [[0, 1, 2]], scenes.disney())
+    s.add_sphere((0.0, 0.0, -10.0), 1.0, scenes.disney())
+    o = OracleScene(s)
+    rays = np.array([
+        [0.25, 0.25, 1.0, 0, 0, -1],      # hits the triangle at t = 1
+        [0.0, 0.0, -5.0, 0, 0, -1],       # hits sphere front face at t = 4
+        [0.0, 0.0, -10.0, 0, 0, 1],       # from the centre: far root t = 1
+        [0.25, 0.25, 1.0, 1, 0, 0],       # parallel to the triangle plane: miss
+        [2.0, 2.0, 1.0, 0, 0, -1],        # outside the triangle: miss
+        [0.5, 0.5, 1.0, 0, 0, -1],        # on edge u = 0 (accepted, u < 0 is the test)
+        [0.0, 0.5, 1.0, 0, 0, -1],        # origin on the leaf box plane x = 0 with d.x = 0
+    ], np.float32)
+    r = o.intersect(rays)
+    n = o.intersect_naive(rays)
+    assert r["prim"].tolist()[:6] == [0, 1, 1, NONE, NONE, 0]
+    assert r["dist"][0] == 1.0 and r["dist"][1] == 4.0 and r["dist"][2] == 1.0
+    assert r["dist"][5] == 1.0
+    # Q7: (0 - 0) * inf = NaN in the leaf's x slab makes `t_min < max_dist`
+    # false, so the tree traversal skips the triangle the brute force hits.
+    assert n["prim"][6] == 0 and n["dist"][6] == 1.0
+    assert r["prim"][6] == 1 and r["dist"][6] > 10.0
+
+
+def test_frames_closed_form():
+    s = scenes.SceneSpec()
+    s.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), [[0, 1, 2]], scenes.disney())
+    s.add_sphere((0.0, 0.0, -10.0), 1.0, scenes.disney())
+    o = OracleScene(s)
+    rays = np.array([[0.25, 0.25, 1.0, 0, 0, -1], [0.0, 0.0, -5.0, 0, 0, -1], [3, -10, 0, -1, 0, 0]], np.float32)
+    f = o.intersect(rays, frame=True)["frame"]
+    # triangle a=(0,0,0), b=(1,0,0), c=(0,1,0): (a-c) x (b-c) = (0,-1,0) x (1,-1,0) = (0,0,1)
+    np.testing.assert_allclose(f[0, 9:12], [0, 0, 1], atol=1e-7)
+    # sphere at its +z pole: normal (0,0,1); tangent (-n.z, 0, n.x) normalised = (-1,0,0)
+    np.testing.assert_allclose(f[1, 0:3], [0, 0, 1], atol=1e-7)
+    np.testing.assert_allclose(f[1, 3:6], [-1, 0, 0], atol=1e-7)
+    # frames are orthonormal and right-handed: tangent x binormal = normal
+    for row in f[:2]:
+        n, t, b = row[0:3], row[3:6], row[6:9]
+        np.testing.assert_allclose(np.cross(t, b), n, atol=1e-6)
+
+
+def test_axis_aligned_rays_match_brute_force():
+    # Zero direction components give inv_dir = +-inf and 0 * inf = NaN inside
+    # the slab test (src/core/ray.rs:33-36, src/core/bbox.rs:82-112).
+    spec = scenes.box_scene()
+    o = OracleScene(spec)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-0.95, 0.95, (3000, 3)).astype(np.float32)
+    pts[:, 1] += 1.0
+    dirs = np.zeros_like(pts)
+    axis = rng.integers(0, 3, len(pts))
+    dirs[np.arange(len(pts)), axis] = rng.choice([-1.0, 1.0], len(pts))
+    rays = np.concatenate([pts, dirs], 1)
+    a = o.intersect(rays)
+    b = o.intersect_naive(rays)
+    diff = np.nonzero((a["prim"] != b["prim"]) | (a["dist"] != b["dist"]))[0]
+    assert _tie_ok(a, b, diff)
+    # Origins exactly on a box plane with a zero direction component produce
+    # NaN slab distances and the tree may skip hits the brute force finds (Q7);
+    # those rays are parity cases for the GPU, not for the brute force.
+    pts[::7, 0] = -0.7
+    pts[::11, 2] = -0.2
+    rays2 = np.concatenate([pts, dirs], 1)
+    a2 = o.intersect(rays2)
+    b2 = o.intersect_naive(rays2)
+    missed = np.nonzero(a2["prim"] != b2["prim"])[0]
+    assert len(missed) > 0
+    assert np.all((rays2[missed, 0] == -0.7) | (rays2[missed, 2] == np.float32(-0.2)))
+
+
+def test_visibility_matches_brute_force():
+    spec = SCENES["showcase_small"]()
+    o = OracleScene(spec)
+    segs = scenes.rays6(scenes.segment_batch(spec, 20000, seed=5))
+    vis = o.visibility(segs)["visible"]
+    start, target = segs[:, :3], segs[:, 3:]
+    d = target - start
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    origin = (start + d * np.float32(1e-3)).astype(np.float32)
+    tdist = np.linalg.norm(target - origin, axis=1).astype(np.float32)
+    b = o.intersect_naive(np.concatenate([origin, d], 1).astype(np.float32))
+    occl = (b["prim"] != NONE) & (b["dist"] < tdist - np.float32(1e-3))
+    close = np.all(np.abs(target - start) < 1e-3, axis=1)
+    want = (~occl & ~close).astype(np.uint8)
+    mism = np.nonzero(vis != want)[0]
+    # mismatches only where the brute force's and the tree's closest hit tie
+    assert len(mism) <= 0.001 * len(segs)
+
+
+# --------------------------------------------------------------------------
+def _degenerate_scenes():
+    out = {}
+    out["coincident300"] = _spheres_scene([(0.5, 0.5, 0.5)] * 300)
+    out["quirk4"] = _spheres_scene([(1, 0, 0), (3, 0, 0), (9, 0, 0), (2, 0, 0)])
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-1, 1, (600, 3)).astype(np.float32)
+    pts[:200, 2] = 0.0                      # a planar subset (flat boxes, NaN/inf SAH costs)
+    pts[200:260] = pts[200]                 # a coincident cluster inside a larger set
+    out["mixed"] = _spheres_scene(pts, radius=0.01)
+    out["geometric"] = _spheres_scene([(2.0 ** (-k), 0, 0) for k in range(40)], radius=1e-4)
+    out["box"] = scenes.box_scene()
+    out["showcase_small"] = scenes.showcase_scene(n_spheres=60, grid=(20, 25))
+    out["ico5"] = scenes.icosphere_scene(level=5)
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(_degenerate_scenes()))
+def test_library_builder_matches_oracle(name, host_ctx):
+    from gammaray_amd.api import Stage
+    spec = _degenerate_scenes()[name]
+    a = OracleScene(spec).export_bvh()
+    b = Stage(spec, ctx=host_ctx).export_bvh()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_abi_exports_every_declared_symbol():
+    import os
+    from gammaray_amd import _lib
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "gammaray.h")).read()
+    declared = set(re.findall(r"\b(gr_[a-z_0-9]+)\s*\(", hdr))
+    L = _lib.lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    assert declared <= set(_lib.SIGNATURES), sorted(declared - set(_lib.SIGNATURES))
+
+
+def test_host_only_context_rejects_tracing(host_ctx):
+    from gammaray_amd.api import Stage
+    from gammaray_amd._lib import GrError
+    st = Stage(scenes.box_scene(), ctx=host_ctx)
+    with pytest.raises(GrError):
+        st.intersect_ptr(0, 0, 0)
+
+
+def test_invalid_mesh_rejected(host_ctx):
+    from gammaray_amd.api import Stage
+    from gammaray_amd._lib import GrError
+    s = scenes.SceneSpec()
+    s.add_mesh(np.zeros((3, 3), np.float32), [[0, 1, 5]], scenes.disney())
+    with pytest.raises(GrError):
+        Stage(s, ctx=host_ctx)
