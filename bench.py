@@ -83,7 +83,8 @@ def main():
     mode = GR_OCCLUSION_EXACT if args.occlusion == "exact" else GR_OCCLUSION_ANYHIT
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     t0 = time.time()
     spec = scenes.icosphere_scene(level=args.level, seed=2)
     log("icosphere L%d: %d triangles (%.1f s)" % (args.level, spec.num_components, time.time() - t0))

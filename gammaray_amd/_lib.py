@@ -36,6 +36,7 @@ SIGNATURES = {
     "gr_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "gr_ctx_destroy": (None, [_vp]),
     "gr_ctx_set_stream": (_i, [_vp, _vp]),
+    "gr_ctx_use_own_stream": (_i, [_vp]),
     "gr_ctx_stream": (_vp, [_vp]),
     "gr_ctx_synchronize": (_i, [_vp]),
     "gr_ctx_last_error": (ctypes.c_char_p, [_vp]),

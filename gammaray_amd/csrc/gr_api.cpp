@@ -50,6 +50,8 @@ void free_device(gr_scene* s) {
     (void)hipFree(s->d_comps);
     (void)hipFree(s->d_comp_ids);
     (void)hipFree(s->d_overflow);
+    (void)hipFree(s->d_next_ray);
+    s->d_next_ray = nullptr;
     s->d_nodes = nullptr;
     s->d_comps = nullptr;
     s->d_comp_ids = nullptr;
@@ -76,7 +78,9 @@ int gr_ctx_create(int device, gr_ctx** out) {
     if (!c) return GR_ERR_NOMEM;
     c->device = device;
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    // A blocking stream: ordered with work on the legacy default stream, where
+    // a caller's uploads often run.
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);
     if (e != hipSuccess) {
         delete c;
         return GR_ERR_HIP;
@@ -102,7 +106,13 @@ void gr_ctx_destroy(gr_ctx* ctx) {
 
 int gr_ctx_set_stream(gr_ctx* ctx, void* stream) {
     if (!ctx) return GR_ERR_INVALID;
-    ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own_stream;
+    ctx->stream = reinterpret_cast<hipStream_t>(stream);
+    return GR_OK;
+}
+
+int gr_ctx_use_own_stream(gr_ctx* ctx) {
+    if (!ctx) return GR_ERR_INVALID;
+    ctx->stream = ctx->own_stream;
     return GR_OK;
 }
 
@@ -227,6 +237,7 @@ int gr_scene_commit(gr_scene* s) {
     if (e == hipSuccess) e = upload(&s->d_comp_ids, lay.comp_ids, ctx->stream);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_overflow), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemsetAsync(s->d_overflow, 0, sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_next_ray), sizeof(uint32_t));
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         free_device(s);
@@ -243,6 +254,7 @@ int gr_scene_commit(gr_scene* s) {
     }
     v.pad = slab_pad();
     v.overflow = s->d_overflow;
+    v.next_ray = s->d_next_ray;
     s->committed = true;
     return GR_OK;
 }

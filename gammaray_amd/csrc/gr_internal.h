@@ -83,6 +83,7 @@ struct SceneView {
     float root_lo[3], root_hi[3];
     float pad;  // 1 + 2 gamma(3)
     unsigned long long* overflow;
+    uint32_t* next_ray;  // work counter of the persistent traversal kernels
 };
 
 struct GpuLayout {
@@ -121,5 +122,6 @@ struct gr_scene {
     gr::GpuComp* d_comps = nullptr;
     uint2* d_comp_ids = nullptr;
     unsigned long long* d_overflow = nullptr;
+    uint32_t* d_next_ray = nullptr;
     gr::SceneView view{};
 };

@@ -125,58 +125,163 @@ __device__ __forceinline__ GpuNode load_node(const GpuNode* nodes, uint32_t i) {
     return n;
 }
 
-struct TraceOut {
-    float t;
-    uint32_t slot;      // closest: winning slot; occlusion: != kNone when occluded
-    bool overflow;
+// ---------------------------------------------------------------------------
+// Persistent traversal.  One 64-lane wave per block; every lane owns one ray
+// at a time.  Lanes whose ray finishes go idle and the wave refills them from
+// a global ray counter once enough are idle, so long and short rays share a
+// wave without leaving lanes empty.  The loop is while-while: all lanes step
+// through interior nodes until each is at a leaf (or done), then leaves are
+// tested together.  The traversal stack keeps its top kLdsStack entries per
+// lane in LDS and spills older entries to private memory, up to the
+// reference's 64 entries.
+constexpr int kWave = 64;
+constexpr int kLdsStack = 16;
+constexpr int kSpill = kStackSize - kLdsStack;
+constexpr int kRefill = 16;  // refill once this many lanes are idle
+
+template <int MODE>
+struct Output {
+    using type = uint32_t;  // visibility flag
+};
+template <>
+struct Output<kClosest> {
+    using type = uint4;  // gr_hit
 };
 
-// One traversal.  MODE kClosest: closest hit.  kOcclusionExact: reference
-// order, stop at the first accepted hit closer than `limit`.  kOcclusionAnyHit:
-// start with closest = limit, which also prunes boxes beyond the target.
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t lane = threadIdx.x;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
 template <int MODE, bool COUNT>
-__device__ __forceinline__ TraceOut trace(const SceneView& sv, const RayFrame& r, float limit,
-                                          uint32_t* cnt) {
-    TraceOut out{0.0f, kNone, false};
-    float closest = MODE == kOcclusionAnyHit ? limit : FLT_MAX;
-    float tmin;
-    if (COUNT) cnt[0] += 1;
-    if (!(slab(sv.root_lo[0], sv.root_lo[1], sv.root_lo[2], sv.root_hi[0], sv.root_hi[1],
-               sv.root_hi[2], r, sv.pad, &tmin) &&
-          tmin < closest))
-        return out;
-    uint32_t stack_ref[kStackSize];
-    float stack_t[kStackSize];
-    int sp = 0;
-    uint32_t cur = sv.root_ref;
+__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __restrict__ in,
+                                                  uint32_t n, typename Output<MODE>::type* __restrict__ out,
+                                                  uint32_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ next_ray) {
+    __shared__ uint32_t s_ref[kLdsStack][kWave];
+    __shared__ float s_t[kLdsStack][kWave];
+    uint32_t spill_ref[kSpill];
+    float spill_t[kSpill];
+    const uint32_t lane = threadIdx.x;
+
+    bool active = false;
+    bool exhausted = false;
+    uint32_t ray = 0, cur = 0, best = kNone;
+    float closest = FLT_MAX, limit = 0.0f;
+    int sp = 0, base = 0;
+    uint32_t c_box = 0, c_tri = 0, c_sph = 0;
+    RayFrame r;
+    r.o = r.d = r.inv = mk(0.0f, 0.0f, 0.0f);
+    r.nx = r.ny = r.nz = false;
+
+    auto finish = [&](bool occluded, bool overflow) {
+        if (MODE == kClosest) {
+            uint4 h = make_uint4(kNone, kNone, 0u, kNone);
+            if (overflow) {
+                h.z = __float_as_uint(-1.0f);
+            } else if (best != kNone) {
+                const uint2 id = sv.comp_ids[best];
+                h = make_uint4(id.x, id.y, __float_as_uint(closest), best);
+            }
+            reinterpret_cast<uint4*>(out)[ray] = h;
+        } else {
+            reinterpret_cast<uint32_t*>(out)[ray] = (occluded || overflow) ? 0u : 1u;
+        }
+        if (overflow) atomicAdd(sv.overflow, 1ull);
+        if (COUNT) {
+            counts[3 * (size_t)ray + 0] = c_box;
+            counts[3 * (size_t)ray + 1] = c_tri;
+            counts[3 * (size_t)ray + 2] = c_sph;
+        }
+        active = false;
+    };
+
+    // Pops entries until one whose entry distance is still below `closest`.
+    auto pop = [&]() -> bool {
+        while (sp > 0) {
+            --sp;
+            uint32_t ref;
+            float t;
+            if (sp < base) {
+                ref = spill_ref[sp];
+                t = spill_t[sp];
+                base = sp;
+            } else {
+                ref = s_ref[sp & (kLdsStack - 1)][lane];
+                t = s_t[sp & (kLdsStack - 1)][lane];
+            }
+            if (t < closest) {
+                cur = ref;
+                return true;
+            }
+        }
+        return false;
+    };
+
     for (;;) {
-        if (cur & kLeafBit) {
-            uint32_t slot = cur & ~kLeafBit;
-            for (;;) {
-                const GpuComp g = load_comp(sv.comps, slot);
-                const uint32_t tag = __float_as_uint(g.q2.y);
-                float dist;
-                if (tag & kTagSphere) {
-                    if (COUNT) cnt[2] += 1;
-                    dist = hit_sphere(g, r);
-                } else {
-                    if (COUNT) cnt[1] += 1;
-                    dist = hit_triangle(g, r);
-                }
-                if (dist != 0.0f && dist < closest) {
-                    closest = dist;
-                    out.slot = slot;
-                    if (MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit)) {
-                        out.t = dist;
-                        return out;
+        // ---- refill idle lanes (wave-uniform control flow)
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            const int nidle = __popcll(idle);
+            if (nidle >= kRefill) {
+                uint32_t first = 0;
+                if (lane == 0) first = atomicAdd(next_ray, (uint32_t)nidle);
+                first = __shfl(first, 0);
+                if (first + (uint32_t)nidle >= n) exhausted = true;
+                if (!active) {
+                    const uint32_t idx = first + (uint32_t)__popcll(idle & lanes_below());
+                    if (idx < n) {
+                        ray = idx;
+                        const float4 a = in[2 * (size_t)idx], b = in[2 * (size_t)idx + 1];
+                        best = kNone;
+                        sp = 0;
+                        base = 0;
+                        c_box = c_tri = c_sph = 0;
+                        bool start = true;
+                        if (MODE == kClosest) {
+                            r = make_frame(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+                            closest = FLT_MAX;
+                        } else {
+                            // Bvh::visibility, src/geom/bvh.rs:460-477.
+                            const vec3 s0 = mk(a.x, a.y, a.z), tg = mk(b.x, b.y, b.z);
+                            const bool close = fabsf(tg.x - s0.x) < 1e-3f && fabsf(tg.y - s0.y) < 1e-3f &&
+                                               fabsf(tg.z - s0.z) < 1e-3f;
+                            if (close) {
+                                start = false;
+                                finish(true, false);
+                            } else {
+                                const vec3 dir = normalized(tg - s0);
+                                const vec3 org = s0 + dir * kRayPush;  // Ray::nudge, ray.rs:44-46
+                                limit = length(tg - org) - 1e-3f;
+                                r = make_frame(org, dir);
+                                closest = MODE == kOcclusionAnyHit ? limit : FLT_MAX;
+                            }
+                        }
+                        if (start) {
+                            float tmin;
+                            if (COUNT) c_box = 1;
+                            if (slab(sv.root_lo[0], sv.root_lo[1], sv.root_lo[2], sv.root_hi[0],
+                                     sv.root_hi[1], sv.root_hi[2], r, sv.pad, &tmin) &&
+                                tmin < closest) {
+                                cur = sv.root_ref;
+                                active = true;
+                            } else {
+                                finish(false, false);
+                            }
+                        }
                     }
                 }
-                if (tag & kTagLast) break;
-                ++slot;
             }
-        } else {
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // ---- interior nodes until every lane sits on a leaf or is done
+        while (active && !(cur & kLeafBit)) {
             const GpuNode nd = load_node(sv.nodes, cur);
-            if (COUNT) cnt[0] += 2;
+            if (COUNT) c_box += 2;
             float t0 = 0.0f, t1 = 0.0f;
             const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
             const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
@@ -190,119 +295,98 @@ __device__ __forceinline__ TraceOut trace(const SceneView& sv, const RayFrame& r
             const float far_t = flip ? t0 : t1;
             if (far_ok) {
                 if (sp == kStackSize) {
-                    out.overflow = true;
-                    out.slot = kNone;
-                    return out;
+                    finish(false, true);
+                    break;
                 }
-                stack_ref[sp] = far_ref;
-                stack_t[sp] = far_t;
+                if (sp - base == kLdsStack) {
+                    spill_ref[base] = s_ref[base & (kLdsStack - 1)][lane];
+                    spill_t[base] = s_t[base & (kLdsStack - 1)][lane];
+                    ++base;
+                }
+                s_ref[sp & (kLdsStack - 1)][lane] = far_ref;
+                s_t[sp & (kLdsStack - 1)][lane] = far_t;
                 ++sp;
             }
             if (near_ok && near_t < closest) {
                 cur = near_ref;
-                continue;
+            } else if (!pop()) {
+                finish(false, false);
             }
         }
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            if (stack_t[sp] < closest) {
-                cur = stack_ref[sp];
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
-    }
-    out.t = closest;
-    if (MODE != kClosest) out.slot = kNone;  // traversal ended without proof of occlusion
-    return out;
-}
 
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_intersect(SceneView sv, const gr_ray* __restrict__ rays,
-                                                   uint32_t n, gr_hit* __restrict__ hits,
-                                                   uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4* rp = reinterpret_cast<const float4*>(rays) + 2 * (size_t)i;
-    const float4 a = rp[0], b = rp[1];
-    const RayFrame r = make_frame(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
-    uint32_t cnt[3] = {0, 0, 0};
-    const TraceOut t = trace<kClosest, COUNT>(sv, r, 0.0f, cnt);
-    uint4 h = make_uint4(kNone, kNone, 0u, kNone);
-    if (t.overflow) {
-        h.z = __float_as_uint(-1.0f);
-        atomicAdd(sv.overflow, 1ull);
-    } else if (t.slot != kNone) {
-        const uint2 id = sv.comp_ids[t.slot];
-        h = make_uint4(id.x, id.y, __float_as_uint(t.t), t.slot);
-    }
-    reinterpret_cast<uint4*>(hits)[i] = h;
-    if (COUNT) {
-        counts[3 * (size_t)i + 0] = cnt[0];
-        counts[3 * (size_t)i + 1] = cnt[1];
-        counts[3 * (size_t)i + 2] = cnt[2];
+        // ---- leaves
+        if (active) {
+            uint32_t slot = cur & ~kLeafBit;
+            bool done = false;
+            for (;;) {
+                const GpuComp g = load_comp(sv.comps, slot);
+                const uint32_t tag = __float_as_uint(g.q2.y);
+                float dist;
+                if (tag & kTagSphere) {
+                    if (COUNT) c_sph += 1;
+                    dist = hit_sphere(g, r);
+                } else {
+                    if (COUNT) c_tri += 1;
+                    dist = hit_triangle(g, r);
+                }
+                if (dist != 0.0f && dist < closest) {
+                    closest = dist;
+                    best = slot;
+                    if (MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit)) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (tag & kTagLast) break;
+                ++slot;
+            }
+            if (done) finish(true, false);
+            else if (!pop()) finish(false, false);
+        }
     }
 }
 
-// Bvh::visibility, src/geom/bvh.rs:460-477.
+__global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
+
 template <int MODE, bool COUNT>
-__global__ __launch_bounds__(256) void k_occluded(SceneView sv, const gr_segment* __restrict__ segs,
-                                                  uint32_t n, uint32_t* __restrict__ visible,
-                                                  uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4* sp = reinterpret_cast<const float4*>(segs) + 2 * (size_t)i;
-    const float4 a = sp[0], b = sp[1];
-    const vec3 start = mk(a.x, a.y, a.z), target = mk(b.x, b.y, b.z);
-    uint32_t cnt[3] = {0, 0, 0};
-    uint32_t vis = 0;
-    const bool close = fabsf(target.x - start.x) < 1e-3f && fabsf(target.y - start.y) < 1e-3f &&
-                       fabsf(target.z - start.z) < 1e-3f;
-    if (!close) {
-        const vec3 dir = normalized(target - start);
-        const vec3 origin = start + dir * kRayPush;  // Ray::nudge, src/core/ray.rs:44-46
-        const float limit = length(target - origin) - 1e-3f;
-        const RayFrame r = make_frame(origin, dir);
-        const TraceOut t = trace<MODE, COUNT>(sv, r, limit, cnt);
-        if (t.overflow) atomicAdd(sv.overflow, 1ull);
-        vis = (t.slot == kNone && !t.overflow) ? 1u : 0u;
+hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, void* out,
+                        uint32_t* counts, hipStream_t s) {
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<MODE, COUNT>, kWave, 0);
+        if (e != hipSuccess) return e;
+        grid = cus * (per_cu > 0 ? per_cu : 1);
     }
-    visible[i] = vis;
-    if (COUNT) {
-        counts[3 * (size_t)i + 0] = cnt[0];
-        counts[3 * (size_t)i + 1] = cnt[1];
-        counts[3 * (size_t)i + 2] = cnt[2];
-    }
+    const uint32_t needed = (n + kWave - 1) / kWave;
+    const uint32_t blocks = needed < (uint32_t)grid ? needed : (uint32_t)grid;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, sv.next_ray);
+    hipLaunchKernelGGL((k_trace<MODE, COUNT>), dim3(blocks), dim3(kWave), 0, s, sv,
+                       reinterpret_cast<const float4*>(in), n,
+                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, sv.next_ray);
+    return hipGetLastError();
 }
-
-inline dim3 grid_for(uint32_t n) { return dim3((n + 255) / 256); }
 
 }  // namespace
 
 hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n, gr_hit* hits,
                             uint32_t* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (counts)
-        hipLaunchKernelGGL(k_intersect<true>, grid_for(n), dim3(256), 0, s, sv, rays, n, hits, counts);
-    else
-        hipLaunchKernelGGL(k_intersect<false>, grid_for(n), dim3(256), 0, s, sv, rays, n, hits, counts);
-    return hipGetLastError();
+    return counts ? launch_trace<kClosest, true>(sv, rays, n, hits, counts, s)
+                  : launch_trace<kClosest, false>(sv, rays, n, hits, counts, s);
 }
 
 hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
                            uint32_t* visible, int mode, uint32_t* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const dim3 g = grid_for(n), b(256);
-    if (mode == GR_OCCLUSION_ANYHIT) {
-        if (counts) hipLaunchKernelGGL((k_occluded<kOcclusionAnyHit, true>), g, b, 0, s, sv, segs, n, visible, counts);
-        else hipLaunchKernelGGL((k_occluded<kOcclusionAnyHit, false>), g, b, 0, s, sv, segs, n, visible, counts);
-    } else {
-        if (counts) hipLaunchKernelGGL((k_occluded<kOcclusionExact, true>), g, b, 0, s, sv, segs, n, visible, counts);
-        else hipLaunchKernelGGL((k_occluded<kOcclusionExact, false>), g, b, 0, s, sv, segs, n, visible, counts);
-    }
-    return hipGetLastError();
+    if (mode == GR_OCCLUSION_ANYHIT)
+        return counts ? launch_trace<kOcclusionAnyHit, true>(sv, segs, n, visible, counts, s)
+                      : launch_trace<kOcclusionAnyHit, false>(sv, segs, n, visible, counts, s);
+    return counts ? launch_trace<kOcclusionExact, true>(sv, segs, n, visible, counts, s)
+                  : launch_trace<kOcclusionExact, false>(sv, segs, n, visible, counts, s);
 }
 
 }  // namespace gr

@@ -53,9 +53,11 @@ typedef struct { float sx, sy, sz, pad0, tx, ty, tz, pad1; } gr_segment;
 #define GR_HOST_ONLY (-1)
 int gr_ctx_create(int device, gr_ctx **out);
 void gr_ctx_destroy(gr_ctx *ctx);
-/* Run subsequent work on an existing HIP stream (hipStream_t); NULL restores
- * the context's own stream. */
+/* Run subsequent work on an existing HIP stream (hipStream_t; NULL is the
+ * legacy default stream).  gr_ctx_use_own_stream() returns to the context's
+ * own (blocking) stream. */
 int gr_ctx_set_stream(gr_ctx *ctx, void *hip_stream);
+int gr_ctx_use_own_stream(gr_ctx *ctx);
 void *gr_ctx_stream(gr_ctx *ctx);
 int gr_ctx_synchronize(gr_ctx *ctx);
 const char *gr_ctx_last_error(gr_ctx *ctx);
