@@ -31,8 +31,30 @@ class SceneInfo(ctypes.Structure):
                 ("build_seconds", ctypes.c_double)]
 
 
+class Camera(ctypes.Structure):
+    _fields_ = [("focal_length", ctypes.c_float), ("horizontal_aperture", ctypes.c_float),
+                ("vertical_aperture", ctypes.c_float), ("f_stop", ctypes.c_float),
+                ("rotate", ctypes.c_float * 4), ("translate", ctypes.c_float * 3)]
+
+
+class TraceParams(ctypes.Structure):
+    _fields_ = [("integrator", _u32), ("spp", _u32), ("max_depth", _u32), ("light_depth", _u32),
+                ("seed", ctypes.c_uint64), ("sample_offset", ctypes.c_uint64), ("batch", _u32),
+                ("reserved", _u32), ("d_sample_radiance", _vp)]
+
+
+GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT = 0, 1, 2
+
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.
 SIGNATURES = {
+    "gr_film_create": (_i, [_vp, _u32, _u32, ctypes.POINTER(_vp)]),
+    "gr_film_destroy": (None, [_vp]),
+    "gr_film_clear": (_i, [_vp]),
+    "gr_film_readback": (_i, [_vp, _f32p]),
+    "gr_film_upload": (_i, [_vp, _f32p]),
+    "gr_trace": (_i, [_vp, ctypes.POINTER(Camera), ctypes.POINTER(TraceParams), _vp]),
+    "gr_film_write_exr": (_i, [_vp, ctypes.c_char_p]),
+    "gr_material_query": (_i, [_vp, _u32, _vp, _u32, _vp]),
     "gr_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "gr_ctx_destroy": (None, [_vp]),
     "gr_ctx_set_stream": (_i, [_vp, _vp]),

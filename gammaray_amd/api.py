@@ -160,3 +160,99 @@ def hits_to_numpy(hits):
     h = hits.cpu().numpy().view(np.uint32)
     return dict(prim=h[:, 0].copy(), comp=h[:, 1].copy(), dist=h[:, 2].view(np.float32).copy(),
                 slot=h[:, 3].copy())
+
+
+# ---------------------------------------------------------------------------
+# Rendering: Film (src/render/film.rs) and Stage::trace (src/render/stage.rs)
+# ---------------------------------------------------------------------------
+INTEGRATORS = {"pt": _lib.GR_INTEGRATOR_PT, "pt_nee": _lib.GR_INTEGRATOR_PT_NEE,
+               "bdpt": _lib.GR_INTEGRATOR_BDPT}
+
+
+def make_camera(spec):
+    """gr_camera from a scenes.CameraSpec (Camera::new arguments)."""
+    c = _lib.Camera()
+    c.focal_length = spec.focal_length
+    c.horizontal_aperture = spec.horizontal_aperture
+    c.vertical_aperture = spec.vertical_aperture
+    c.f_stop = spec.f_stop
+    for k in range(4):
+        c.rotate[k] = spec.rotate[k]
+    for k in range(3):
+        c.translate[k] = spec.translate[k]
+    return c
+
+
+class Film:
+    """Accumulation film on the device: per pixel (accum rgb, weight)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self._h = ctypes.c_void_p()
+        check(lib().gr_film_create(ctx.handle, width, height, ctypes.byref(self._h)), ctx.handle,
+              "film_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def clear(self):
+        check(lib().gr_film_clear(self._h), self.ctx.handle, "film_clear")
+
+    def readback(self):
+        out = np.zeros((self.height * self.width, 4), np.float32)
+        check(lib().gr_film_readback(self._h, _p(out, _f32p)), self.ctx.handle, "film_readback")
+        return out
+
+    def upload(self, rgbw):
+        a = np.ascontiguousarray(rgbw, np.float32)
+        assert a.size == self.width * self.height * 4
+        check(lib().gr_film_upload(self._h, _p(a, _f32p)), self.ctx.handle, "film_upload")
+
+    def image(self):
+        px = self.readback()
+        return (px[:, :3] / px[:, 3:4]).reshape(self.height, self.width, 3)
+
+    def write_exr(self, path):
+        check(lib().gr_film_write_exr(self._h, path.encode()), self.ctx.handle, "write_exr")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gr_film_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trace(stage, film, integrator="pt", spp=1, max_depth=0, light_depth=0, seed=1, sample_offset=0,
+          batch=0, sample_radiance_ptr=None, camera=None):
+    """spp iterations of Stage::trace(&camera, &integrator, &mut film)."""
+    p = _lib.TraceParams()
+    p.integrator = INTEGRATORS[integrator] if isinstance(integrator, str) else integrator
+    p.spp = spp
+    p.max_depth = max_depth
+    p.light_depth = light_depth
+    p.seed = seed
+    p.sample_offset = sample_offset
+    p.batch = batch
+    p.d_sample_radiance = sample_radiance_ptr
+    cam = make_camera(camera if camera is not None else stage.spec.camera)
+    check(lib().gr_trace(stage.handle, ctypes.byref(cam), ctypes.byref(p), film.handle),
+          stage.ctx.handle, "trace")
+
+
+def material_query(stage, prim, items):
+    """Device evaluation of f_world / pdf_world / light_world / sample_world
+    (layout documented at gr_material_query)."""
+    import torch
+    x = torch.from_numpy(np.ascontiguousarray(items, np.float32).reshape(-1, 24)).cuda()
+    out = torch.empty((x.shape[0], 20), dtype=torch.float32, device=x.device)
+    check(lib().gr_material_query(stage.handle, prim, ctypes.c_void_p(x.data_ptr()), x.shape[0],
+                                  ctypes.c_void_p(out.data_ptr())), stage.ctx.handle, "material_query")
+    stage.ctx.synchronize()
+    return out.cpu().numpy()

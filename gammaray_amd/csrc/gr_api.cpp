@@ -51,6 +51,9 @@ void free_device(gr_scene* s) {
     (void)hipFree(s->d_comp_ids);
     (void)hipFree(s->d_overflow);
     (void)hipFree(s->d_next_ray);
+    (void)hipFree(s->d_shade_blob);
+    s->d_shade_blob = nullptr;
+    s->shade = gr::ShadeView{};
     s->d_next_ray = nullptr;
     s->d_nodes = nullptr;
     s->d_comps = nullptr;
@@ -227,11 +230,13 @@ int gr_scene_commit(gr_scene* s) {
     for (const Prim& p : s->prims)
         if ((int)p.material[0] == 1) s->num_lights++;
     if (ctx->device == GR_HOST_ONLY) {
+        (void)upload_shading(s, nullptr);  // builds the material table; uploads nothing
         s->committed = true;
         return GR_OK;
     }
 
     hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = upload_shading(s, nullptr);
     if (e == hipSuccess) e = upload(&s->d_nodes, lay.nodes, ctx->stream);
     if (e == hipSuccess) e = upload(&s->d_comps, lay.comps, ctx->stream);
     if (e == hipSuccess) e = upload(&s->d_comp_ids, lay.comp_ids, ctx->stream);
@@ -317,6 +322,96 @@ int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_v
     if (n && (!d_segs || !d_visible)) return fail(s->ctx, GR_ERR_INVALID, "null segment/output buffer%s");
     hipError_t e = launch_occluded(s->view, d_segs, n, d_visible, mode, d_counts, s->ctx->stream);
     return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "occlusion launch");
+}
+
+int gr_film_create(gr_ctx* ctx, uint32_t width, uint32_t height, gr_film** out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    *out = nullptr;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    if (width == 0 || height == 0 || (uint64_t)width * height > 0x7fffffffull)
+        return fail(ctx, GR_ERR_INVALID, "bad film size%s");
+    gr_film* f = new (std::nothrow) gr_film();
+    if (!f) return fail(ctx, GR_ERR_NOMEM, "out of host memory%s");
+    f->ctx = ctx;
+    f->width = width;
+    f->height = height;
+    const size_t bytes = (size_t)width * height * sizeof(float4);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&f->d_pixels), bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(f->d_pixels, 0, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(f->d_pixels);
+        delete f;
+        return hip_fail(ctx, e, "film allocation");
+    }
+    *out = f;
+    return GR_OK;
+}
+
+void gr_film_destroy(gr_film* f) {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipFree(f->d_pixels);
+    release_render_buffers(f->rb);
+    delete f;
+}
+
+int gr_film_clear(gr_film* f) {
+    if (!f) return GR_ERR_INVALID;
+    hipError_t e = hipMemsetAsync(f->d_pixels, 0, (size_t)f->width * f->height * sizeof(float4), f->ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(f->ctx, e, "film clear");
+}
+
+int gr_film_readback(gr_film* f, float* rgbw) {
+    if (!f || !rgbw) return GR_ERR_INVALID;
+    hipError_t e = hipMemcpyAsync(rgbw, f->d_pixels, (size_t)f->width * f->height * sizeof(float4),
+                                  hipMemcpyDeviceToHost, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(f->ctx, e, "film readback");
+}
+
+int gr_film_upload(gr_film* f, const float* rgbw) {
+    if (!f || !rgbw) return GR_ERR_INVALID;
+    hipError_t e = hipMemcpyAsync(f->d_pixels, rgbw, (size_t)f->width * f->height * sizeof(float4),
+                                  hipMemcpyHostToDevice, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    return e == hipSuccess ? GR_OK : hip_fail(f->ctx, e, "film upload");
+}
+
+int gr_trace(gr_scene* s, const gr_camera* cam, const gr_trace_params* prm, gr_film* film) {
+    if (!s || !cam || !prm || !film) return GR_ERR_INVALID;
+    gr_ctx* ctx = s->ctx;
+    if (!s->committed) return fail(ctx, GR_ERR_STATE, "scene not committed%s");
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    if (film->ctx != ctx) return fail(ctx, GR_ERR_INVALID, "film belongs to another context%s");
+    if (prm->integrator > GR_INTEGRATOR_BDPT) return fail(ctx, GR_ERR_INVALID, "unknown integrator%s");
+    if ((prm->integrator != GR_INTEGRATOR_PT) && s->shade.num_lights == 0)
+        return fail(ctx, GR_ERR_INVALID, "integrator samples lights but the scene has none%s");
+    if (!(cam->focal_length > 0.0f)) return fail(ctx, GR_ERR_INVALID, "camera focal length must be positive%s");
+    if (prm->spp == 0) return GR_OK;
+    hipError_t e = render_trace(s, *cam, *prm, film);
+    if (e == hipErrorNotSupported) return fail(ctx, GR_ERR_INVALID, "integrator not available%s");
+    return e == hipSuccess ? GR_OK : hip_fail(ctx, e, "trace");
+}
+
+int gr_film_write_exr(gr_film* f, const char* path) {
+    if (!f || !path) return GR_ERR_INVALID;
+    std::vector<float> px((size_t)f->width * f->height * 4);
+    int r = gr_film_readback(f, px.data());
+    if (r != GR_OK) return r;
+    if (!write_exr(px.data(), f->width, f->height, path))
+        return fail(f->ctx, GR_ERR_INVALID, "cannot write %s", path);
+    return GR_OK;
+}
+
+int gr_material_query(gr_scene* s, uint32_t prim, const float* d_in, uint32_t n, float* d_out) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    if (prim >= s->prims.size()) return fail(s->ctx, GR_ERR_INVALID, "prim index out of range%s");
+    if (n && (!d_in || !d_out)) return fail(s->ctx, GR_ERR_INVALID, "null buffer%s");
+    hipError_t e = material_query(s, prim, d_in, n, d_out);
+    return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "material query");
 }
 
 int gr_scene_take_overflow_count(gr_scene* s, uint64_t* out) {

@@ -100,6 +100,35 @@ hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n,
                             uint32_t* counts, hipStream_t s);
 hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
                            uint32_t* visible, int mode, uint32_t* counts, hipStream_t s);
+// Queue forms: the count is read on the device from *n_ptr (<= n_max).
+hipError_t launch_intersect_queue(const SceneView& sv, const gr_ray* rays, uint32_t n_max,
+                                  const uint32_t* n_ptr, gr_hit* hits, hipStream_t s);
+hipError_t launch_occluded_queue(const SceneView& sv, const gr_segment* segs, uint32_t n_max,
+                                 const uint32_t* n_ptr, uint32_t* visible, hipStream_t s);
+
+// ---- shading data ------------------------------------------------------------
+// Per prim: where its mesh data lives, or its sphere, plus its area.
+struct GpuPrim {
+    uint32_t type, tri_off, vert_off, norm_off;
+    uint32_t uv_off, ntris, cdf_off, pad0;
+    float center[3], radius;
+    float area, pad1, pad2, pad3;
+};
+static_assert(sizeof(GpuPrim) == 64, "prim layout");
+
+struct GpuMaterial;
+
+struct ShadeView {
+    const GpuMaterial* mats;
+    const GpuPrim* prims;
+    const float4* verts;
+    const float4* normals;
+    const float2* uvs;
+    const uint32_t* tris;  // 9 per triangle, sentinels resolved
+    const float* cdf;      // per mesh: area CDF over its triangles
+    const uint32_t* lights;
+    uint32_t num_lights;
+};
 
 }  // namespace gr
 
@@ -124,4 +153,44 @@ struct gr_scene {
     unsigned long long* d_overflow = nullptr;
     uint32_t* d_next_ray = nullptr;
     gr::SceneView view{};
+    // shading data (gr_render.hip)
+    void* d_shade_blob = nullptr;  // one allocation holding every shading array
+    gr::ShadeView shade{};
 };
+
+// Per-film wavefront buffers, grown on demand by gr_trace.
+struct RenderBuffers {
+    size_t capacity = 0;  // paths
+    void* blob = nullptr;
+    float4 *rays[2] = {nullptr, nullptr};   // 2 float4 per ray (gr_ray)
+    uint32_t *queue[2] = {nullptr, nullptr};
+    uint4* hits = nullptr;
+    float4 *T = nullptr, *L = nullptr, *aux = nullptr;
+    float2* st = nullptr;
+    float4* seg = nullptr;      // shadow segments, 2 float4 each
+    float4* contrib = nullptr;  // shadow contributions
+    uint32_t* seg_path = nullptr;
+    uint32_t* vis = nullptr;
+    uint32_t* counts = nullptr;  // [0..1] queue lengths, [2] shadow queue length
+    // BDPT vertex storage
+    size_t bdpt_capacity = 0;
+    void* bdpt_blob = nullptr;
+};
+
+struct gr_film {
+    gr_ctx* ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    float4* d_pixels = nullptr;  // (accum r, g, b, weight)
+    RenderBuffers rb;
+};
+
+namespace gr {
+// gr_render.hip
+hipError_t upload_shading(gr_scene* s, std::vector<GpuPrim>* prims_out);
+hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_params& prm, gr_film* film);
+hipError_t material_query(gr_scene* sc, uint32_t prim, const float* in, uint32_t n, float* out);
+void release_render_buffers(RenderBuffers& rb);
+// gr_exr.cpp
+bool write_exr(const float* rgbw, uint32_t width, uint32_t height, const char* path);
+void encode_exr(const float* rgbw, uint32_t width, uint32_t height, std::vector<uint8_t>* out);
+}  // namespace gr

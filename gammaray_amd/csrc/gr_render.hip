@@ -1,0 +1,809 @@
+// gr_render.hip -- wavefront path tracing on gfx950: camera rays, shading and
+// bounce loop, next-event estimation, film accumulation.
+//
+// One render pass is the reference's Stage::trace (src/render/stage.rs:23-36)
+// for `spp` iterations: per iteration every pixel gets one jittered sample
+// (src/render/film.rs:55-76), the integrator runs per sample, and samples
+// splat into the film with the Mitchell filter (film.rs:78-114).  Here the
+// samples of several iterations are traced together as one wavefront:
+//   k_camera            sample points + camera rays for every path
+//   per bounce:         k_trace (closest hit) over the live-path queue,
+//                       k_shade_* (material sampling, emission, Russian
+//                       roulette, next-event shadow rays) compacting survivors
+//                       into the next queue with one atomic per wave,
+//                       [NEE] k_trace (visibility) + k_shadow_accum
+//   k_film_gather       each pixel gathers its 9x9 neighbourhood of samples in
+//                       the order the reference's serial splat visits them, so
+//                       the film sums are the reference's, operation for
+//                       operation, for equal sample radiances.
+// Random numbers are addressed by (path, stream, bounce, slot) through
+// Philox4x32-10, identically to the CPU oracle.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gr_internal.h"
+#include "gr_shading.h"
+
+namespace gr {
+
+// ---------------------------------------------------------------------------
+// Host: material table and camera
+// ---------------------------------------------------------------------------
+namespace {
+
+void set3(float* v, vec3 c) {
+    v[0] = c.x;
+    v[1] = c.y;
+    v[2] = c.z;
+}
+vec3 lerp3h(vec3 a, vec3 b, float t) {
+    return mk(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z));
+}
+vec3 tint(vec3 c) {  // Vec::tint, src/core/vector.rs:218-234
+    const float l = 0.21f * c.x + 0.71f * c.y + 0.08f * c.z;
+    return l > 0.0f ? c / l : mk(1.0f, 1.0f, 1.0f);
+}
+void ggx_params(GpuLobe& l, float roughness, float anisotropic) {  // util.rs:128-133
+    const float aspect = sqrtf(1.0f - anisotropic * 0.9f);
+    l.v[0] = fmaxf(0.001f, roughness * roughness / aspect);
+    l.v[1] = fmaxf(0.001f, roughness * roughness * aspect);
+}
+
+}  // namespace
+
+GpuMaterial build_material(const float rec[GR_MAT_FLOATS]) {
+    GpuMaterial m;
+    std::memset(&m, 0, sizeof(m));
+    const int kind = (int)rec[0];
+    const vec3 one = mk(1.0f, 1.0f, 1.0f);
+    if (kind == 1) {  // Material::diffuse_light
+        m.has_light = 1;
+        m.light[0] = rec[1];
+        m.light[1] = rec[2];
+        m.light[2] = rec[3];
+        return m;
+    }
+    if (kind == 2 || kind == 3) {  // Material::diffuse / mirror
+        m.nlobes = 1;
+        m.lobes[0].type = kind == 2 ? kLobePerfectDiffuse : kLobePerfectMirror;
+        return m;
+    }
+    // DisneyMaterialBuilder::build, src/material/material.rs:292-333
+    const vec3 base = mk(rec[1], rec[2], rec[3]);
+    const float roughness = rec[4], anisotropic = rec[5], ior = rec[6], metallic = rec[7];
+    const float specular_trans = rec[8], specular_tint = rec[9], sheen = rec[10], sheen_tint = rec[11];
+    const float clearcoat = rec[12], clearcoat_gloss = rec[13];
+    const float diffuse_weight = (1.0f - metallic) * (1.0f - specular_trans);
+    const float trans_weight = (1.0f - metallic) * specular_trans;
+    if (diffuse_weight > 0.0f) {  // DisneyDiffuseRefl::new, lobes.rs:83-91
+        GpuLobe& l = m.lobes[m.nlobes++];
+        l.type = kLobeDisneyDiffuse;
+        set3(&l.v[0], base * diffuse_weight);
+        set3(&l.v[3], lerp3h(one, tint(base), sheen_tint) * (sheen * diffuse_weight));
+        l.v[6] = roughness;
+    }
+    if (ior > 1.0f) {  // DisneySpecularRefl::new_aniso, lobes.rs:212-225
+        GpuLobe& l = m.lobes[m.nlobes++];
+        l.type = kLobeDisneySpecular;
+        ggx_params(l, roughness, anisotropic);
+        set3(&l.v[2], base);
+        set3(&l.v[5], lerp3h(one, tint(base), specular_tint));
+        l.v[8] = fmaxf(ior, 1.01f);
+        l.v[9] = metallic;
+        set3(&l.v[10], one);
+    }
+    if (clearcoat > 0.0f) {  // DisneyClearcoatRefl::new, lobes.rs:232-242
+        GpuLobe& l = m.lobes[m.nlobes++];
+        l.type = kLobeClearcoat;
+        l.v[0] = 0.1f + clearcoat_gloss * (0.001f - 0.1f);
+        set3(&l.v[1], one * 0.04f);
+        set3(&l.v[4], one * (0.25f * clearcoat));
+    }
+    if (trans_weight > 0.0f) {  // DisneySpecularTrans::new_aniso, lobes.rs:258-268
+        GpuLobe& l = m.lobes[m.nlobes++];
+        l.type = kLobeSpecularTrans;
+        ggx_params(l, roughness, anisotropic);
+        l.v[2] = fmaxf(ior, 1.01f);
+        set3(&l.v[3], mk(sqrtf(base.x), sqrtf(base.y), sqrtf(base.z)) * trans_weight);
+    }
+    return m;
+}
+
+// Camera::new's xform = translation * rotation (src/core/camera.rs:42-61,
+// src/core/matrix.rs:73-121,317-400), row-major.
+void camera_matrix(const gr_camera& c, float m[16]) {
+    float t[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, c.translate[0], c.translate[1], c.translate[2], 1};
+    const float r0 = c.rotate[0], ix = c.rotate[1], iy = c.rotate[2], iz = c.rotate[3];
+    float r[16] = {0};
+    r[0] = 1.0f - 2.0f * (iy * iy + iz * iz);
+    r[1] = 2.0f * (ix * iy + iz * r0);
+    r[2] = 2.0f * (iz * ix - iy * r0);
+    r[4] = 2.0f * (ix * iy - iz * r0);
+    r[5] = 1.0f - 2.0f * (iz * iz + ix * ix);
+    r[6] = 2.0f * (iy * iz + ix * r0);
+    r[8] = 2.0f * (iz * ix + iy * r0);
+    r[9] = 2.0f * (iy * iz - ix * r0);
+    r[10] = 1.0f - 2.0f * (iy * iy + ix * ix);
+    r[15] = 1.0f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            m[4 * i + j] = t[4 * i] * r[j] + t[4 * i + 1] * r[4 + j] + t[4 * i + 2] * r[8 + j] + t[4 * i + 3] * r[12 + j];
+}
+
+// ---------------------------------------------------------------------------
+// Host: shading data upload (called from gr_scene_commit)
+// ---------------------------------------------------------------------------
+hipError_t upload_shading(gr_scene* s, std::vector<GpuPrim>* prims_out) {
+    const auto& P = s->prims;
+    std::vector<GpuMaterial> mats(P.size());
+    std::vector<GpuPrim> prims(P.size());
+    std::vector<float4> verts, normals;
+    std::vector<float2> uvs;
+    std::vector<uint32_t> tris, lights;
+    std::vector<float> cdf;
+    for (size_t i = 0; i < P.size(); ++i) {
+        const Prim& p = P[i];
+        mats[i] = build_material(p.material);
+        if (mats[i].has_light) lights.push_back((uint32_t)i);
+        GpuPrim g;
+        std::memset(&g, 0, sizeof(g));
+        g.type = p.type;
+        if (p.type == kPrimMesh) {
+            g.tri_off = (uint32_t)tris.size();
+            g.vert_off = (uint32_t)verts.size();
+            g.norm_off = (uint32_t)normals.size();
+            g.uv_off = (uint32_t)uvs.size();
+            g.ntris = p.ntris;
+            g.cdf_off = (uint32_t)cdf.size();
+            for (const vec3& v : p.verts) verts.push_back(make_float4(v.x, v.y, v.z, 0.0f));
+            for (const vec3& v : p.normals) normals.push_back(make_float4(v.x, v.y, v.z, 0.0f));
+            for (const vec3& v : p.uvs) uvs.push_back(make_float2(v.x, v.y));
+            tris.insert(tris.end(), p.tris.begin(), p.tris.end());
+            // area CDF, src/geom/mesh.rs:161-170
+            float total = 0.0f;
+            const size_t c0 = cdf.size();
+            for (uint32_t t = 0; t < p.ntris; ++t) {
+                const uint32_t* tr = &p.tris[9 * (size_t)t];
+                const vec3 e1 = p.verts[tr[0]] - p.verts[tr[2]];
+                const vec3 e2 = p.verts[tr[1]] - p.verts[tr[2]];
+                total += 0.5f * length(cross(e1, e2));
+                cdf.push_back(total);
+            }
+            for (size_t k = c0; k < cdf.size(); ++k) cdf[k] = cdf[k] / total;
+            g.area = total;
+        } else {
+            g.center[0] = p.center.x;
+            g.center[1] = p.center.y;
+            g.center[2] = p.center.z;
+            g.radius = p.radius;
+            g.area = 4.0f * kPi * p.radius * p.radius;
+        }
+        prims[i] = g;
+    }
+    if (prims_out) *prims_out = prims;
+    if (s->ctx->device == GR_HOST_ONLY) return hipSuccess;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_mats = off; off = align(off + mats.size() * sizeof(GpuMaterial));
+    const size_t o_prims = off; off = align(off + prims.size() * sizeof(GpuPrim));
+    const size_t o_verts = off; off = align(off + verts.size() * sizeof(float4));
+    const size_t o_norms = off; off = align(off + normals.size() * sizeof(float4));
+    const size_t o_uvs = off; off = align(off + uvs.size() * sizeof(float2));
+    const size_t o_tris = off; off = align(off + tris.size() * sizeof(uint32_t));
+    const size_t o_cdf = off; off = align(off + cdf.size() * sizeof(float));
+    const size_t o_lights = off; off = align(off + lights.size() * sizeof(uint32_t) + 4);
+    char* blob = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&blob), off);
+    if (e != hipSuccess) return e;
+    s->d_shade_blob = blob;
+    auto put = [&](size_t o, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(blob + o, src, bytes, hipMemcpyHostToDevice, s->ctx->stream);
+    };
+    put(o_mats, mats.data(), mats.size() * sizeof(GpuMaterial));
+    put(o_prims, prims.data(), prims.size() * sizeof(GpuPrim));
+    put(o_verts, verts.data(), verts.size() * sizeof(float4));
+    put(o_norms, normals.data(), normals.size() * sizeof(float4));
+    put(o_uvs, uvs.data(), uvs.size() * sizeof(float2));
+    put(o_tris, tris.data(), tris.size() * sizeof(uint32_t));
+    put(o_cdf, cdf.data(), cdf.size() * sizeof(float));
+    put(o_lights, lights.data(), lights.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    ShadeView& v = s->shade;
+    v.mats = reinterpret_cast<const GpuMaterial*>(blob + o_mats);
+    v.prims = reinterpret_cast<const GpuPrim*>(blob + o_prims);
+    v.verts = reinterpret_cast<const float4*>(blob + o_verts);
+    v.normals = reinterpret_cast<const float4*>(blob + o_norms);
+    v.uvs = reinterpret_cast<const float2*>(blob + o_uvs);
+    v.tris = reinterpret_cast<const uint32_t*>(blob + o_tris);
+    v.cdf = reinterpret_cast<const float*>(blob + o_cdf);
+    v.lights = reinterpret_cast<const uint32_t*>(blob + o_lights);
+    v.num_lights = (uint32_t)lights.size();
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+enum { kStreamCamera = 0, kStreamLight = 1, kStreamFilm = 2 };
+// Slots within one bounce's draws: block 0 = slots 0..3, block 1 = 4..7,
+// block 2 = 8..11.
+enum { kSlotLobe = 0, kSlotU1 = 1, kSlotU2 = 2, kSlotRR = 3, kSlotLight = 4, kSlotLightTri = 5,
+       kSlotLightU1 = 6, kSlotLightU2 = 7, kSlotLightDir1 = 8, kSlotLightDir2 = 9 };
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                        uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// Four consecutive draws (slots 4*block .. 4*block+3) of one bounce.
+__device__ __forceinline__ uint4 draws(uint64_t seed, uint64_t path, uint32_t stream, uint32_t bounce,
+                                       uint32_t block) {
+    return philox((uint32_t)path, (uint32_t)(path >> 32), bounce, (stream << 16) | block, (uint32_t)seed,
+                  (uint32_t)(seed >> 32));
+}
+
+__device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+__device__ __forceinline__ vec3 ld3(const float4 v) { return mk(v.x, v.y, v.z); }
+
+// Mesh::compute_surface_props (src/geom/mesh.rs:190-240).
+__device__ Frame mesh_frame(const ShadeView& sh, const GpuPrim& p, uint32_t comp, float u, float v, float w) {
+    const uint32_t* t = sh.tris + p.tri_off + 9 * (size_t)comp;
+    const vec3 a = ld3(sh.verts[p.vert_off + t[0]]), b = ld3(sh.verts[p.vert_off + t[1]]),
+               c = ld3(sh.verts[p.vert_off + t[2]]);
+    const vec3 e1 = a - c, e2 = b - c;
+    const float2 at = sh.uvs[p.uv_off + t[6]], bt = sh.uvs[p.uv_off + t[7]], ct = sh.uvs[p.uv_off + t[8]];
+    const vec3 uv1 = mk(at.x - ct.x, at.y - ct.y, 0.0f - 0.0f);
+    const vec3 uv2 = mk(bt.x - ct.x, bt.y - ct.y, 0.0f - 0.0f);
+    Frame f;
+    f.gn = normalized(cross(e1, e2));
+    if (t[3] == kNone) {
+        f.n = f.gn;
+    } else {
+        const vec3 an = ld3(sh.normals[p.norm_off + t[3]]), bn = ld3(sh.normals[p.norm_off + t[4]]),
+                   cn = ld3(sh.normals[p.norm_off + t[5]]);
+        f.n = normalized((an * u + bn * v) + cn * w);
+    }
+    const float det = uv1.x * uv2.y + uv1.y * uv2.x;
+    if (det == 0.0f) {
+        dev::coord_system(f.n, &f.t, &f.b);
+    } else {
+        const float inv = 1.0f / det;
+        const vec3 dpdu = (uv1 * uv2.y - uv2 * uv1.y) * inv;
+        f.b = normalized(cross(f.n, dpdu));
+        f.t = cross(f.b, f.n);
+    }
+    return f;
+}
+
+// Sphere::compute_surface_props (src/geom/sphere.rs:28-45).
+__device__ Frame sphere_frame(const GpuPrim& p, vec3 pt) {
+    Frame f;
+    f.n = normalized(pt - mk(p.center[0], p.center[1], p.center[2]));
+    if (nearly_zero(f.n.x) && nearly_zero(f.n.z)) f.t = mk(1.0f, 0.0f, 0.0f);
+    else f.t = normalized(mk(-f.n.z, 0.0f, f.n.x));
+    f.b = cross(f.n, f.t);
+    f.gn = f.n;
+    return f;
+}
+
+// Frame of a traversal hit: u, v are recomputed from the same stored triangle
+// record and ray, so they are the values the traversal saw.
+__device__ Frame hit_frame(const SceneView& sv, const ShadeView& sh, const gr_hit& h, vec3 o, vec3 d) {
+    const GpuPrim& p = sh.prims[h.prim];
+    if (p.type == kPrimMesh) {
+        const float4* q = reinterpret_cast<const float4*>(sv.comps + h.slot);
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        const vec3 c = mk(q0.x, q0.y, q0.z), e1 = mk(q0.w, q1.x, q1.y), e2 = mk(q1.z, q1.w, q2.x);
+        const vec3 pp = cross(d, e2);
+        const float inv_det = 1.0f / dot(e1, pp);
+        const vec3 t = o - c;
+        const float u = dot(t, pp) * inv_det;
+        const vec3 qq = cross(t, e1);
+        const float v = dot(d, qq) * inv_det;
+        return mesh_frame(sh, p, h.comp, u, v, 1.0f - u - v);
+    }
+    return sphere_frame(p, o + d * h.t);
+}
+
+// Index of the first cdf entry >= target (CumulativeDistribution::ind_sample,
+// src/core/random.rs:248-255).
+__device__ uint32_t cdf_search(const float* cdf, uint32_t n, float target) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (cdf[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+struct LightPoint {
+    vec3 p;
+    Frame f;
+    uint32_t prim;
+    float point_pdf;
+};
+
+// Bvh::sample_light's point part (src/geom/bvh.rs:481-496 via Prim::sample_world,
+// src/geom/mesh.rs:310-327, src/geom/sphere.rs:103-111) from the draws of
+// slots 4..7.
+__device__ LightPoint sample_light_point(const ShadeView& sh, uint4 r) {
+    LightPoint lp;
+    const uint32_t k = (uint32_t)(((uint64_t)r.x * sh.num_lights) >> 32);
+    lp.prim = sh.lights[k];
+    const GpuPrim& p = sh.prims[lp.prim];
+    if (p.type == kPrimMesh) {
+        const uint32_t ti = cdf_search(sh.cdf + p.cdf_off, p.ntris, u01(r.y));
+        const uint32_t* t = sh.tris + p.tri_off + 9 * (size_t)ti;
+        const vec3 a = ld3(sh.verts[p.vert_off + t[0]]), b = ld3(sh.verts[p.vert_off + t[1]]),
+                   c = ld3(sh.verts[p.vert_off + t[2]]);
+        const float sa = sqrtf(u01(r.z));
+        const float u = 1.0f - sa, v = u01(r.w) * sa;
+        const float w = 1.0f - u - v;
+        lp.p = (a * u + b * v) + c * w;
+        lp.f = mesh_frame(sh, p, ti, u, v, w);
+        lp.point_pdf = 1.0f / p.area;
+    } else {
+        lp.p = mk(p.center[0], p.center[1], p.center[2]) + dev::uniform_sphere(u01(r.z), u01(r.w)) * p.radius;
+        lp.f = sphere_frame(p, lp.p);
+        lp.point_pdf = 1.0f / (4.0f * kPi * p.radius * p.radius);
+    }
+    lp.point_pdf = lp.point_pdf / (float)sh.num_lights;
+    return lp;
+}
+
+__device__ __forceinline__ float light_area_pdf(const ShadeView& sh, uint32_t prim) {
+    const GpuPrim& p = sh.prims[prim];
+    const float pdf = p.type == kPrimMesh ? 1.0f / p.area : 1.0f / (4.0f * kPi * p.radius * p.radius);
+    return pdf / (float)sh.num_lights;
+}
+
+__device__ __forceinline__ float power_heuristic(float f_pdf, float g_pdf) {  // math.rs:137-142
+    const float f = 1.0f * f_pdf, g = 1.0f * g_pdf;
+    return (f * f) / (f * f + g * g);
+}
+
+__device__ __forceinline__ void russian_roulette(vec3& T, uint32_t depth, float rv) {
+    const float lum = dev::luminance(T);
+    const float a = dev::clamp_unit(lum);
+    const float live = depth > 20 ? 0.10f + a * (0.75f - 0.10f) : 0.25f + a * (1.00f - 0.25f);
+    if (rv < live) T = T / live;
+    else T = mk(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t lane = threadIdx.x & 63u;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Appends `alive` lanes to a queue with one atomic per wave; returns the slot.
+__device__ __forceinline__ uint32_t wave_append(bool alive, uint32_t* counter) {
+    const uint64_t m = __ballot(alive);
+    uint32_t base = 0;
+    if ((threadIdx.x & 63u) == 0 && m) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    return base + (uint32_t)__popcll(m & lanes_below());
+}
+
+struct RenderArgs {
+    SceneView sv;
+    ShadeView sh;
+    uint32_t width, height, npaths, max_depth;
+    uint64_t seed, path_base;
+    float cam[16];
+    float wx, wy;
+    RenderBuffers rb;
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Sample point (film.rs:55-76) and camera ray (camera.rs:87-97) per path.
+__global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npaths) return;
+    const uint32_t npx = a.width * a.height;
+    const uint32_t i = p % npx;
+    const uint32_t row = i / a.width, col = i % a.width;
+    const uint64_t path = a.path_base + p;
+    const uint4 r = draws(a.seed, path, kStreamFilm, 0, 0);
+    const float rj = (0.5f + (float)row) + (-2.0f + 2.0f * 2.0f * u01(r.x));
+    const float cj = (0.5f + (float)col) + (-2.0f + 2.0f * 2.0f * u01(r.y));
+    const float s = dev::lerp(-1.0f, 1.0f, cj / (float)a.width);
+    const float t = dev::lerp(-1.0f, 1.0f, rj / (float)a.height);
+    a.rb.st[p] = make_float2(s, t);
+    const float* m = a.cam;
+    const vec3 dir = normalized(mk(a.wx * s, a.wy * t, -1.0f));
+    const float z = 0.0f;
+    const float x = z * m[0] + z * m[4] + z * m[8] + m[12];
+    const float y = z * m[1] + z * m[5] + z * m[9] + m[13];
+    const float zz = z * m[2] + z * m[6] + z * m[10] + m[14];
+    const float w = z * m[3] + z * m[7] + z * m[11] + m[15];
+    const vec3 d = mk(dir.x * m[0] + dir.y * m[4] + dir.z * m[8], dir.x * m[1] + dir.y * m[5] + dir.z * m[9],
+                      dir.x * m[2] + dir.y * m[6] + dir.z * m[10]);
+    a.rb.rays[0][2 * (size_t)p] = make_float4(x / w, y / w, zz / w, 0.0f);
+    a.rb.rays[0][2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    a.rb.queue[0][p] = p;
+    a.rb.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.rb.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.rb.aux[p] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);  // previous bsdf pdf, previous bounce specular
+    if (p == 0) {
+        a.rb.counts[0] = a.npaths;
+        a.rb.counts[1] = 0;
+        a.rb.counts[2] = 0;
+    }
+}
+
+__global__ void k_reset(uint32_t* c, uint32_t which) { c[which] = 0u; }
+
+// One bounce of PathTracerIntegrator::integrate (src/render/integrators.rs:42-99).
+template <bool NEE>
+__global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.rb.counts[cur];
+    bool alive = false;
+    bool shadow = false;
+    uint32_t p = 0;
+    vec3 new_o = mk(0.0f, 0.0f, 0.0f), new_d = new_o, seg_to = new_o, c_nee = new_o, hp = new_o;
+    if (i < n) {
+        p = a.rb.queue[cur][i];
+        const uint4 hv = a.rb.hits[i];
+        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
+        const float4 ro = a.rb.rays[cur][2 * (size_t)i], rd = a.rb.rays[cur][2 * (size_t)i + 1];
+        const vec3 o = ld3(ro), d = ld3(rd);
+        vec3 T = ld3(a.rb.T[p]);
+        vec3 L = ld3(a.rb.L[p]);
+        const uint64_t path = a.path_base + p;
+        if (h.prim != kNone) {
+            const Frame f = hit_frame(a.sv, a.sh, h, o, d);
+            const GpuMaterial& m = a.sh.mats[h.prim];
+            const vec3 wi = -d;
+            hp = o + d * h.t;
+            const uint4 r0 = draws(a.seed, path, kStreamCamera, depth, 0);
+            if (!NEE) {
+                const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                L = L + mul(T, smp.emission);
+                T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
+                new_o = hp + smp.outgoing * kRayPush;
+                new_d = smp.outgoing;
+                if (depth > 10 || dev::nearly_zero3(T)) russian_roulette(T, depth, u01(r0.w));
+                alive = !dev::is_zero(T);
+            } else {
+                const float4 aux = a.rb.aux[p];
+                const vec3 le = dev::light_world(m, wi, f);
+                if (!dev::is_zero(le)) {
+                    if (aux.y != 0.0f) {
+                        L = L + mul(T, le);
+                    } else {
+                        const float pl = light_area_pdf(a.sh, h.prim) * (h.t * h.t) / fabsf(dot(f.gn, wi));
+                        L = L + mul(T, le) * power_heuristic(aux.x, pl);
+                    }
+                }
+                if (m.nlobes != 0) {
+                    if (a.sh.num_lights > 0 && dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0) {
+                        const LightPoint lp = sample_light_point(a.sh, draws(a.seed, path, kStreamCamera, depth, 1));
+                        const vec3 to = lp.p - hp;
+                        const float d2 = dot(to, to);
+                        const float dist = sqrtf(d2);
+                        const vec3 wo = to / dist;
+                        const vec3 le_l = dev::light_world(a.sh.mats[lp.prim], -wo, lp.f);
+                        const float cos_l = fabsf(dot(lp.f.gn, wo));
+                        if (!dev::is_zero(le_l) && cos_l > 0.0f && d2 > 0.0f) {
+                            const vec3 fr = dev::f_world(m, wi, wo, f, true);
+                            if (!dev::is_zero(fr)) {
+                                const float pdf_light = lp.point_pdf * d2 / cos_l;
+                                const float pdf_bsdf = dev::pdf_world(m, wi, wo, f);
+                                const float w = power_heuristic(pdf_light, pdf_bsdf);
+                                c_nee = mul(mul(T, fr), le_l) * (fabsf(dot(f.n, wo)) * w / pdf_light);
+                                seg_to = lp.p;
+                                shadow = true;
+                            }
+                        }
+                    }
+                    const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                    T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
+                    a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
+                    new_o = hp + smp.outgoing * kRayPush;
+                    new_d = smp.outgoing;
+                    if (depth > 10 || dev::nearly_zero3(T)) russian_roulette(T, depth, u01(r0.w));
+                    alive = !dev::is_zero(T);
+                }
+            }
+        }
+        a.rb.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
+        a.rb.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
+        if (a.max_depth != 0 && depth + 1 >= a.max_depth) alive = false;
+    }
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t slot = wave_append(alive, &a.rb.counts[nxt]);
+    if (alive) {
+        a.rb.queue[nxt][slot] = p;
+        a.rb.rays[nxt][2 * (size_t)slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+        a.rb.rays[nxt][2 * (size_t)slot + 1] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+    }
+    if (NEE) {
+        const uint32_t ss = wave_append(shadow, &a.rb.counts[2]);
+        if (shadow) {
+            a.rb.seg[2 * (size_t)ss] = make_float4(hp.x, hp.y, hp.z, 0.0f);
+            a.rb.seg[2 * (size_t)ss + 1] = make_float4(seg_to.x, seg_to.y, seg_to.z, 0.0f);
+            a.rb.contrib[ss] = make_float4(c_nee.x, c_nee.y, c_nee.z, 0.0f);
+            a.rb.seg_path[ss] = p;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.rb.counts[2] || a.rb.vis[j] == 0) return;
+    const uint32_t p = a.rb.seg_path[j];
+    const float4 c = a.rb.contrib[j];
+    float4 L = a.rb.L[p];
+    L.x = L.x + c.x;
+    L.y = L.y + c.y;
+    L.z = L.z + c.z;
+    a.rb.L[p] = L;
+}
+
+__global__ __launch_bounds__(256) void k_export_samples(RenderArgs a, float4* out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < a.npaths) out[p] = a.rb.L[p];
+}
+
+// mitchell_filter1 with B = C = 1/3 (src/core/math.rs:92-108).
+__device__ __forceinline__ float mitchell1(float x) {
+    const float B = 1.0f / 3.0f, C = 1.0f / 3.0f;
+    const float tx = fabsf(2.0f * x);
+    if (tx > 1.0f)
+        return ((-B - 6.0f * C) * (tx * tx * tx) + (6.0f * B + 30.0f * C) * (tx * tx) + (-12.0f * B - 48.0f * C) * tx +
+                (8.0f * B + 24.0f * C)) * (1.0f / 6.0f);
+    return ((12.0f - 9.0f * B - 6.0f * C) * (tx * tx * tx) + (-18.0f + 12.0f * B + 6.0f * C) * (tx * tx) +
+            (6.0f - 2.0f * B)) * (1.0f / 6.0f);
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+__device__ __forceinline__ int f2i(float f) {  // saturating, NaN -> 0 (Rust `as`)
+    if (f != f) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return -2147483647 - 1;
+    return (int)f;
+}
+
+constexpr int kTile = 16, kApron = 4, kSpan = kTile + 2 * kApron;
+
+// Film::report_samples as a gather (film.rs:78-114): pixel (x, y) visits the
+// samples of pixels y-4..y+4, x-4..x+4 in row-major order -- the order the
+// serial splat reaches it -- and applies each sample's own footprint test.
+__global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters) {
+    __shared__ float s_s[kSpan * kSpan], s_t[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
+        s_b[kSpan * kSpan];
+    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const int x = x0 + tx, y = y0 + ty;
+    const int W = (int)a.width, H = (int)a.height;
+    const bool inside = x < W && y < H;
+    float4 acc = inside ? film[(size_t)y * W + x] : make_float4(0, 0, 0, 0);
+    const float wf = (float)a.width, hf = (float)a.height;
+    const int last_col = W - 1, last_row = H - 1;
+    const uint32_t npx = a.width * a.height;
+    for (uint32_t it = 0; it < iters; ++it) {
+        for (int k = threadIdx.x; k < kSpan * kSpan; k += kTile * kTile) {
+            const int sx = x0 - kApron + k % kSpan, sy = y0 - kApron + k / kSpan;
+            if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                const size_t p = (size_t)it * npx + (size_t)sy * W + sx;
+                const float2 st = a.rb.st[p];
+                const float4 L = a.rb.L[p];
+                s_s[k] = st.x;
+                s_t[k] = st.y;
+                s_r[k] = L.x;
+                s_g[k] = L.y;
+                s_b[k] = L.z;
+            }
+        }
+        __syncthreads();
+        if (inside) {
+            for (int sy = y - kApron; sy <= y + kApron; ++sy) {
+                if (sy < 0 || sy >= H) continue;
+                for (int sx = x - kApron; sx <= x + kApron; ++sx) {
+                    if (sx < 0 || sx >= W) continue;
+                    const int k = (sy - (y0 - kApron)) * kSpan + (sx - (x0 - kApron));
+                    const float s = s_s[k], t = s_t[k];
+                    const float col_cont = 0.0f + (0.5f * (s + 1.0f)) * (wf - 0.0f);
+                    const float row_cont = 0.0f + (0.5f * (t + 1.0f)) * (hf - 0.0f);
+                    const float cd = col_cont - 0.5f, rd = row_cont - 0.5f;
+                    const int min_col = clampi(f2i(ceilf(cd - 2.0f)), 0, last_col);
+                    const int max_col = clampi(f2i(floorf(cd + 2.0f)), 0, last_col);
+                    const int min_row = clampi(f2i(ceilf(rd - 2.0f)), 0, last_row);
+                    const int max_row = clampi(f2i(floorf(rd + 2.0f)), 0, last_row);
+                    if (x < min_col || x > max_col || y < min_row || y > max_row) continue;
+                    const float wt = mitchell1(((float)x - cd) / 2.0f) * mitchell1(((float)y - rd) / 2.0f);
+                    acc.x = acc.x + s_r[k] * wt;
+                    acc.y = acc.y + s_g[k] * wt;
+                    acc.z = acc.z + s_b[k] * wt;
+                    acc.w += wt;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (inside) film[(size_t)y * W + x] = acc;
+}
+
+// Per-function material evaluation (test hook for gr_material_query).
+__global__ void k_material_query(ShadeView sh, uint32_t prim, const float* in, uint32_t n, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = in + 24 * (size_t)i;
+    float* r = out + 20 * (size_t)i;
+    const GpuMaterial& m = sh.mats[prim];
+    const vec3 wi = mk(q[0], q[1], q[2]), wo = mk(q[3], q[4], q[5]);
+    Frame f;
+    f.n = mk(q[6], q[7], q[8]);
+    f.t = mk(q[9], q[10], q[11]);
+    f.b = mk(q[12], q[13], q[14]);
+    f.gn = mk(q[15], q[16], q[17]);
+    const bool c2l = q[21] != 0.0f;
+    const vec3 fw = dev::f_world(m, wi, wo, f, c2l);
+    const float pw = dev::pdf_world(m, wi, wo, f);
+    const vec3 lw = dev::light_world(m, wi, f);
+    const dev::MatSample s = dev::sample_world(m, wi, f, c2l, __float_as_uint(q[18]), q[19], q[20]);
+    r[0] = fw.x; r[1] = fw.y; r[2] = fw.z; r[3] = pw;
+    r[4] = lw.x; r[5] = lw.y; r[6] = lw.z;
+    r[7] = s.emission.x; r[8] = s.emission.y; r[9] = s.emission.z;
+    r[10] = s.radiance.x; r[11] = s.radiance.y; r[12] = s.radiance.z;
+    r[13] = s.outgoing.x; r[14] = s.outgoing.y; r[15] = s.outgoing.z;
+    r[16] = s.pdf; r[17] = (float)s.kind;
+    r[18] = (float)dev::count_lobes(m, kLobeDiffuse | kLobeGlossy); r[19] = 0.0f;
+}
+
+inline dim3 blocks_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// Grows the film's wavefront buffers to hold `paths` paths.
+hipError_t reserve(RenderBuffers& rb, size_t paths) {
+    if (rb.capacity >= paths) return hipSuccess;
+    (void)hipFree(rb.blob);
+    rb.blob = nullptr;
+    rb.capacity = 0;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    size_t o_rays0 = off; off = align(off + paths * 32);
+    size_t o_rays1 = off; off = align(off + paths * 32);
+    size_t o_q0 = off; off = align(off + paths * 4);
+    size_t o_q1 = off; off = align(off + paths * 4);
+    size_t o_hits = off; off = align(off + paths * 16);
+    size_t o_T = off; off = align(off + paths * 16);
+    size_t o_L = off; off = align(off + paths * 16);
+    size_t o_aux = off; off = align(off + paths * 16);
+    size_t o_st = off; off = align(off + paths * 8);
+    size_t o_seg = off; off = align(off + paths * 32);
+    size_t o_con = off; off = align(off + paths * 16);
+    size_t o_sp = off; off = align(off + paths * 4);
+    size_t o_vis = off; off = align(off + paths * 4);
+    size_t o_cnt = off; off = align(off + 64);
+    char* b = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b), off);
+    if (e != hipSuccess) return e;
+    rb.blob = b;
+    rb.capacity = paths;
+    rb.rays[0] = reinterpret_cast<float4*>(b + o_rays0);
+    rb.rays[1] = reinterpret_cast<float4*>(b + o_rays1);
+    rb.queue[0] = reinterpret_cast<uint32_t*>(b + o_q0);
+    rb.queue[1] = reinterpret_cast<uint32_t*>(b + o_q1);
+    rb.hits = reinterpret_cast<uint4*>(b + o_hits);
+    rb.T = reinterpret_cast<float4*>(b + o_T);
+    rb.L = reinterpret_cast<float4*>(b + o_L);
+    rb.aux = reinterpret_cast<float4*>(b + o_aux);
+    rb.st = reinterpret_cast<float2*>(b + o_st);
+    rb.seg = reinterpret_cast<float4*>(b + o_seg);
+    rb.contrib = reinterpret_cast<float4*>(b + o_con);
+    rb.seg_path = reinterpret_cast<uint32_t*>(b + o_sp);
+    rb.vis = reinterpret_cast<uint32_t*>(b + o_vis);
+    rb.counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    return hipSuccess;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host driver
+// ---------------------------------------------------------------------------
+hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_params& prm, gr_film* film) {
+    hipStream_t s = sc->ctx->stream;
+    const uint32_t W = film->width, H = film->height;
+    const size_t npx = (size_t)W * H;
+    uint32_t batch = prm.batch;
+    if (batch == 0) batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, (1u << 24) / npx));
+    batch = std::min(batch, prm.spp);
+    const size_t paths = npx * batch;
+    if (paths > 0x7fffffffull) return hipErrorInvalidValue;
+    hipError_t e = reserve(film->rb, paths);
+    if (e != hipSuccess) return e;
+
+    RenderArgs a;
+    a.sv = sc->view;
+    a.sh = sc->shade;
+    a.width = W;
+    a.height = H;
+    a.max_depth = prm.max_depth;
+    a.seed = prm.seed;
+    camera_matrix(cam, a.cam);
+    a.wx = cam.horizontal_aperture / (cam.focal_length * 2.0f);  // window_max, camera.rs:76-79
+    a.wy = cam.vertical_aperture / (cam.focal_length * 2.0f);
+    a.rb = film->rb;
+    const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
+    if (prm.integrator == GR_INTEGRATOR_BDPT) return hipErrorNotSupported;
+    const dim3 gather_grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
+
+    for (uint32_t done = 0; done < prm.spp && e == hipSuccess; done += batch) {
+        const uint32_t nb = std::min(batch, prm.spp - done);
+        a.npaths = (uint32_t)(npx * nb);
+        a.path_base = (prm.sample_offset + done) * (uint64_t)npx;
+        hipLaunchKernelGGL(k_camera, blocks_for(a.npaths), dim3(256), 0, s, a);
+        uint32_t cur = 0;
+        for (uint32_t depth = 0; prm.max_depth == 0 || depth < prm.max_depth; ++depth) {
+            if (prm.max_depth == 0) {
+                // Unbounded paths (the reference's loop): stop once no path is live.
+                uint32_t live = 0;
+                e = hipMemcpyAsync(&live, film->rb.counts + cur, 4, hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess || live == 0) break;
+            }
+            e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), a.npaths,
+                                       film->rb.counts + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, cur ^ 1u);
+            if (nee) {
+                hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 2u);
+                hipLaunchKernelGGL(k_shade<true>, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                e = launch_occluded_queue(sc->view, reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
+                                          film->rb.counts + 2, film->rb.vis, s);
+                if (e != hipSuccess) break;
+                hipLaunchKernelGGL(k_shadow_accum, blocks_for(a.npaths), dim3(256), 0, s, a);
+            } else {
+                hipLaunchKernelGGL(k_shade<false>, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+            }
+            e = hipGetLastError();
+            cur ^= 1u;
+        }
+        if (e != hipSuccess) break;
+        if (prm.d_sample_radiance)
+            hipLaunchKernelGGL(k_export_samples, blocks_for(a.npaths), dim3(256), 0, s, a,
+                               reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx);
+        hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t material_query(gr_scene* sc, uint32_t prim, const float* in, uint32_t n, float* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_material_query, blocks_for(n), dim3(256), 0, sc->ctx->stream, sc->shade, prim, in, n, out);
+    return hipGetLastError();
+}
+
+void release_render_buffers(RenderBuffers& rb) {
+    (void)hipFree(rb.blob);
+    (void)hipFree(rb.bdpt_blob);
+    rb = RenderBuffers{};
+}
+
+}  // namespace gr

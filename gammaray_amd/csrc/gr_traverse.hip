@@ -157,12 +157,15 @@ template <int MODE, bool COUNT>
 __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __restrict__ in,
                                                   uint32_t n, typename Output<MODE>::type* __restrict__ out,
                                                   uint32_t* __restrict__ counts,
-                                                  uint32_t* __restrict__ next_ray) {
+                                                  uint32_t* __restrict__ next_ray,
+                                                  const uint32_t* __restrict__ n_ptr) {
     __shared__ uint32_t s_ref[kLdsStack][kWave];
     __shared__ float s_t[kLdsStack][kWave];
     uint32_t spill_ref[kSpill];
     float spill_t[kSpill];
     const uint32_t lane = threadIdx.x;
+    // A ray count that earlier kernels on the stream produced (queue length).
+    if (n_ptr) n = __builtin_amdgcn_readfirstlane(*n_ptr);
 
     bool active = false;
     bool exhausted = false;
@@ -348,9 +351,11 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
 
 __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
+// n: the ray count, or (with n_ptr) an upper bound of the count stored at
+// *n_ptr by earlier work on the stream.
 template <int MODE, bool COUNT>
-hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, void* out,
-                        uint32_t* counts, hipStream_t s) {
+hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, const uint32_t* n_ptr,
+                        void* out, uint32_t* counts, hipStream_t s) {
     static int grid = 0;
     if (grid == 0) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -366,7 +371,7 @@ hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, void* o
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, sv.next_ray);
     hipLaunchKernelGGL((k_trace<MODE, COUNT>), dim3(blocks), dim3(kWave), 0, s, sv,
                        reinterpret_cast<const float4*>(in), n,
-                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, sv.next_ray);
+                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, sv.next_ray, n_ptr);
     return hipGetLastError();
 }
 
@@ -375,18 +380,30 @@ hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, void* o
 hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n, gr_hit* hits,
                             uint32_t* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    return counts ? launch_trace<kClosest, true>(sv, rays, n, hits, counts, s)
-                  : launch_trace<kClosest, false>(sv, rays, n, hits, counts, s);
+    return counts ? launch_trace<kClosest, true>(sv, rays, n, nullptr, hits, counts, s)
+                  : launch_trace<kClosest, false>(sv, rays, n, nullptr, hits, counts, s);
 }
 
 hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
                            uint32_t* visible, int mode, uint32_t* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (mode == GR_OCCLUSION_ANYHIT)
-        return counts ? launch_trace<kOcclusionAnyHit, true>(sv, segs, n, visible, counts, s)
-                      : launch_trace<kOcclusionAnyHit, false>(sv, segs, n, visible, counts, s);
-    return counts ? launch_trace<kOcclusionExact, true>(sv, segs, n, visible, counts, s)
-                  : launch_trace<kOcclusionExact, false>(sv, segs, n, visible, counts, s);
+        return counts ? launch_trace<kOcclusionAnyHit, true>(sv, segs, n, nullptr, visible, counts, s)
+                      : launch_trace<kOcclusionAnyHit, false>(sv, segs, n, nullptr, visible, counts, s);
+    return counts ? launch_trace<kOcclusionExact, true>(sv, segs, n, nullptr, visible, counts, s)
+                  : launch_trace<kOcclusionExact, false>(sv, segs, n, nullptr, visible, counts, s);
+}
+
+hipError_t launch_intersect_queue(const SceneView& sv, const gr_ray* rays, uint32_t n_max,
+                                  const uint32_t* n_ptr, gr_hit* hits, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    return launch_trace<kClosest, false>(sv, rays, n_max, n_ptr, hits, nullptr, s);
+}
+
+hipError_t launch_occluded_queue(const SceneView& sv, const gr_segment* segs, uint32_t n_max,
+                                 const uint32_t* n_ptr, uint32_t* visible, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    return launch_trace<kOcclusionExact, false>(sv, segs, n_max, n_ptr, visible, nullptr, s);
 }
 
 }  // namespace gr

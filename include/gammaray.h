@@ -122,6 +122,53 @@ int gr_occluded(gr_scene *scene, const gr_segment *d_segs, uint32_t n, uint32_t 
  * prim = GR_NONE with t = -1 (closest hit) or not-visible (visibility). */
 int gr_scene_take_overflow_count(gr_scene *scene, uint64_t *out);
 
+/* ---- rendering (Stage::trace, src/render/stage.rs:23-36) ---------------- */
+/* Camera::new arguments (src/core/camera.rs:42-61); f_stop is carried but, as
+ * in the reference, unused (pinhole camera). */
+typedef struct {
+    float focal_length, horizontal_aperture, vertical_aperture, f_stop;
+    float rotate[4];    /* quaternion (real, i, j, k) */
+    float translate[3];
+} gr_camera;
+
+#define GR_INTEGRATOR_PT 0      /* PathTracerIntegrator, src/render/integrators.rs:39-100 */
+#define GR_INTEGRATOR_PT_NEE 1  /* path tracing + next-event estimation with MIS (no reference counterpart) */
+#define GR_INTEGRATOR_BDPT 2    /* BdptIntegrator, src/render/integrators.rs:102-418 */
+
+typedef struct {
+    uint32_t integrator;
+    uint32_t spp;            /* iterations; one sample per pixel each (film.rs:55-76) */
+    uint32_t max_depth;      /* intersections per camera path; 0 = unbounded (the reference) */
+    uint32_t light_depth;    /* BDPT: bounces after the light vertex; 0 = reference cap */
+    uint64_t seed;           /* RNG key */
+    uint64_t sample_offset;  /* index of the first iteration (progressive rendering, resume) */
+    uint32_t batch;          /* iterations traced together; 0 = automatic */
+    uint32_t reserved;
+    float *d_sample_radiance; /* optional device buffer: spp*W*H float4 per-sample radiance */
+} gr_trace_params;
+
+/* FilmPixel grid (src/render/film.rs:27-53): per pixel (accum rgb, weight). */
+int gr_film_create(gr_ctx *ctx, uint32_t width, uint32_t height, gr_film **out);
+void gr_film_destroy(gr_film *film);
+int gr_film_clear(gr_film *film);
+/* Host copy of the film: width*height*4 floats (r, g, b, weight), row 0 first. */
+int gr_film_readback(gr_film *film, float *rgbw);
+/* Restores a film from a host copy (checkpoint / resume). */
+int gr_film_upload(gr_film *film, const float *rgbw);
+/* spp iterations of Stage::trace with the given integrator; samples splat into
+ * the film with the reference's Mitchell filter in the reference's order. */
+int gr_trace(gr_scene *scene, const gr_camera *camera, const gr_trace_params *params, gr_film *film);
+/* ExrWriter::update + write (src/render/exr.rs:168-199): scanline EXR, B,G,R
+ * float channels, rows flipped, value = accum / weight. */
+int gr_film_write_exr(gr_film *film, const char *path);
+
+/* Per-function material evaluation on the device (test hook): n items of 24
+ * floats in d_in -- wi(3), wo(3), normal, tangent, binormal, geom_normal (12),
+ * u_lobe (uint32 bits), u1, u2, camera_to_light, pad(2); 20 floats out per item
+ * -- f_world(3), pdf_world, light_world(3), sample_world emission(3),
+ * radiance(3), outgoing(3), pdf, kind, count_lobes(DIFFUSE|GLOSSY), pad. */
+int gr_material_query(gr_scene *scene, uint32_t prim, const float *d_in, uint32_t n, float *d_out);
+
 #ifdef __cplusplus
 }
 #endif

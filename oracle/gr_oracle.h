@@ -24,6 +24,7 @@
 #ifndef GR_ORACLE_H
 #define GR_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -77,6 +78,33 @@ void orc_intersect_naive(const orc_scene *s, const float *rays, int64_t n, uint3
  * out: 1 = visible.  counts optional as for orc_intersect. */
 int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t *out,
                        uint32_t *counts, int nthreads);
+
+/* ---- rendering (orc_render.c) ---- */
+#define ORC_PT 0      /* PathTracerIntegrator + max_depth cap */
+#define ORC_PT_NEE 1  /* path tracing with next-event estimation + MIS */
+#define ORC_BDPT 2    /* BdptIntegrator with camera/light vertex caps */
+/* Camera::new's xform = translation * rotation (row-major 4x4). */
+void orc_camera_matrix(const float quat[4], const float translate[3], float out[16]);
+/* Camera::compute_ray for n lens points st (2 floats each) -> rays (6 each). */
+void orc_camera_rays(const float cam[16], float wx, float wy, const float *st, int64_t n, float *rays);
+float orc_mitchell1(float x);
+/* Film::compute_sample_points for iteration `sample_offset` (2 floats/pixel). */
+void orc_sample_points(uint64_t seed, uint64_t sample_offset, uint32_t width, uint32_t height, float *st);
+/* Film::report_samples for n samples (st, colors) in order: film holds
+ * (r, g, b, weight) per pixel. */
+void orc_report_samples(float *film, uint32_t width, uint32_t height, const float *st, const float *colors,
+                        int64_t n);
+/* spp iterations of Stage::trace: per iteration, sample points, integrate
+ * every pixel's sample (path id = iteration * W * H + pixel), splat in sample
+ * order.  colors (optional) receives spp * W * H * 3 sample radiances. */
+int orc_render(const orc_scene *s, const float cam[16], float wx, float wy, uint32_t width,
+               uint32_t height, int integrator, uint32_t max_depth, uint32_t light_depth,
+               uint64_t seed, uint64_t sample_offset, uint32_t spp, float *film, float *colors,
+               int nthreads);
+void orc_material_query(const orc_scene *s, uint32_t prim, int64_t n, const float *in, float *out);
+/* ExrWriter bytes for a film (rgbw per pixel); returns the size, writes up to
+ * cap bytes to out (may be NULL). */
+size_t orc_exr_encode(const float *rgbw, uint32_t width, uint32_t height, uint8_t *out, size_t cap);
 
 /* Philox4x32-10 block function (counter-based RNG shared by the GPU path). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -55,6 +55,20 @@ def lib():
         L.orc_visibility.argtypes = [_vp, _f32p, ctypes.c_int64, _u8p, _u32p, ctypes.c_int]
         L.orc_visibility.restype = ctypes.c_int64
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_camera_matrix.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_camera_rays.argtypes = [_f32p, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_int64, _f32p]
+        L.orc_mitchell1.argtypes = [ctypes.c_float]
+        L.orc_mitchell1.restype = ctypes.c_float
+        L.orc_sample_points.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _f32p]
+        L.orc_report_samples.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                         ctypes.c_int64]
+        L.orc_render.argtypes = [_vp, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, _f32p, _f32p,
+                                 ctypes.c_int]
+        L.orc_material_query.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int64, _f32p, _f32p]
+        L.orc_exr_encode.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _u8p, ctypes.c_size_t]
+        L.orc_exr_encode.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
@@ -169,3 +183,82 @@ class OracleScene:
         if counts:
             res["counts"] = cnt
         return res
+
+
+# ---- rendering -------------------------------------------------------------
+PT, PT_NEE, BDPT = 0, 1, 2
+
+
+def camera_matrix(cam):
+    """Camera::new's xform (row-major 4x4) for a gammaray_amd.scenes.CameraSpec."""
+    q = np.ascontiguousarray(cam.rotate, np.float32)
+    t = np.ascontiguousarray(cam.translate, np.float32)
+    out = np.zeros(16, np.float32)
+    lib().orc_camera_matrix(_p(q, _f32p), _p(t, _f32p), _p(out, _f32p))
+    return out
+
+
+def window_max(cam):
+    """Camera::window_max, src/core/camera.rs:76-79 (f32)."""
+    f = np.float32(cam.focal_length)
+    return (np.float32(cam.horizontal_aperture) / (f * np.float32(2.0)),
+            np.float32(cam.vertical_aperture) / (f * np.float32(2.0)))
+
+
+def camera_rays(cam, st):
+    st = np.ascontiguousarray(st, np.float32).reshape(-1, 2)
+    m = camera_matrix(cam)
+    wx, wy = window_max(cam)
+    out = np.zeros((len(st), 6), np.float32)
+    lib().orc_camera_rays(_p(m, _f32p), float(wx), float(wy), _p(st, _f32p), len(st), _p(out, _f32p))
+    return out
+
+
+def mitchell1(x):
+    return lib().orc_mitchell1(float(x))
+
+
+def sample_points(seed, iteration, width, height):
+    st = np.zeros((height * width, 2), np.float32)
+    lib().orc_sample_points(seed, iteration, width, height, _p(st, _f32p))
+    return st
+
+
+def report_samples(film, width, height, st, colors):
+    film = np.ascontiguousarray(film, np.float32)
+    st = np.ascontiguousarray(st, np.float32).reshape(-1, 2)
+    colors = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+    assert len(st) == len(colors)
+    lib().orc_report_samples(_p(film, _f32p), width, height, _p(st, _f32p), _p(colors, _f32p), len(st))
+    return film
+
+
+def exr_encode(rgbw, width, height):
+    rgbw = np.ascontiguousarray(rgbw, np.float32)
+    n = lib().orc_exr_encode(_p(rgbw, _f32p), width, height, None, 0)
+    out = np.zeros(n, np.uint8)
+    lib().orc_exr_encode(_p(rgbw, _f32p), width, height, _p(out, _u8p), n)
+    return out.tobytes()
+
+
+def render(scene, spec, width, height, integrator, spp, max_depth=0, light_depth=0, seed=1,
+           sample_offset=0, film=None, colors=False, threads=None):
+    """spp iterations of Stage::trace on the oracle.  Returns (film (H*W, 4),
+    per-sample radiance (spp, H*W, 3) or None)."""
+    m = camera_matrix(spec.camera)
+    wx, wy = window_max(spec.camera)
+    f = np.zeros((height * width, 4), np.float32) if film is None else np.ascontiguousarray(film, np.float32)
+    c = np.zeros((spp, height * width, 3), np.float32) if colors else None
+    r = lib().orc_render(scene._h, _p(m, _f32p), float(wx), float(wy), width, height, integrator,
+                         max_depth, light_depth, seed, sample_offset, spp, _p(f, _f32p), _p(c, _f32p),
+                         threads or default_threads())
+    if r != 0:
+        raise RuntimeError("orc_render failed")
+    return f, c
+
+
+def material_query(scene, prim, items):
+    items = np.ascontiguousarray(items, np.float32).reshape(-1, 24)
+    out = np.zeros((len(items), 20), np.float32)
+    lib().orc_material_query(scene._h, prim, len(items), _p(items, _f32p), _p(out, _f32p))
+    return out

@@ -1,0 +1,172 @@
+"""GPU parity of the render path (camera, materials, integrators, film, EXR).
+
+The GPU draws every random number from the same addressed Philox stream as
+the oracle, so for one seed both render the same paths:
+  * material f / pdf evaluations without transcendental calls are bit-exact;
+    sampling (sin/cos/log/pow from each side's math library) within 1e-4;
+  * per-sample path radiance matches sample by sample (allowing the rare path
+    a last-bit difference in a transcendental sends another way);
+  * with a scene whose radiance is exactly known per sample, the film is
+    bit-exact (same sample points, same Mitchell weights, same summation
+    order);
+  * independent seeds agree statistically (per-pixel 3 sigma).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gammaray_amd import scenes
+from gammaray_amd.api import Film, Stage, material_query, trace
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _sample_radiance(stage, film, integrator, spp, **kw):
+    import torch
+    W, H = film.width, film.height
+    buf = torch.zeros((spp * W * H, 4), dtype=torch.float32, device="cuda")
+    trace(stage, film, integrator, spp=spp, sample_radiance_ptr=buf.data_ptr(), **kw)
+    stage.ctx.synchronize()
+    return buf.cpu().numpy()[:, :3].reshape(spp, H * W, 3)
+
+
+def _light_wall_scene():
+    s = scenes.SceneSpec(name="lightwall")
+    v, f = scenes.quad((0.0, 0.0, -5.0), (50.0, 0.0, 0.0), (0.0, 50.0, 0.0), (0.0, 0.0, 1.0))
+    s.add_mesh(v, f, scenes.diffuse_light((15.0, 7.0, 3.0)))
+    s.camera = scenes.CameraSpec(focal_length=1.0, horizontal_aperture=1.0, vertical_aperture=0.75)
+    return s
+
+
+def test_film_and_camera_bit_exact(gpu_ctx):
+    """Every sample sees the light (radiance exactly Le), so the film sums
+    depend only on sample points, Mitchell weights and summation order."""
+    print("case film bit-exact", flush=True)
+    spec = _light_wall_scene()
+    W, H = 40, 30
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    trace(stage, film, "pt", spp=3, max_depth=1, seed=11)
+    trace(stage, film, "pt", spp=2, max_depth=1, seed=11, sample_offset=3)
+    g = film.readback()
+    o = O.OracleScene(spec)
+    ref, _ = O.render(o, spec, W, H, O.PT, spp=5, max_depth=1, seed=11)
+    assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
+
+
+def _query_items(rng, n):
+    items = np.zeros((n, 24), np.float32)
+    wi = rng.normal(size=(n, 3))
+    wo = rng.normal(size=(n, 3))
+    nrm = rng.normal(size=(n, 3))
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    t = np.cross(nrm, rng.normal(size=(n, 3)))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    gn = nrm + 0.1 * rng.normal(size=(n, 3))
+    gn /= np.linalg.norm(gn, axis=1, keepdims=True)
+    items[:, 0:3], items[:, 3:6], items[:, 6:9] = wi, wo, nrm
+    items[:, 9:12], items[:, 12:15], items[:, 15:18] = t, b, gn
+    items[:, 18] = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    items[:, 19:21] = rng.random((n, 2))
+    items[:, 21] = rng.integers(0, 2, n)
+    return items
+
+
+def test_material_functions_match_oracle(gpu_ctx):
+    print("case material query", flush=True)
+    rng = np.random.default_rng(5)
+    spec = scenes.showcase_scene(n_spheres=40, grid=(10, 10))
+    spec.add_sphere((9, 9, 9), 0.5, scenes.diffuse())
+    spec.add_sphere((9, 9, 7), 0.5, scenes.mirror())
+    stage = Stage(spec, ctx=gpu_ctx)
+    o = O.OracleScene(spec)
+    for prim in range(len(spec.prims)):
+        items = _query_items(rng, 4096)
+        g = material_query(stage, prim, items)
+        r = O.material_query(o, prim, items)
+        has_cc = spec.prims[prim].material[0] == 0 and spec.prims[prim].material[12] > 0
+        # f_world, pdf_world, light_world and count_lobes: no transcendental
+        # call unless a clearcoat (ln) is present -> bit-exact.
+        cols = [0, 1, 2, 3, 4, 5, 6, 18]
+        if not has_cc:
+            assert np.array_equal(g[:, cols].view(np.uint32), r[:, cols].view(np.uint32)), prim
+        else:
+            np.testing.assert_allclose(g[:, cols], r[:, cols], rtol=2e-5, atol=1e-6)
+        # sample_world: emission exact; kind exact; sampled values agree to
+        # 1e-4 except where a last-bit difference in sin/cos/pow flips a branch
+        assert np.array_equal(g[:, 7:10], r[:, 7:10])
+        close = np.all(np.isclose(g[:, 10:17], r[:, 10:17], rtol=1e-4, atol=1e-5), axis=1)
+        assert close.mean() > 0.999, (prim, close.mean())
+        assert np.mean(g[:, 17] == r[:, 17]) > 0.999
+
+
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE)])
+def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
+    """Same seed -> the same paths: per-sample radiance agrees."""
+    print("case paired", integrator, flush=True)
+    spec = scenes.box_scene() if integrator == "pt" else scenes.showcase_scene(n_spheres=60, grid=(40, 40))
+    W, H, spp = 48, 40, 4
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, seed=21)
+    o = O.OracleScene(spec)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, seed=21, colors=True)
+    close = np.all(np.isclose(g, r, rtol=1e-3, atol=1e-5), axis=2)
+    assert close.mean() > 0.99, close.mean()
+    np.testing.assert_allclose(g.reshape(-1, 3).mean(0), r.reshape(-1, 3).mean(0), rtol=2e-2)
+    assert np.isfinite(g).all()
+
+
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE)])
+def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
+    """Per-pixel means of GPU and oracle (different seeds) within 3 sigma."""
+    print("case 3sigma", integrator, flush=True)
+    spec = scenes.box_scene()
+    W, H, spp = 32, 32, 128
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, seed=101)
+    o = O.OracleScene(spec)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, seed=202, colors=True)
+    lg, lr = g.sum(2), r.sum(2)  # (spp, pixels)
+    mg, mr = lg.mean(0), lr.mean(0)
+    se = np.sqrt(lg.var(0, ddof=1) / spp + lr.var(0, ddof=1) / spp)
+    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
+    assert np.mean(z > 3.0) < 0.02, np.mean(z > 3.0)
+    tot = abs(lg.mean() - lr.mean()) / np.sqrt(lg.var() / lg.size + lr.var() / lr.size)
+    assert tot < 4.0, tot
+
+
+def test_resume_is_bit_exact(gpu_ctx):
+    """8 iterations at once == 4 + checkpoint (readback/upload) + 4."""
+    print("case resume", flush=True)
+    spec = scenes.box_scene()
+    W, H = 32, 24
+    stage = Stage(spec, ctx=gpu_ctx)
+    a = Film(gpu_ctx, W, H)
+    trace(stage, a, "pt_nee", spp=8, max_depth=6, seed=9, batch=3)
+    b = Film(gpu_ctx, W, H)
+    trace(stage, b, "pt_nee", spp=4, max_depth=6, seed=9)
+    ck = b.readback()
+    c = Film(gpu_ctx, W, H)
+    c.upload(ck)
+    trace(stage, c, "pt_nee", spp=4, max_depth=6, seed=9, sample_offset=4)
+    assert np.array_equal(a.readback().view(np.uint32), c.readback().view(np.uint32))
+
+
+def test_exr_bytes_match_oracle(gpu_ctx, tmp_path):
+    print("case exr", flush=True)
+    spec = scenes.box_scene()
+    W, H = 20, 12
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    trace(stage, film, "pt", spp=2, max_depth=4, seed=3)
+    path = os.path.join(str(tmp_path), "out.exr")
+    film.write_exr(path)
+    data = open(path, "rb").read()
+    assert data == O.exr_encode(film.readback(), W, H)

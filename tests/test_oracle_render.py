@@ -1,0 +1,269 @@
+"""CPU tests pinning the oracle's render components.
+
+  * Mitchell filter (B = C = 1/3): closed-form values (src/core/math.rs:92-123).
+  * Film splat: a single sample's footprint and weights (src/render/film.rs:78-114).
+  * Camera: pinhole projection and the translate-then-rotate transform
+    (src/core/camera.rs:42-97).
+  * Lobes: closed forms at normal incidence, reciprocity, and sampling
+    consistency E[1/pdf] = 2 pi for the diffuse and GGX lobes; the GTR1
+    clearcoat pdf is pinned as NOT normalised (it omits cos theta_h,
+    src/material/util.rs:305-308).
+  * Integrators: a Lambertian floor under a quad light has a closed form
+    (polygon irradiance); PT and PT+NEE must converge to it.
+  * EXR: parsed back by an independent reader written from the OpenEXR
+    layout (magic, attributes, offset table, scanlines).
+"""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+from gammaray_amd import scenes
+from oracle import oracle as O
+
+
+def test_mitchell_closed_forms():
+    assert O.mitchell1(0.0) == pytest.approx((6.0 - 2.0 / 3.0) / 6.0, rel=1e-6)
+    assert O.mitchell1(0.5) == pytest.approx(1.0 / 18.0, rel=1e-5)
+    assert O.mitchell1(-0.5) == pytest.approx(1.0 / 18.0, rel=1e-5)
+    assert abs(O.mitchell1(1.0)) < 1e-6
+    # negative lobe between |2x| = 1 and 2
+    assert O.mitchell1(0.75) < 0.0
+
+
+def test_single_sample_splat():
+    W, H = 9, 7
+    film = np.zeros((W * H, 4), np.float32)
+    # sample exactly at the centre of pixel (4, 3): s, t such that col_cont = 4.5
+    s = np.float32(-1.0 + 2.0 * 4.5 / W)
+    t = np.float32(-1.0 + 2.0 * 3.5 / H)
+    film = O.report_samples(film, W, H, np.array([[s, t]], np.float32), np.array([[1, 2, 3]], np.float32))
+    w = film[:, 3].reshape(H, W)
+    # footprint: pixels within +-2 of (4, 3); weight 0 at distance 2
+    expect = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            if abs(x - 4) <= 2 and abs(y - 3) <= 2:
+                expect[y, x] = O.mitchell1((x - 4) / 2.0) * O.mitchell1((y - 3) / 2.0)
+    np.testing.assert_allclose(w, expect, atol=2e-6)
+    np.testing.assert_allclose(film[:, 1].reshape(H, W), 2 * expect, atol=4e-6)
+
+
+def test_camera_projection_and_transform():
+    cam = scenes.CameraSpec(focal_length=2.0, horizontal_aperture=1.0, vertical_aperture=0.5,
+                            translate=(1.0, 2.0, 3.0))
+    r = O.camera_rays(cam, np.array([[0.0, 0.0], [1.0, 1.0], [-1.0, 0.0]], np.float32))
+    np.testing.assert_allclose(r[:, :3], [[1, 2, 3]] * 3, atol=1e-6)
+    np.testing.assert_allclose(r[0, 3:], [0, 0, -1], atol=1e-7)
+    # window_max = aperture / (2 f): (0.25, 0.125) at s = t = 1
+    d = np.array([0.25, 0.125, -1.0])
+    np.testing.assert_allclose(r[1, 3:], d / np.linalg.norm(d), rtol=1e-6)
+    # rotation by -90 degrees about x turns the view direction -z into -y
+    c = math.cos(-math.pi / 4)
+    s = math.sin(-math.pi / 4)
+    cam2 = scenes.CameraSpec(focal_length=1.0, horizontal_aperture=1.0, vertical_aperture=1.0,
+                             rotate=(c, s, 0.0, 0.0))
+    r2 = O.camera_rays(cam2, np.array([[0.0, 0.0]], np.float32))
+    np.testing.assert_allclose(r2[0, 3:], [0, -1, 0], atol=1e-6)
+
+
+# --------------------------------------------------------------------------
+def _one_material_scene(rec):
+    s = scenes.SceneSpec()
+    s.add_sphere((0, 0, 0), 1.0, rec)
+    return s, O.OracleScene(s)
+
+
+def _items_local(wi, wo, u=None, c2l=1.0):
+    n = len(wi)
+    it = np.zeros((n, 24), np.float32)
+    it[:, 0:3] = wi
+    it[:, 3:6] = wo
+    it[:, 6:9] = [0, 0, 1]     # shading normal
+    it[:, 9:12] = [1, 0, 0]    # tangent
+    it[:, 12:15] = [0, 1, 0]   # binormal
+    it[:, 15:18] = [0, 0, 1]   # geometric normal
+    if u is not None:
+        it[:, 18] = u[:, 0].view(np.float32)
+        it[:, 19:21] = u[:, 1:3].view(np.float32)
+    it[:, 21] = c2l
+    return it
+
+
+def test_lobe_closed_forms_at_normal_incidence():
+    z = np.array([[0, 0, 1.0]], np.float32)
+    # Disney diffuse alone (ior 1 -> no specular lobe): f = base / pi
+    _, o = _one_material_scene(scenes.disney(base_color=(0.2, 0.4, 0.6), ior=1.0, roughness=0.7))
+    f = O.material_query(o, 0, _items_local(z, z))[0, 0:3]
+    np.testing.assert_allclose(f, np.array([0.2, 0.4, 0.6]) / math.pi, rtol=1e-6)
+    # metallic GGX alone: f = base * D(z) / 4 = base / (4 pi a^2), a = r^2
+    r = 0.5
+    _, o = _one_material_scene(scenes.disney(base_color=(0.9, 0.5, 0.1), metallic=1.0, roughness=r))
+    f = O.material_query(o, 0, _items_local(z, z))[0, 0:3]
+    a = r * r
+    np.testing.assert_allclose(f, np.array([0.9, 0.5, 0.1]) / (4 * math.pi * a * a), rtol=1e-5)
+    # PerfectDiffuse: 0.5 / pi; PerfectMirror pdf is reported as 1
+    _, o = _one_material_scene(scenes.diffuse())
+    q = O.material_query(o, 0, _items_local(z, z))[0]
+    np.testing.assert_allclose(q[0:3], 0.5 / math.pi, rtol=1e-6)
+    _, o = _one_material_scene(scenes.mirror())
+    q = O.material_query(o, 0, _items_local(z, z))[0]
+    assert q[3] == 1.0 and q[17] == 2 + 8  # SPECULAR | REFLECTION
+
+
+def _hemi_dirs(rng, n):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, 2] = np.abs(d[:, 2])
+    return d.astype(np.float32)
+
+
+@pytest.mark.parametrize("rec", [
+    scenes.disney(base_color=(0.8, 0.3, 0.5), roughness=0.6, sheen=0.7, ior=1.5),
+    scenes.disney(base_color=(0.8, 0.3, 0.5), roughness=0.3, metallic=1.0, anisotropic=0.6),
+    scenes.disney(base_color=(0.8, 0.3, 0.5), roughness=0.4, clearcoat=1.0, clearcoat_gloss=0.5),
+])
+def test_reflection_reciprocity(rec):
+    rng = np.random.default_rng(2)
+    _, o = _one_material_scene(rec)
+    a, b = _hemi_dirs(rng, 2000), _hemi_dirs(rng, 2000)
+    f1 = O.material_query(o, 0, _items_local(a, b))[:, 0:3]
+    f2 = O.material_query(o, 0, _items_local(b, a))[:, 0:3]
+    np.testing.assert_allclose(f1, f2, rtol=2e-4, atol=1e-6)
+
+
+def _mean_inv_pdf(rec, n=400000, seed=3):
+    rng = np.random.default_rng(seed)
+    _, o = _one_material_scene(rec)
+    wi = np.tile(np.array([[0.3, -0.2, 0.9327379]], np.float32), (n, 1))
+    u = np.zeros((n, 3), np.uint32)
+    u[:, 0] = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    u[:, 1:3] = rng.random((n, 2)).astype(np.float32).view(np.uint32)
+    q = O.material_query(o, 0, _items_local(wi, wi, u))
+    valid = q[:, 17] != 0
+    inv = np.where(valid, 1.0 / q[:, 16], 0.0)
+    return inv.mean(), inv.std() / math.sqrt(n)
+
+
+def test_sampling_density_matches_pdf_diffuse_and_ggx():
+    # E[1/pdf(o)] over sampled o equals the measure of the support (2 pi)
+    # when the reported pdf is the sampling density.
+    for rec in (scenes.disney(ior=1.0), scenes.disney(metallic=1.0, roughness=0.5)):
+        m, se = _mean_inv_pdf(rec)
+        assert abs(m - 2 * math.pi) < max(4 * se, 0.02 * 2 * math.pi), (m, se)
+
+
+def test_clearcoat_pdf_is_not_normalised():
+    # GTR1's pdf is D(h) without the cos(theta_h) and Jacobian terms
+    # (src/material/util.rs:305-308): E[1/pdf] differs from 2 pi.
+    m, se = _mean_inv_pdf(scenes.disney(metallic=1.0, ior=1.0, clearcoat=1.0, clearcoat_gloss=0.5))
+    assert abs(m - 2 * math.pi) > 0.1 * 2 * math.pi
+
+
+# --------------------------------------------------------------------------
+def _polygon_irradiance(p, n, verts, le):
+    """Irradiance at p (normal n) from a Lambertian polygon of radiance le
+    (Lambert's polygon formula)."""
+    v = verts - p
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    e = 0.0
+    for i in range(len(v)):
+        a, b = v[i], v[(i + 1) % len(v)]
+        c = np.cross(a, b)
+        theta = math.acos(np.clip(np.dot(a, b), -1, 1))
+        e += theta * np.dot(n, c / np.linalg.norm(c))
+    return abs(e) * 0.5 * le
+
+
+def _floor_scene():
+    s = scenes.SceneSpec(name="floor")
+    fv, ff = scenes.quad((0.0, 0.0, 0.0), (20.0, 0.0, 0.0), (0.0, 0.0, 20.0), (0.0, 1.0, 0.0))
+    s.add_mesh(fv, ff, scenes.diffuse())
+    lv, lf = scenes.quad((0.3, 2.0, -0.2), (0.5, 0.0, 0.0), (0.0, 0.0, 0.4), (0.0, -1.0, 0.0))
+    s.add_mesh(lv, lf, scenes.diffuse_light((10.0, 10.0, 10.0)))
+    c, sn = math.cos(-math.pi / 4), math.sin(-math.pi / 4)
+    # xform = translation * rotation in row-vector form: the translation is
+    # rotated too (src/core/camera.rs:51-53), so (0, 0, 1.5) lands at (0, 1.5, 0).
+    s.camera = scenes.CameraSpec(focal_length=1.0, horizontal_aperture=0.4, vertical_aperture=0.4,
+                                 rotate=(c, sn, 0.0, 0.0), translate=(0.0, 0.0, 1.5))
+    return s, lv
+
+
+@pytest.mark.parametrize("integ", [O.PT, O.PT_NEE])
+def test_lambert_floor_under_quad_light(integ):
+    s, light = _floor_scene()
+    o = O.OracleScene(s)
+    W = H = 8
+    spp = 2000 if integ == O.PT else 300
+    _, col = O.render(o, s, W, H, integ, spp=spp, max_depth=2, seed=7, colors=True)
+    expect = np.zeros((spp, W * H))
+    for it in range(spp):
+        st = O.sample_points(7, it, W, H)
+        rays = O.camera_rays(s.camera, st)
+        t = -rays[:, 1] / rays[:, 4]
+        p = rays[:, :3] + rays[:, 3:] * t[:, None]
+        for k in range(W * H):
+            expect[it, k] = 0.5 / math.pi * _polygon_irradiance(p[k], np.array([0, 1.0, 0]), light, 10.0)
+        if it >= 40:  # irradiance varies slowly; reuse the mean expectation
+            expect[it + 1:] = expect[: it + 1].mean(0)
+            break
+    got = col[:, :, 0]
+    err = (got - expect).mean()
+    se = got.std() / math.sqrt(got.size)
+    assert abs(err) < 4 * se + 1e-3 * expect.mean(), (got.mean(), expect.mean(), se)
+
+
+# --------------------------------------------------------------------------
+def _read_exr(data):
+    """Minimal reader of an uncompressed scanline OpenEXR file."""
+    magic, version = struct.unpack_from("<ii", data, 0)
+    assert magic == 20000630 and version & 0xFF == 2
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        end = data.index(b"\0", pos)
+        name = data[pos:end].decode()
+        pos = end + 1
+        end = data.index(b"\0", pos)
+        typ = data[pos:end].decode()
+        pos = end + 1
+        (size,) = struct.unpack_from("<i", data, pos)
+        pos += 4
+        attrs[name] = (typ, data[pos:pos + size])
+        pos += size
+    pos += 1
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    chl = attrs["channels"][1]
+    names, q = [], 0
+    while chl[q] != 0:
+        e = chl.index(b"\0", q)
+        names.append(chl[q:e].decode())
+        ptype = struct.unpack_from("<i", chl, e + 1)[0]
+        assert ptype == 2
+        q = e + 1 + 16
+    offsets = struct.unpack_from("<%dQ" % h, data, pos)
+    img = {}
+    for yi, off in enumerate(offsets):
+        y, nbytes = struct.unpack_from("<iI", data, off)
+        assert y == yi and nbytes == 4 * w * len(names)
+        for ci, nm in enumerate(names):
+            img.setdefault(nm, np.zeros((h, w), np.float32))[y] = np.frombuffer(
+                data, np.float32, w, off + 8 + 4 * w * ci)
+    return attrs, names, img
+
+
+def test_exr_layout_reads_back():
+    W, H = 3, 2
+    rng = np.random.default_rng(1)
+    film = rng.random((H * W, 4)).astype(np.float32) + 0.5
+    data = O.exr_encode(film, W, H)
+    attrs, names, img = _read_exr(data)
+    assert names == ["B", "G", "R"]
+    assert attrs["compression"][1] == b"\0" and attrs["lineOrder"][1] == b"\0"
+    assert struct.unpack("<f", attrs["screenWindowWidth"][1])[0] == float(W)
+    rgb = (film[:, :3] / film[:, 3:]).reshape(H, W, 3)
+    # rows are flipped: file line y is film row H - 1 - y (src/render/exr.rs:150)
+    np.testing.assert_array_equal(img["R"], rgb[::-1, :, 0])
+    np.testing.assert_array_equal(img["G"], rgb[::-1, :, 1])
+    np.testing.assert_array_equal(img["B"], rgb[::-1, :, 2])
