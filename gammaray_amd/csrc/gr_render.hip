@@ -675,6 +675,244 @@ __global__ void k_material_query(ShadeView sh, uint32_t prim, const float* in, u
     r[18] = (float)dev::count_lobes(m, kLobeDiffuse | kLobeGlossy); r[19] = 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// BDPT (BdptIntegrator, src/render/integrators.rs:102-418)
+// ---------------------------------------------------------------------------
+// A subpath vertex (BdptVertex, integrators.rs:107-130) in 7 float4: the
+// fields the estimator reads.  pdf_forward / pdf_reverse are tracked by the
+// reference but never enter its estimate (uniform strategy weights), so they
+// are not stored.
+struct BVert {
+    float4 a;  // incoming.xyz, prim (bits; kNone = escaped)
+    float4 b;  // point.xyz, connectible (bits)
+    float4 c;  // shading normal.xyz, throughput.x
+    float4 d;  // tangent.xyz, throughput.y
+    float4 e;  // binormal.xyz, throughput.z
+    float4 f;  // geometric normal.xyz, emission.x
+    float4 g;  // emission.y, emission.z
+};
+
+struct BdptArgs {
+    BVert* cam;   // [cmax][npaths]
+    BVert* lig;   // [lmax][npaths]
+    uint32_t* nc;
+    uint32_t* nl;        // light vertex count; bit 31 set = the light sample failed
+    uint32_t* mask;      // visibility bits per path, mwords words each
+    uint2* entry;        // shadow queue: (path, strategy)
+    uint32_t cmax, lmax, mwords;
+};
+
+constexpr uint32_t kBdptMaxDepth = 16;    // BDPT_MAX_DEPTH, integrators.rs:103
+constexpr uint32_t kLightFailed = 0x80000000u;
+
+__device__ __forceinline__ void store_vert(BVert* v, vec3 wi, uint32_t prim, vec3 pt, bool conn, const Frame& f,
+                                           vec3 T, vec3 le) {
+    v->a = make_float4(wi.x, wi.y, wi.z, __uint_as_float(prim));
+    v->b = make_float4(pt.x, pt.y, pt.z, __uint_as_float(conn ? 1u : 0u));
+    v->c = make_float4(f.n.x, f.n.y, f.n.z, T.x);
+    v->d = make_float4(f.t.x, f.t.y, f.t.z, T.y);
+    v->e = make_float4(f.b.x, f.b.y, f.b.z, T.z);
+    v->f = make_float4(f.gn.x, f.gn.y, f.gn.z, le.x);
+    v->g = make_float4(le.y, le.z, 0.0f, 0.0f);
+}
+
+struct VertView {
+    vec3 wi, pt, T, le;
+    Frame f;
+    uint32_t prim;
+    bool conn;
+};
+
+__device__ __forceinline__ VertView load_vert(const BVert* v) {
+    VertView o;
+    const float4 a = v->a, b = v->b, c = v->c, d = v->d, e = v->e, f = v->f, g = v->g;
+    o.wi = mk(a.x, a.y, a.z);
+    o.prim = __float_as_uint(a.w);
+    o.pt = mk(b.x, b.y, b.z);
+    o.conn = __float_as_uint(b.w) != 0u;
+    o.f.n = mk(c.x, c.y, c.z);
+    o.f.t = mk(d.x, d.y, d.z);
+    o.f.b = mk(e.x, e.y, e.z);
+    o.f.gn = mk(f.x, f.y, f.z);
+    o.T = mk(c.w, d.w, e.w);
+    o.le = mk(f.w, g.x, g.y);
+    return o;
+}
+
+// correct_shading_normal, integrators.rs:232-251.
+__device__ __forceinline__ float correct_shading_normal(vec3 wi, vec3 wo, const Frame& f, bool c2l) {
+    if (c2l) return 1.0f;
+    const float num = fabsf(dot(wi, f.n)) * fabsf(dot(wo, f.gn));
+    const float den = fabsf(dot(wi, f.gn)) * fabsf(dot(wo, f.n));
+    return den == 0.0f ? 0.0f : num / den;
+}
+
+// convert_density, integrators.rs:253-266.
+__device__ __forceinline__ float convert_density(vec3 prev, vec3 next, const Frame& nf) {
+    const float dist = length(next - prev);
+    if (dist == 0.0f) return 0.0f;
+    const vec3 dir = (next - prev) / dist;
+    return fabsf(dot(nf.n, dir)) / (dist * dist);
+}
+
+// One step of random_walk (integrators.rs:138-229) for every queued subpath.
+template <bool CAMERA>
+__global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t k, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.rb.counts[cur];
+    bool alive = false;
+    uint32_t p = 0;
+    vec3 new_o = mk(0.0f, 0.0f, 0.0f), new_d = new_o;
+    if (i < n) {
+        p = a.rb.queue[cur][i];
+        const uint4 hv = a.rb.hits[i];
+        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
+        const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
+        vec3 T = ld3(a.rb.T[p]);
+        BVert* v = (CAMERA ? b.cam : b.lig) + (size_t)k * a.npaths + p;
+        if (h.prim != kNone) {
+            const vec3 prev_T = T;
+            const vec3 hp = o + d * h.t;
+            const vec3 wi = -d;
+            const Frame f = hit_frame(a.sv, a.sh, h, o, d);
+            const GpuMaterial& m = a.sh.mats[h.prim];
+            const uint4 r0 = draws(a.seed, a.path_base + p, CAMERA ? kStreamCamera : kStreamLight, k, 0);
+            const dev::MatSample smp = dev::sample_world(m, wi, f, CAMERA, r0.x, u01(r0.y), u01(r0.z));
+            const bool conn = dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0;
+            T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
+            T = T * correct_shading_normal(wi, smp.outgoing, f, CAMERA);
+            new_o = hp + smp.outgoing * kRayPush;
+            new_d = smp.outgoing;
+            store_vert(v, wi, h.prim, hp, conn, f, prev_T, smp.emission);
+            if (k + 1 >= 4 || dev::nearly_zero3(T)) {
+                const float live = 0.25f + dev::clamp_unit(dev::luminance(T)) * (0.75f - 0.25f);
+                if (u01(r0.w) < live) T = T / live;
+                else T = mk(0.0f, 0.0f, 0.0f);
+            }
+            alive = !dev::is_zero(T) && k + 1 < (CAMERA ? b.cmax : b.lmax);
+        } else {
+            // the escaped ray: a vertex that never connects (integrators.rs:210-226)
+            Frame z;
+            z.n = z.t = z.b = z.gn = mk(0.0f, 0.0f, 0.0f);
+            T = mk(0.0f, 0.0f, 0.0f);
+            store_vert(v, -d, kNone, mk(0.0f, 0.0f, 0.0f), false, z, T, T);
+        }
+        a.rb.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
+        if (CAMERA) b.nc[p] = k + 1;
+        else b.nl[p] = k + 1;
+    }
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t slot = wave_append(alive, &a.rb.counts[nxt]);
+    if (alive) {
+        a.rb.queue[nxt][slot] = p;
+        a.rb.rays[nxt][2 * (size_t)slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+        a.rb.rays[nxt][2 * (size_t)slot + 1] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+    }
+}
+
+// Light sample and the light vertex (integrators.rs:373-395): Bvh::sample_light
+// with Prim::sample_ray_world's cosine direction (src/geom/prim.rs:37-51).
+__global__ __launch_bounds__(256) void k_light_start(RenderArgs a, BdptArgs b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool alive = false;
+    vec3 org = mk(0.0f, 0.0f, 0.0f), dir = org;
+    if (p < a.npaths) {
+        const uint64_t path = a.path_base + p;
+        const LightPoint lp = sample_light_point(a.sh, draws(a.seed, path, kStreamLight, 0, 1));
+        const uint4 r2 = draws(a.seed, path, kStreamLight, 0, 2);
+        const vec3 dl = dev::cosine_hemisphere(u01(r2.x), u01(r2.y), false);
+        const float dir_pdf = fabsf(dl.z) * kInvPi;
+        vec3 t, bn;
+        dev::coord_system(lp.f.gn, &t, &bn);
+        const vec3 gn = lp.f.gn;
+        dir = mk(t.x * dl.x + bn.x * dl.y + gn.x * dl.z, t.y * dl.x + bn.y * dl.y + gn.y * dl.z,
+                 t.z * dl.x + bn.z * dl.y + gn.z * dl.z);
+        if (lp.point_pdf == 0.0f || dir_pdf == 0.0f) {
+            b.nl[p] = kLightFailed;
+        } else {
+            const vec3 init = dev::light_world(a.sh.mats[lp.prim], dir, lp.f) *
+                              (fabsf(dot(gn, dir)) / (lp.point_pdf * dir_pdf));
+            const vec3 one = mk(1.0f, 1.0f, 1.0f);
+            store_vert(b.lig + p, mk(0.0f, 0.0f, 0.0f), lp.prim, lp.p, true, lp.f, one / lp.point_pdf,
+                       mk(0.0f, 0.0f, 0.0f));
+            b.nl[p] = 1;
+            a.rb.T[p] = make_float4(init.x, init.y, init.z, 0.0f);
+            org = lp.p + dir * kRayPush;
+            alive = !dev::is_zero(init) && 1 < b.lmax;
+        }
+        for (uint32_t w = 0; w < b.mwords; ++w) b.mask[(size_t)p * b.mwords + w] = 0u;
+    }
+    const uint32_t slot = wave_append(alive, &a.rb.counts[0]);
+    if (alive) {
+        a.rb.queue[0][slot] = p;
+        a.rb.rays[0][2 * (size_t)slot] = make_float4(org.x, org.y, org.z, 0.0f);
+        a.rb.rays[0][2 * (size_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    }
+}
+
+struct Connection {
+    vec3 contrib;
+    bool needs_vis;
+    vec3 from, to;
+};
+
+// BdptIntegrator::connect without the visibility test (integrators.rs:269-343).
+__device__ Connection connect(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l) {
+    Connection r;
+    r.contrib = mk(0.0f, 0.0f, 0.0f);
+    r.needs_vis = false;
+    const VertView cv = load_vert(b.cam + (size_t)(c - 1) * a.npaths + p);
+    if (l == 0) {
+        if (cv.prim != kNone) r.contrib = mul(cv.T, cv.le);
+        return r;
+    }
+    const VertView lv = load_vert(b.lig + (size_t)(l - 1) * a.npaths + p);
+    if (cv.prim == kNone || lv.prim == kNone) return r;
+    if (!cv.conn || !lv.conn) return r;
+    const vec3 c2l = normalized(lv.pt - cv.pt);
+    const vec3 l2c = -c2l;
+    const vec3 cr = dev::f_world(a.sh.mats[cv.prim], cv.wi, c2l, cv.f, true);
+    const vec3 ce = l == 1 ? dev::light_world(a.sh.mats[lv.prim], l2c, lv.f)
+                           : dev::f_world(a.sh.mats[lv.prim], lv.wi, l2c, lv.f, false);
+    const float g = fabsf(dot(cv.f.n, c2l));
+    const float cd = convert_density(cv.pt, lv.pt, lv.f);
+    r.contrib = mul(mul(mul(cv.T, cr), ce), lv.T) * (g * cd);
+    r.needs_vis = !dev::nearly_zero3(r.contrib);
+    r.from = cv.pt;
+    r.to = lv.pt;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_vis_scatter(RenderArgs a, BdptArgs b) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.rb.counts[2] || a.rb.vis[j] == 0) return;
+    const uint2 e = b.entry[j];
+    atomicOr(&b.mask[(size_t)e.x * b.mwords + e.y / 32], 1u << (e.y % 32));
+}
+
+// Sums the strategies in the reference's order (integrators.rs:401-412).
+__global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npaths) return;
+    uint32_t nc = b.nc[p];
+    const uint32_t nl = b.nl[p];
+    vec3 L = mk(0.0f, 0.0f, 0.0f);
+    if (nl & kLightFailed) nc = 0;
+    for (uint32_t c = 1; c < nc + 1; ++c) {
+        for (uint32_t l = 0; l < nl + 1; ++l) {
+            if (c + l > kBdptMaxDepth) continue;
+            Connection cn = connect(a, b, p, c, l);
+            if (l > 0 && cn.needs_vis) {
+                const uint32_t s = (c - 1) * (b.lmax + 1) + l;
+                if (!(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) cn.contrib = mk(0.0f, 0.0f, 0.0f);
+            }
+            const float w = 1.0f / (float)(c + l);
+            L = L + cn.contrib * w;
+        }
+    }
+    a.rb.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
 inline dim3 blocks_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // Grows the film's wavefront buffers to hold `paths` paths.
@@ -721,6 +959,131 @@ hipError_t reserve(RenderBuffers& rb, size_t paths) {
     return hipSuccess;
 }
 
+// BDPT buffers: vertices for cmax camera and lmax light vertices per path,
+// counts, visibility masks, and a shadow queue of `shadow` entries.
+hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_t lmax, size_t shadow,
+                        BdptArgs* b, float4** seg, uint32_t** vis) {
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const uint32_t mwords = (cmax * (lmax + 1) + 31) / 32;
+    size_t off = 0;
+    const size_t o_cam = off; off = align(off + (size_t)cmax * npaths * sizeof(BVert));
+    const size_t o_lig = off; off = align(off + (size_t)lmax * npaths * sizeof(BVert));
+    const size_t o_nc = off; off = align(off + npaths * 4);
+    const size_t o_nl = off; off = align(off + npaths * 4);
+    const size_t o_mask = off; off = align(off + npaths * mwords * 4);
+    const size_t o_entry = off; off = align(off + shadow * sizeof(uint2));
+    const size_t o_seg = off; off = align(off + shadow * 32);
+    const size_t o_vis = off; off = align(off + shadow * 4);
+    if (rb.bdpt_capacity < off) {
+        (void)hipFree(rb.bdpt_blob);
+        rb.bdpt_blob = nullptr;
+        rb.bdpt_capacity = 0;
+        hipError_t e = hipMalloc(&rb.bdpt_blob, off);
+        if (e != hipSuccess) return e;
+        rb.bdpt_capacity = off;
+    }
+    char* base = static_cast<char*>(rb.bdpt_blob);
+    b->cam = reinterpret_cast<BVert*>(base + o_cam);
+    b->lig = reinterpret_cast<BVert*>(base + o_lig);
+    b->nc = reinterpret_cast<uint32_t*>(base + o_nc);
+    b->nl = reinterpret_cast<uint32_t*>(base + o_nl);
+    b->mask = reinterpret_cast<uint32_t*>(base + o_mask);
+    b->entry = reinterpret_cast<uint2*>(base + o_entry);
+    b->cmax = cmax;
+    b->lmax = lmax;
+    b->mwords = mwords;
+    *seg = reinterpret_cast<float4*>(base + o_seg);
+    *vis = reinterpret_cast<uint32_t*>(base + o_vis);
+    return hipSuccess;
+}
+
+}  // namespace
+
+namespace {
+// Emits the shadow segment of every strategy of paths [p0, p1) that needs a
+// visibility test (wave-compacted).
+__global__ __launch_bounds__(256) void k_connect_emit_range(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+    const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nc = 0, nl = 0;
+    if (p < p1) {
+        nc = b.nc[p];
+        nl = b.nl[p];
+        if (nl & kLightFailed) nc = 0;
+    }
+    const uint32_t total = nc * (nl + 1);
+    for (uint32_t s = 0; __ballot(s < total) != 0; ++s) {
+        bool emit = false;
+        Connection cn;
+        uint32_t c = 0, l = 0;
+        if (s < total) {
+            c = s / (nl + 1) + 1;
+            l = s % (nl + 1);
+            if (c + l <= kBdptMaxDepth) {
+                cn = connect(a, b, p, c, l);
+                emit = l > 0 && cn.needs_vis;
+            }
+        }
+        const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
+        if (emit) {
+            a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
+            a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
+            b.entry[slot] = make_uint2(p, (c - 1) * (b.lmax + 1) + l);
+        }
+    }
+}
+
+// BdptIntegrator::integrate for every path of one batch, then the film splat.
+hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb,
+                      float4* sample_out, hipStream_t s) {
+    const size_t npaths = a.npaths;
+    const size_t per_path_shadow = (size_t)cmax * lmax;
+    // shadow entries per connection chunk: at most ~2^26 (2.75 GiB of queue)
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 26) / per_path_shadow));
+    const size_t shadow = chunk * per_path_shadow;
+    BdptArgs b;
+    float4* seg = nullptr;
+    uint32_t* vis = nullptr;
+    hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, &b, &seg, &vis);
+    if (e != hipSuccess) return e;
+    a.rb.seg = seg;
+    a.rb.vis = vis;
+    uint32_t* cnt = film->rb.counts;
+    hipLaunchKernelGGL(k_camera, blocks_for(npaths), dim3(256), 0, s, a);
+    uint32_t cur = 0;
+    for (uint32_t k = 0; k < cmax && e == hipSuccess; ++k) {
+        e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths,
+                                   cnt + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
+        hipLaunchKernelGGL(k_walk<true>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
+        cur ^= 1u;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 0u);
+    hipLaunchKernelGGL(k_light_start, blocks_for(npaths), dim3(256), 0, s, a, b);
+    cur = 0;
+    for (uint32_t k = 1; k < lmax && e == hipSuccess; ++k) {
+        e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths,
+                                   cnt + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
+        hipLaunchKernelGGL(k_walk<false>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
+        cur ^= 1u;
+    }
+    for (size_t p0 = 0; p0 < npaths && e == hipSuccess; p0 += chunk) {
+        const size_t p1 = std::min(npaths, p0 + chunk);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 2u);
+        hipLaunchKernelGGL(k_connect_emit_range, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0,
+                           (uint32_t)p1);
+        e = launch_occluded_queue(sc->view, reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
+                                  vis, s);
+        hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_connect_sum, blocks_for(npaths), dim3(256), 0, s, a, b);
+    if (sample_out) hipLaunchKernelGGL(k_export_samples, blocks_for(npaths), dim3(256), 0, s, a, sample_out);
+    const dim3 gather_grid((a.width + kTile - 1) / kTile, (a.height + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb);
+    return hipGetLastError();
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -750,7 +1113,24 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     a.wy = cam.vertical_aperture / (cam.focal_length * 2.0f);
     a.rb = film->rb;
     const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
-    if (prm.integrator == GR_INTEGRATOR_BDPT) return hipErrorNotSupported;
+    if (prm.integrator == GR_INTEGRATOR_BDPT) {
+        const uint32_t cmax = (prm.max_depth == 0 || prm.max_depth > kBdptMaxDepth) ? kBdptMaxDepth : prm.max_depth;
+        const uint32_t lmax =
+            (prm.light_depth == 0 || prm.light_depth + 1 > kBdptMaxDepth) ? kBdptMaxDepth : prm.light_depth + 1;
+        // vertex storage bounded to ~8 GiB per batch
+        const size_t per_iter = npx * (cmax + lmax) * sizeof(BVert);
+        uint32_t bb = (uint32_t)std::max<size_t>(1, ((size_t)8 << 30) / per_iter);
+        bb = std::min(bb, batch);
+        for (uint32_t done = 0; done < prm.spp && e == hipSuccess; done += bb) {
+            const uint32_t nb = std::min(bb, prm.spp - done);
+            a.npaths = (uint32_t)(npx * nb);
+            a.path_base = (prm.sample_offset + done) * (uint64_t)npx;
+            float4* out = prm.d_sample_radiance ? reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx
+                                                : nullptr;
+            e = bdpt_batch(sc, a, film, cmax, lmax, nb, out, s);
+        }
+        return e;
+    }
     const dim3 gather_grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
 
     for (uint32_t done = 0; done < prm.spp && e == hipSuccess; done += batch) {

@@ -105,7 +105,7 @@ def test_material_functions_match_oracle(gpu_ctx):
         assert np.mean(g[:, 17] == r[:, 17]) > 0.999
 
 
-@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE)])
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
 def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
     """Same seed -> the same paths: per-sample radiance agrees."""
     print("case paired", integrator, flush=True)
@@ -113,16 +113,16 @@ def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
     W, H, spp = 48, 40, 4
     stage = Stage(spec, ctx=gpu_ctx)
     film = Film(gpu_ctx, W, H)
-    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, seed=21)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, light_depth=5, seed=21)
     o = O.OracleScene(spec)
-    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, seed=21, colors=True)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, light_depth=5, seed=21, colors=True)
     close = np.all(np.isclose(g, r, rtol=1e-3, atol=1e-5), axis=2)
     assert close.mean() > 0.99, close.mean()
     np.testing.assert_allclose(g.reshape(-1, 3).mean(0), r.reshape(-1, 3).mean(0), rtol=2e-2)
     assert np.isfinite(g).all()
 
 
-@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE)])
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
 def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
     """Per-pixel means of GPU and oracle (different seeds) within 3 sigma."""
     print("case 3sigma", integrator, flush=True)
@@ -130,9 +130,9 @@ def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
     W, H, spp = 32, 32, 128
     stage = Stage(spec, ctx=gpu_ctx)
     film = Film(gpu_ctx, W, H)
-    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, seed=101)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=6, light_depth=5, seed=101)
     o = O.OracleScene(spec)
-    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, seed=202, colors=True)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, light_depth=5, seed=202, colors=True)
     lg, lr = g.sum(2), r.sum(2)  # (spp, pixels)
     mg, mr = lg.mean(0), lr.mean(0)
     se = np.sqrt(lg.var(0, ddof=1) / spp + lr.var(0, ddof=1) / spp)

@@ -267,3 +267,25 @@ def test_exr_layout_reads_back():
     np.testing.assert_array_equal(img["R"], rgb[::-1, :, 0])
     np.testing.assert_array_equal(img["G"], rgb[::-1, :, 1])
     np.testing.assert_array_equal(img["B"], rgb[::-1, :, 2])
+
+
+def _lambert_box():
+    s = scenes.box_scene()
+    for p in s.prims:
+        if p.material[0] == 0:
+            p.material = scenes.diffuse()
+    return s
+
+
+def test_bdpt_is_unbiased_against_unbounded_pt():
+    """With only Lambertian (connectible) vertices every strategy of a path up
+    to 16 vertices exists, so the reference's uniform 1/(c+l) weights make
+    BDPT an unbiased estimator of the same image as the unbounded PT."""
+    s = _lambert_box()
+    o = O.OracleScene(s)
+    W = H = 16
+    _, a = O.render(o, s, W, H, O.PT, spp=256, seed=1, colors=True)
+    _, b = O.render(o, s, W, H, O.BDPT, spp=64, seed=2, colors=True)
+    la, lb = a.sum(2).ravel(), b.sum(2).ravel()
+    z = abs(la.mean() - lb.mean()) / math.sqrt(la.var() / la.size + lb.var() / lb.size)
+    assert z < 4.0, (la.mean(), lb.mean(), z)
