@@ -26,6 +26,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 1 << 29  # 512 MiB, read once and written once
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
 NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
 
@@ -59,6 +60,61 @@ def cpu_baseline(spec, rays8, segs8, sample, threads):
                        "closest %.2f s, shadow %.2f s)" % (sample, sample, threads, t1 - t0, t2 - t1))
 
 
+PATH_WORKLOADS = [
+    # name, scene factory, width, height, spp, max_depth, integrator, light_depth, cpu spp
+    ("box256_pt", "box_scene", 256, 256, 16, 6, "pt", 0, 4),
+    ("showcase1080_pt_nee", "showcase_scene", 1920, 1080, 64, 8, "pt_nee", 0, 1),
+    ("showcase1080_bdpt", "showcase_scene", 1920, 1080, 64, 8, "bdpt", 5, 1),
+]
+
+
+def bench_paths(ctx, stream, cpu, threads, only=None):
+    """Path-tracing throughput (Mpaths/s, one camera sample = one path) of the
+    render workloads, GPU time by stream events around gr_trace, plus the
+    oracle's throughput on a bounded number of iterations of the same frame."""
+    import torch
+    from gammaray_amd import scenes
+    from gammaray_amd.api import Film, Stage, trace
+    out = {}
+    for name, factory, W, H, spp, depth, integ, ldepth, cpu_spp in PATH_WORKLOADS:
+        if only and name not in only:
+            continue
+        spec = getattr(scenes, factory)()
+        stage = Stage(spec, ctx=ctx)
+        film = Film(ctx, W, H)
+        trace(stage, film, integ, spp=1, max_depth=depth, light_depth=ldepth, seed=1)  # warm
+        torch.cuda.synchronize()
+        film.clear()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        trace(stage, film, integ, spp=spp, max_depth=depth, light_depth=ldepth, seed=7)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        rec = {"mpaths_s": round(W * H * spp / (ms * 1e-3) / 1e6, 2), "ms": round(ms, 3), "width": W,
+               "height": H, "spp": spp, "max_depth": depth, "integrator": integ,
+               "primitives": spec.num_components}
+        if ldepth:
+            rec["light_depth"] = ldepth
+        if cpu:
+            from oracle import oracle as O
+            o = O.OracleScene(spec)
+            oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT}[integ]
+            t0 = time.perf_counter()
+            O.render(o, spec, W, H, oi, spp=cpu_spp, max_depth=depth, light_depth=ldepth, seed=7,
+                     threads=threads)
+            dt = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"mpaths_s": round(W * H * cpu_spp / dt / 1e6, 4), "cores": threads,
+                                   "kind": "port", "sample": "%d iteration(s) of the same frame" % cpu_spp}
+            rec["gpu_over_cpu"] = round(rec["mpaths_s"] / rec["cpu_baseline"]["mpaths_s"], 1)
+        log("%s: %.1f Mpaths/s (%.2f ms)%s" % (name, rec["mpaths_s"], ms,
+                                              ", cpu %.3f Mpaths/s" % rec["cpu_baseline"]["mpaths_s"] if cpu else ""))
+        out[name] = rec
+        film.close()
+        stage.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +125,10 @@ def main():
     ap.add_argument("--occlusion", choices=["exact", "anyhit"], default="exact")
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for counter runs: one %d MiB device copy (calibration), then one launch "
+                         "of each traversal kernel; no timing, no JSON" % (COPY_PROBE_BYTES >> 20))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the film/"
@@ -103,6 +163,18 @@ def main():
     segs = torch.from_numpy(segs8).to(dev)
     hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
     vis = torch.empty((n,), dtype=torch.int32, device=dev)
+
+    if args.traffic_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        stage.intersect_ptr(rays.data_ptr(), n, hits.data_ptr())
+        stage.occluded_ptr(segs.data_ptr(), n, vis.data_ptr(), mode)
+        torch.cuda.synchronize()
+        log("traffic probe done")
+        return
 
     # Work counts of the reference-order traversal (untimed pass).
     _, cnt_c = stage.intersect(rays, counts=True)
@@ -157,10 +229,14 @@ def main():
                     shadow_kernel_ms=round(ms_shadow, 4))
 
     cpu = None
+    threads = max(1, min(16, os.cpu_count() or 1))
     if not args.no_cpu_baseline:
-        threads = max(1, min(16, os.cpu_count() or 1))
         cpu = cpu_baseline(spec, rays8, segs8, min(args.cpu_sample, n), threads)
         log("cpu baseline %.4f Grays/s on %d threads" % (cpu["value"], threads))
+    del rays, segs, hits, vis, rays8, segs8
+    paths = None
+    if not args.no_paths:
+        paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads)
 
     out = {
         "metric": "Grays/s (closest-hit + shadow rays, incoherent, 1 GPU)",
@@ -181,6 +257,7 @@ def main():
                    "bvh_depth": info["max_depth"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "paths": paths,
     }
     print(json.dumps(out), flush=True)
 

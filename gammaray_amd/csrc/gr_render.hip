@@ -699,6 +699,7 @@ struct BdptArgs {
     uint32_t* nl;        // light vertex count; bit 31 set = the light sample failed
     uint32_t* mask;      // visibility bits per path, mwords words each
     uint2* entry;        // shadow queue: (path, strategy)
+    float4* slots;       // per (strategy, path): contribution * weight, needs-visibility flag
     uint32_t cmax, lmax, mwords;
 };
 
@@ -890,10 +891,13 @@ __global__ __launch_bounds__(256) void k_vis_scatter(RenderArgs a, BdptArgs b) {
     atomicOr(&b.mask[(size_t)e.x * b.mwords + e.y / 32], 1u << (e.y % 32));
 }
 
-// Sums the strategies in the reference's order (integrators.rs:401-412).
-__global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npaths) return;
+// Sums the strategies of paths [p0, p1) in the reference's order
+// (integrators.rs:401-412): light += connect(c, l) * weight(c, l), with the
+// strategies whose visibility test failed contributing zero.
+__global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+    const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1) return;
+    const uint32_t chunk = p1 - p0, q = p - p0;
     uint32_t nc = b.nc[p];
     const uint32_t nl = b.nl[p];
     vec3 L = mk(0.0f, 0.0f, 0.0f);
@@ -901,13 +905,14 @@ __global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b) {
     for (uint32_t c = 1; c < nc + 1; ++c) {
         for (uint32_t l = 0; l < nl + 1; ++l) {
             if (c + l > kBdptMaxDepth) continue;
-            Connection cn = connect(a, b, p, c, l);
-            if (l > 0 && cn.needs_vis) {
-                const uint32_t s = (c - 1) * (b.lmax + 1) + l;
-                if (!(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) cn.contrib = mk(0.0f, 0.0f, 0.0f);
+            const uint32_t s = (c - 1) * (b.lmax + 1) + l;
+            const float4 v = b.slots[(size_t)s * chunk + q];
+            vec3 cw = mk(v.x, v.y, v.z);
+            if (v.w != 0.0f && !(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) {
+                const float w = 1.0f / (float)(c + l);
+                cw = mk(0.0f, 0.0f, 0.0f) * w;
             }
-            const float w = 1.0f / (float)(c + l);
-            L = L + cn.contrib * w;
+            L = L + cw;
         }
     }
     a.rb.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
@@ -962,7 +967,7 @@ hipError_t reserve(RenderBuffers& rb, size_t paths) {
 // BDPT buffers: vertices for cmax camera and lmax light vertices per path,
 // counts, visibility masks, and a shadow queue of `shadow` entries.
 hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_t lmax, size_t shadow,
-                        BdptArgs* b, float4** seg, uint32_t** vis) {
+                        size_t nslots, BdptArgs* b, float4** seg, uint32_t** vis) {
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const uint32_t mwords = (cmax * (lmax + 1) + 31) / 32;
     size_t off = 0;
@@ -974,6 +979,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     const size_t o_entry = off; off = align(off + shadow * sizeof(uint2));
     const size_t o_seg = off; off = align(off + shadow * 32);
     const size_t o_vis = off; off = align(off + shadow * 4);
+    const size_t o_slots = off; off = align(off + nslots * sizeof(float4));
     if (rb.bdpt_capacity < off) {
         (void)hipFree(rb.bdpt_blob);
         rb.bdpt_blob = nullptr;
@@ -989,6 +995,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     b->nl = reinterpret_cast<uint32_t*>(base + o_nl);
     b->mask = reinterpret_cast<uint32_t*>(base + o_mask);
     b->entry = reinterpret_cast<uint2*>(base + o_entry);
+    b->slots = reinterpret_cast<float4*>(base + o_slots);
     b->cmax = cmax;
     b->lmax = lmax;
     b->mwords = mwords;
@@ -1000,35 +1007,35 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
 }  // namespace
 
 namespace {
-// Emits the shadow segment of every strategy of paths [p0, p1) that needs a
-// visibility test (wave-compacted).
-__global__ __launch_bounds__(256) void k_connect_emit_range(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
-    const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nc = 0, nl = 0;
-    if (p < p1) {
-        nc = b.nc[p];
-        nl = b.nl[p];
-        if (nl & kLightFailed) nc = 0;
+// One thread per (strategy, path) of paths [p0, p1), strategy-major so that a
+// wave reads the vertices of consecutive paths: evaluates connect(c, l) without
+// its visibility test, stores contribution * weight, and queues the shadow
+// segment of every strategy that needs one (wave-compacted).
+__global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+    const uint32_t chunk = p1 - p0;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nstrat = b.cmax * (b.lmax + 1);
+    bool emit = false;
+    uint32_t p = 0, s = 0;
+    Connection cn;
+    if (t < (size_t)nstrat * chunk) {
+        s = (uint32_t)(t / chunk);
+        p = p0 + (uint32_t)(t % chunk);
+        const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
+        const uint32_t nl = b.nl[p];
+        if (!(nl & kLightFailed) && c <= b.nc[p] && l <= nl && c + l <= kBdptMaxDepth) {
+            cn = connect(a, b, p, c, l);
+            const float w = 1.0f / (float)(c + l);
+            const vec3 cw = cn.contrib * w;
+            emit = l > 0 && cn.needs_vis;
+            b.slots[t] = make_float4(cw.x, cw.y, cw.z, emit ? 1.0f : 0.0f);
+        }
     }
-    const uint32_t total = nc * (nl + 1);
-    for (uint32_t s = 0; __ballot(s < total) != 0; ++s) {
-        bool emit = false;
-        Connection cn;
-        uint32_t c = 0, l = 0;
-        if (s < total) {
-            c = s / (nl + 1) + 1;
-            l = s % (nl + 1);
-            if (c + l <= kBdptMaxDepth) {
-                cn = connect(a, b, p, c, l);
-                emit = l > 0 && cn.needs_vis;
-            }
-        }
-        const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
-        if (emit) {
-            a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
-            a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
-            b.entry[slot] = make_uint2(p, (c - 1) * (b.lmax + 1) + l);
-        }
+    const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
+    if (emit) {
+        a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
+        a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
+        b.entry[slot] = make_uint2(p, s);
     }
 }
 
@@ -1036,14 +1043,14 @@ __global__ __launch_bounds__(256) void k_connect_emit_range(RenderArgs a, BdptAr
 hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb,
                       float4* sample_out, hipStream_t s) {
     const size_t npaths = a.npaths;
-    const size_t per_path_shadow = (size_t)cmax * lmax;
-    // shadow entries per connection chunk: at most ~2^26 (2.75 GiB of queue)
-    const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 26) / per_path_shadow));
-    const size_t shadow = chunk * per_path_shadow;
+    const size_t nstrat = (size_t)cmax * (lmax + 1);
+    // strategies per connection chunk: at most 2^25 (512 MiB of slots)
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 25) / nstrat));
+    const size_t shadow = chunk * cmax * lmax;
     BdptArgs b;
     float4* seg = nullptr;
     uint32_t* vis = nullptr;
-    hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, &b, &seg, &vis);
+    hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, chunk * nstrat, &b, &seg, &vis);
     if (e != hipSuccess) return e;
     a.rb.seg = seg;
     a.rb.vis = vis;
@@ -1071,14 +1078,14 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     for (size_t p0 = 0; p0 < npaths && e == hipSuccess; p0 += chunk) {
         const size_t p1 = std::min(npaths, p0 + chunk);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 2u);
-        hipLaunchKernelGGL(k_connect_emit_range, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0,
+        hipLaunchKernelGGL(k_connect_eval, blocks_for((p1 - p0) * nstrat), dim3(256), 0, s, a, b, (uint32_t)p0,
                            (uint32_t)p1);
         e = launch_occluded_queue(sc->view, reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
                                   vis, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(k_connect_sum, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_connect_sum, blocks_for(npaths), dim3(256), 0, s, a, b);
     if (sample_out) hipLaunchKernelGGL(k_export_samples, blocks_for(npaths), dim3(256), 0, s, a, sample_out);
     const dim3 gather_grid((a.width + kTile - 1) / kTile, (a.height + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb);
