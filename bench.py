@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
+    ap.add_argument("--tuning", default="auto",
+                    help="'auto' (autotune on the closest-hit batch before timing), 'default', or "
+                         "k=v pairs, e.g. sort_bits=18,lds_stack=16,waves_per_cu=0,refill=16")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: one %d MiB device copy (calibration), then one launch "
                          "of each traversal kernel; no timing, no JSON" % (COPY_PROBE_BYTES >> 20))
@@ -188,6 +191,17 @@ def main():
         cnt_c[:, 0].mean(), cnt_c[:, 1].mean(), cnt_s[:, 0].mean(), cnt_s[:, 1].mean()))
     del cnt_c, cnt_s
 
+    # Traversal settings: chosen before the timed region.
+    if args.tuning == "auto":
+        t0 = time.time()
+        tuning = stage.autotune(rays)
+        log("autotuned %s (%.1f s)" % (tuning, time.time() - t0))
+    elif args.tuning == "default":
+        tuning = stage.get_tuning()
+    else:
+        kv = dict(item.split("=") for item in args.tuning.split(","))
+        tuning = stage.set_tuning(**{k: int(v) for k, v in kv.items()})
+
     def step(ev=None):
         if ev is not None:
             ev[0].record(stream)
@@ -202,12 +216,16 @@ def main():
         step()
     torch.cuda.synchronize()
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    ctx.set_kernel_timing(True)
+    ctx.take_kernel_times()
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
         step(events[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    ktimes = ctx.take_kernel_times()
+    ctx.set_kernel_timing(False)
     if stage.take_overflow_count() != 0:
         log("warning: traversal stack overflow reported")
     ms_closest = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
@@ -216,17 +234,27 @@ def main():
     value = 2.0 * n / (ms_step * 1e-3) / 1e9
     log("closest %.3f ms, shadow %.3f ms, step %.3f ms" % (ms_closest, ms_shadow, ms_step))
 
-    dominant = "closest" if ms_closest >= ms_shadow else "shadow"
-    ms_dom = ms_closest if dominant == "closest" else ms_shadow
+    # Per-launch durations of the traversal kernels alone (HIP events on the
+    # stream they run on, recorded by the library around each launch).
+    def avg(kind):
+        ms, cnt = ktimes[kind]
+        return ms / max(cnt, 1)
+    k_closest, k_shadow = avg("trace_closest"), avg("trace_shadow")
+    sort_ms = ktimes["ray_sort"][0] / args.steps
+    log("kernels: closest %.3f ms, shadow %.3f ms, sort %.3f ms per step" % (k_closest, k_shadow, sort_ms))
+    dominant = "closest" if k_closest >= k_shadow else "shadow"
+    ms_dom = k_closest if dominant == "closest" else k_shadow
     bytes_dom = bytes_closest if dominant == "closest" else bytes_shadow
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_intersect" if dominant == "closest" else "k_occluded",
+                    kernel="k_trace<%s>" % ("closest" if dominant == "closest" else "occlusion"),
                     algorithmic_bytes_per_launch=int(bytes_dom),
                     kernel_ms=round(ms_dom, 4),
-                    shadow_kernel_gbs=round(bytes_shadow / (ms_shadow * 1e-3) / 1e9, 1),
-                    shadow_kernel_ms=round(ms_shadow, 4))
+                    shadow_kernel_gbs=round(bytes_shadow / (k_shadow * 1e-3) / 1e9, 1),
+                    shadow_kernel_ms=round(k_shadow, 4),
+                    sort_ms_per_step=round(sort_ms, 4),
+                    closest_call_ms=round(ms_closest, 4), shadow_call_ms=round(ms_shadow, 4))
 
     cpu = None
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -253,7 +281,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "incoherent ray cast, displaced icosphere L%d" % args.level,
                    "triangles": spec.num_components, "closest_rays": n, "shadow_segments": n,
-                   "occlusion_mode": args.occlusion, "bvh_nodes": info["num_nodes"],
+                   "occlusion_mode": args.occlusion, "tuning": tuning, "bvh_nodes": info["num_nodes"],
                    "bvh_depth": info["max_depth"]},
         "roofline": roofline,
         "cpu_baseline": cpu,

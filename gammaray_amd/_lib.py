@@ -37,6 +37,10 @@ class Camera(ctypes.Structure):
                 ("rotate", ctypes.c_float * 4), ("translate", ctypes.c_float * 3)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("sort_bits", _u32), ("lds_stack", _u32), ("waves_per_cu", _u32), ("refill", _u32)]
+
+
 class TraceParams(ctypes.Structure):
     _fields_ = [("integrator", _u32), ("spp", _u32), ("max_depth", _u32), ("light_depth", _u32),
                 ("seed", ctypes.c_uint64), ("sample_offset", ctypes.c_uint64), ("batch", _u32),
@@ -72,6 +76,11 @@ SIGNATURES = {
     "gr_intersect_closest": (_i, [_vp, _vp, _u32, _vp, _vp]),
     "gr_occluded": (_i, [_vp, _vp, _u32, _vp, _i, _vp]),
     "gr_scene_take_overflow_count": (_i, [_vp, _u64p]),
+    "gr_scene_set_tuning": (_i, [_vp, _vp]),
+    "gr_ctx_set_kernel_timing": (_i, [_vp, _i]),
+    "gr_ctx_take_kernel_times": (_i, [_vp, _vp, _vp]),
+    "gr_scene_get_tuning": (_i, [_vp, _vp]),
+    "gr_scene_autotune": (_i, [_vp, _vp, _u32, _vp, _vp]),
 }
 
 _lib = None

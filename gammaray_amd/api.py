@@ -45,6 +45,19 @@ class Context:
     def synchronize(self):
         check(lib().gr_ctx_synchronize(self._h), self._h, "synchronize")
 
+    KERNEL_KINDS = ("trace_closest", "trace_shadow", "ray_sort")
+
+    def set_kernel_timing(self, enable=True):
+        check(lib().gr_ctx_set_kernel_timing(self._h, 1 if enable else 0), self._h, "set_kernel_timing")
+
+    def take_kernel_times(self):
+        """{kind: (summed ms, launches)} since the previous call (HIP events on
+        the context's stream around each traversal / sort launch)."""
+        ms = (ctypes.c_double * len(self.KERNEL_KINDS))()
+        cnt = (ctypes.c_uint64 * len(self.KERNEL_KINDS))()
+        check(lib().gr_ctx_take_kernel_times(self._h, ms, cnt), self._h, "take_kernel_times")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_KINDS)}
+
     def close(self):
         if self._h:
             lib().gr_ctx_destroy(self._h)
@@ -137,6 +150,28 @@ class Stage:
         cnt = torch.empty((n, 3), dtype=torch.int32, device=segs.device) if counts else None
         self.occluded_ptr(segs.data_ptr(), n, vis.data_ptr(), mode, cnt.data_ptr() if counts else None)
         return vis, cnt
+
+    def get_tuning(self):
+        t = _lib.Tuning()
+        check(lib().gr_scene_get_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "get_tuning")
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
+
+    def set_tuning(self, sort_bits=0, lds_stack=16, waves_per_cu=0, refill=16):
+        t = _lib.Tuning(sort_bits, lds_stack, waves_per_cu, refill)
+        check(lib().gr_scene_set_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "set_tuning")
+        return self.get_tuning()
+
+    def autotune(self, rays):
+        """Times traversal settings on `rays` ((n, 8) float32 device tensor) and
+        keeps the fastest; returns the chosen setting."""
+        import torch
+        n = rays.shape[0]
+        hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        t = _lib.Tuning()
+        check(lib().gr_scene_autotune(self._h, ctypes.c_void_p(rays.data_ptr()), n,
+                                      ctypes.c_void_p(hits.data_ptr()), ctypes.byref(t)),
+              self.ctx.handle, "autotune")
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
     def take_overflow_count(self):
         v = ctypes.c_uint64()

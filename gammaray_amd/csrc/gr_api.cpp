@@ -7,6 +7,7 @@
 // message in gr_ctx_last_error().
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,7 @@ void free_device(gr_scene* s) {
     (void)hipFree(s->d_overflow);
     (void)hipFree(s->d_next_ray);
     (void)hipFree(s->d_shade_blob);
+    gr::release_sort(s->sort);
     s->d_shade_blob = nullptr;
     s->shade = gr::ShadeView{};
     s->d_next_ray = nullptr;
@@ -62,6 +64,36 @@ void free_device(gr_scene* s) {
 }
 
 }  // namespace
+
+namespace gr {
+hipEvent_t timing_begin(gr_ctx*, hipStream_t s) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventRecord(e, s) != hipSuccess) {
+        (void)hipEventDestroy(e);
+        return nullptr;
+    }
+    return e;
+}
+
+void timing_end(gr_ctx* ctx, int kind, hipEvent_t begin, hipStream_t s) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess || hipEventRecord(e, s) != hipSuccess) {
+        if (e) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(begin);
+        return;
+    }
+    ctx->timed.push_back(gr_ctx::TimedLaunch{kind, begin, e});
+}
+}  // namespace gr
+
+static void drop_timed(gr_ctx* ctx) {
+    for (auto& t : ctx->timed) {
+        (void)hipEventDestroy(t.begin);
+        (void)hipEventDestroy(t.end);
+    }
+    ctx->timed.clear();
+}
 
 extern "C" {
 
@@ -100,6 +132,8 @@ void gr_ctx_destroy(gr_ctx* ctx) {
         return;
     }
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_timed(ctx);
     if (ctx->own_stream) {
         (void)hipStreamSynchronize(ctx->own_stream);
         (void)hipStreamDestroy(ctx->own_stream);
@@ -129,6 +163,37 @@ int gr_ctx_synchronize(gr_ctx* ctx) {
 }
 
 const char* gr_ctx_last_error(gr_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gr_ctx_set_kernel_timing(gr_ctx* ctx, int enable) {
+    if (!ctx) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    ctx->timing = enable != 0;
+    return GR_OK;
+}
+
+int gr_ctx_take_kernel_times(gr_ctx* ctx, double* ms, uint64_t* launches) {
+    if (!ctx) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    double acc[GR_KERNEL_KINDS] = {0.0};
+    uint64_t cnt[GR_KERNEL_KINDS] = {0};
+    hipError_t e = hipSuccess;
+    for (auto& t : ctx->timed) {
+        if (e == hipSuccess) e = hipEventSynchronize(t.end);
+        float v = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&v, t.begin, t.end);
+        if (e == hipSuccess && t.kind >= 0 && t.kind < GR_KERNEL_KINDS) {
+            acc[t.kind] += v;
+            cnt[t.kind] += 1;
+        }
+    }
+    drop_timed(ctx);
+    if (e != hipSuccess) return hip_fail(ctx, e, "kernel timing");
+    for (int k = 0; k < GR_KERNEL_KINDS; ++k) {
+        if (ms) ms[k] = acc[k];
+        if (launches) launches[k] = cnt[k];
+    }
+    return GR_OK;
+}
 
 int gr_scene_create(gr_ctx* ctx, gr_scene** out) {
     if (!ctx || !out) return GR_ERR_INVALID;
@@ -308,7 +373,8 @@ int gr_intersect_closest(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* 
     if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
     if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
     if (n && (!d_rays || !d_hits)) return fail(s->ctx, GR_ERR_INVALID, "null ray/hit buffer%s");
-    hipError_t e = launch_intersect(s->view, d_rays, n, d_hits, d_counts, s->ctx->stream);
+    hipError_t e = s->tuning.sort_bits ? gr::reserve_sort(s->sort, n) : hipSuccess;
+    if (e == hipSuccess) e = launch_intersect(gr::trace_setup(s), d_rays, n, nullptr, d_hits, d_counts, s->ctx->stream);
     return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "intersect launch");
 }
 
@@ -320,8 +386,105 @@ int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_v
         return fail(s->ctx, GR_ERR_INVALID, "unknown occlusion mode%s");
     if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
     if (n && (!d_segs || !d_visible)) return fail(s->ctx, GR_ERR_INVALID, "null segment/output buffer%s");
-    hipError_t e = launch_occluded(s->view, d_segs, n, d_visible, mode, d_counts, s->ctx->stream);
+    hipError_t e = s->tuning.sort_bits ? gr::reserve_sort(s->sort, n) : hipSuccess;
+    if (e == hipSuccess)
+        e = launch_occluded(gr::trace_setup(s), d_segs, n, nullptr, d_visible, mode, d_counts, s->ctx->stream);
     return e == hipSuccess ? GR_OK : hip_fail(s->ctx, e, "occlusion launch");
+}
+
+static bool valid_tuning(const gr_tuning& t) {
+    return t.sort_bits <= 30 && (t.lds_stack == 8 || t.lds_stack == 16 || t.lds_stack == 32) && t.refill >= 1 &&
+           t.refill <= 64 && t.waves_per_cu <= 64;
+}
+
+int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
+    if (!s || !t) return GR_ERR_INVALID;
+    if (!valid_tuning(*t)) return fail(s->ctx, GR_ERR_INVALID, "tuning out of range%s");
+    s->tuning.sort_bits = gr::sort_key_bits(t->sort_bits);
+    s->tuning.lds_stack = t->lds_stack;
+    s->tuning.waves_per_cu = t->waves_per_cu;
+    s->tuning.refill = t->refill;
+    return GR_OK;
+}
+
+int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
+    if (!s || !out) return GR_ERR_INVALID;
+    out->sort_bits = s->tuning.sort_bits;
+    out->lds_stack = s->tuning.lds_stack;
+    out->waves_per_cu = s->tuning.waves_per_cu;
+    out->refill = s->tuning.refill;
+    return GR_OK;
+}
+
+namespace {
+// Shortest of three timed closest-hit launches (after one untimed) in ms.
+hipError_t time_closest(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* rays, uint32_t n, gr_hit* hits,
+                        hipEvent_t a, hipEvent_t b, float* best) {
+    const gr::TraceSetup ts{s->view, tu, &s->sort};
+    hipStream_t st = s->ctx->stream;
+    *best = 3.0e38f;
+    for (int rep = 0; rep < 4; ++rep) {
+        hipError_t e = hipEventRecord(a, st);
+        if (e == hipSuccess) e = gr::launch_intersect(ts, rays, n, nullptr, hits, nullptr, st);
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e != hipSuccess) return e;
+        if (rep > 0 && ms < *best) *best = ms;
+    }
+    return hipSuccess;
+}
+}  // namespace
+
+int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, gr_tuning* out) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    if (n == 0 || !d_rays || !d_hits) return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays and a hit buffer%s");
+    hipError_t e = gr::reserve_sort(s->sort, n);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    gr::TraceTuning best{};  // sort off, 16 LDS entries, occupancy limit, refill at 16
+    float best_ms = 0.0f;
+    if (e == hipSuccess) e = time_closest(s, best, d_rays, n, d_hits, a, b, &best_ms);
+    // One knob at a time; a candidate must win by more than 1 % to replace the incumbent.
+    auto consider = [&](gr::TraceTuning cand) {
+        if (e != hipSuccess) return;
+        float ms = 0.0f;
+        e = time_closest(s, cand, d_rays, n, d_hits, a, b, &ms);
+        if (e == hipSuccess && ms < 0.99f * best_ms) {
+            best = cand;
+            best_ms = ms;
+        }
+    };
+    for (uint32_t bits : {12u, 18u, 24u, 30u}) {
+        gr::TraceTuning c = best;
+        c.sort_bits = bits;
+        consider(c);
+    }
+    for (uint32_t lds : {8u, 32u}) {
+        gr::TraceTuning c = best;
+        c.lds_stack = lds;
+        consider(c);
+    }
+    const int occ = gr::trace_occupancy(best.lds_stack);
+    for (int frac : {3, 2, 1}) {
+        gr::TraceTuning c = best;
+        c.waves_per_cu = (uint32_t)std::max(1, occ * frac / 4);
+        if ((int)c.waves_per_cu < occ) consider(c);
+    }
+    for (uint32_t rf : {4u, 8u, 32u}) {
+        gr::TraceTuning c = best;
+        c.refill = rf;
+        consider(c);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "autotune");
+    s->tuning = best;
+    return out ? gr_scene_get_tuning(s, out) : GR_OK;
 }
 
 int gr_film_create(gr_ctx* ctx, uint32_t width, uint32_t height, gr_film** out) {

@@ -95,16 +95,53 @@ struct GpuLayout {
 };
 GpuLayout make_gpu_layout(const FlatTree& t, const std::vector<Prim>& prims);
 
+// ---- ray sort (gr_sort.hip) -------------------------------------------------
+struct SortScratch {
+    size_t cap = 0;  // keys
+    void* blob = nullptr;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;    // 256 digit counts per 4096-key block
+    uint32_t* totals = nullptr;  // 256
+};
+// Rounds a requested key width to the widths the sort uses (0 or 3 + 3m).
+uint32_t sort_key_bits(uint32_t bits);
+hipError_t reserve_sort(SortScratch& ss, size_t n);
+void release_sort(SortScratch& ss);
+// Sorts the first n (= *n_ptr if given, at most n_max) rays or segments of
+// `in` by (octant, origin Morton code); *perm receives the ray indices in
+// sorted order, or nullptr when bits == 0.  ss must hold n_max keys.
+hipError_t sort_rays(const SceneView& sv, const float4* in, uint32_t n_max, const uint32_t* n_ptr, bool segments,
+                     uint32_t bits, SortScratch& ss, const uint32_t** perm, hipStream_t s);
+
 // ---- launchers (gr_traverse.hip) -------------------------------------------
-hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n, gr_hit* hits,
-                            uint32_t* counts, hipStream_t s);
-hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
+// Settings of the traversal launch (gr_tuning).
+struct TraceTuning {
+    uint32_t sort_bits = 0;
+    uint32_t lds_stack = 16;
+    uint32_t waves_per_cu = 0;
+    uint32_t refill = 16;
+};
+
+// Everything a traversal launch needs besides its rays.
+struct TraceSetup {
+    SceneView sv;
+    TraceTuning tu;
+    SortScratch* sort;       // holds >= n_max keys when tu.sort_bits != 0
+    gr_ctx* timing = nullptr;  // set when kernel timing is on
+};
+
+// Kernel timing (gr_api.cpp): brackets launches with events on stream s.
+hipEvent_t timing_begin(gr_ctx* ctx, hipStream_t s);
+void timing_end(gr_ctx* ctx, int kind, hipEvent_t begin, hipStream_t s);
+
+// n_ptr: if not null, the ray count is read on the device from *n_ptr (<= n).
+hipError_t launch_intersect(const TraceSetup& ts, const gr_ray* rays, uint32_t n, const uint32_t* n_ptr,
+                            gr_hit* hits, uint32_t* counts, hipStream_t s);
+hipError_t launch_occluded(const TraceSetup& ts, const gr_segment* segs, uint32_t n, const uint32_t* n_ptr,
                            uint32_t* visible, int mode, uint32_t* counts, hipStream_t s);
-// Queue forms: the count is read on the device from *n_ptr (<= n_max).
-hipError_t launch_intersect_queue(const SceneView& sv, const gr_ray* rays, uint32_t n_max,
-                                  const uint32_t* n_ptr, gr_hit* hits, hipStream_t s);
-hipError_t launch_occluded_queue(const SceneView& sv, const gr_segment* segs, uint32_t n_max,
-                                 const uint32_t* n_ptr, uint32_t* visible, hipStream_t s);
+// Resident waves per CU the traversal kernel reaches with this LDS stack depth.
+int trace_occupancy(uint32_t lds_stack);
 
 // ---- shading data ------------------------------------------------------------
 // Per prim: where its mesh data lives, or its sphere, plus its area.
@@ -138,6 +175,12 @@ struct gr_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
+    bool timing = false;
+    struct TimedLaunch {
+        int kind;
+        hipEvent_t begin, end;
+    };
+    std::vector<TimedLaunch> timed;
 };
 
 struct gr_scene {
@@ -153,6 +196,8 @@ struct gr_scene {
     unsigned long long* d_overflow = nullptr;
     uint32_t* d_next_ray = nullptr;
     gr::SceneView view{};
+    gr::TraceTuning tuning{};
+    gr::SortScratch sort{};
     // shading data (gr_render.hip)
     void* d_shade_blob = nullptr;  // one allocation holding every shading array
     gr::ShadeView shade{};
@@ -185,6 +230,9 @@ struct gr_film {
 };
 
 namespace gr {
+inline TraceSetup trace_setup(gr_scene* s) {
+    return TraceSetup{s->view, s->tuning, &s->sort, s->ctx->timing ? s->ctx : nullptr};
+}
 // gr_render.hip
 hipError_t upload_shading(gr_scene* s, std::vector<GpuPrim>* prims_out);
 hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_params& prm, gr_film* film);

@@ -1051,6 +1051,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     float4* seg = nullptr;
     uint32_t* vis = nullptr;
     hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, chunk * nstrat, &b, &seg, &vis);
+    if (e == hipSuccess && sc->tuning.sort_bits) e = reserve_sort(sc->sort, std::max(npaths, shadow));
     if (e != hipSuccess) return e;
     a.rb.seg = seg;
     a.rb.vis = vis;
@@ -1058,8 +1059,8 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     hipLaunchKernelGGL(k_camera, blocks_for(npaths), dim3(256), 0, s, a);
     uint32_t cur = 0;
     for (uint32_t k = 0; k < cmax && e == hipSuccess; ++k) {
-        e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths,
-                                   cnt + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+        e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
+                             reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
         hipLaunchKernelGGL(k_walk<true>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
         cur ^= 1u;
@@ -1069,8 +1070,8 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     hipLaunchKernelGGL(k_light_start, blocks_for(npaths), dim3(256), 0, s, a, b);
     cur = 0;
     for (uint32_t k = 1; k < lmax && e == hipSuccess; ++k) {
-        e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths,
-                                   cnt + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+        e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
+                             reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
         hipLaunchKernelGGL(k_walk<false>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
         cur ^= 1u;
@@ -1080,8 +1081,8 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 2u);
         hipLaunchKernelGGL(k_connect_eval, blocks_for((p1 - p0) * nstrat), dim3(256), 0, s, a, b, (uint32_t)p0,
                            (uint32_t)p1);
-        e = launch_occluded_queue(sc->view, reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
-                                  vis, s);
+        e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
+                            vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(k_connect_sum, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
     }
@@ -1106,6 +1107,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     const size_t paths = npx * batch;
     if (paths > 0x7fffffffull) return hipErrorInvalidValue;
     hipError_t e = reserve(film->rb, paths);
+    if (e == hipSuccess && sc->tuning.sort_bits) e = reserve_sort(sc->sort, paths);
     if (e != hipSuccess) return e;
 
     RenderArgs a;
@@ -1154,15 +1156,15 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
                 if (e == hipSuccess) e = hipStreamSynchronize(s);
                 if (e != hipSuccess || live == 0) break;
             }
-            e = launch_intersect_queue(sc->view, reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), a.npaths,
-                                       film->rb.counts + cur, reinterpret_cast<gr_hit*>(film->rb.hits), s);
+            e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), a.npaths, film->rb.counts + cur,
+                             reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, cur ^ 1u);
             if (nee) {
                 hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 2u);
                 hipLaunchKernelGGL(k_shade<true>, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
-                e = launch_occluded_queue(sc->view, reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
-                                          film->rb.counts + 2, film->rb.vis, s);
+                e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
+                                    film->rb.counts + 2, film->rb.vis, GR_OCCLUSION_EXACT, nullptr, s);
                 if (e != hipSuccess) break;
                 hipLaunchKernelGGL(k_shadow_accum, blocks_for(a.npaths), dim3(256), 0, s, a);
             } else {

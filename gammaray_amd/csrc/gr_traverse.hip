@@ -125,19 +125,18 @@ __device__ __forceinline__ GpuNode load_node(const GpuNode* nodes, uint32_t i) {
     return n;
 }
 
+
 // ---------------------------------------------------------------------------
 // Persistent traversal.  One 64-lane wave per block; every lane owns one ray
 // at a time.  Lanes whose ray finishes go idle and the wave refills them from
-// a global ray counter once enough are idle, so long and short rays share a
-// wave without leaving lanes empty.  The loop is while-while: all lanes step
-// through interior nodes until each is at a leaf (or done), then leaves are
-// tested together.  The traversal stack keeps its top kLdsStack entries per
-// lane in LDS and spills older entries to private memory, up to the
-// reference's 64 entries.
+// a global ray counter once `refill` are idle, so long and short rays share a
+// wave without leaving lanes empty.  Rays are taken in the order of `perm`
+// (a coherence sort) when given; results always go to the ray's own index.
+// The loop is while-while: all lanes step through interior nodes until each is
+// at a leaf (or done), then leaves are tested together.  The traversal stack
+// keeps its top LDS entries per lane in LDS and spills older entries to
+// private memory, up to the reference's 64 entries.
 constexpr int kWave = 64;
-constexpr int kLdsStack = 16;
-constexpr int kSpill = kStackSize - kLdsStack;
-constexpr int kRefill = 16;  // refill once this many lanes are idle
 
 template <int MODE>
 struct Output {
@@ -153,19 +152,23 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <int MODE, bool COUNT>
-__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __restrict__ in,
-                                                  uint32_t n, typename Output<MODE>::type* __restrict__ out,
-                                                  uint32_t* __restrict__ counts,
-                                                  uint32_t* __restrict__ next_ray,
-                                                  const uint32_t* __restrict__ n_ptr) {
-    __shared__ uint32_t s_ref[kLdsStack][kWave];
-    __shared__ float s_t[kLdsStack][kWave];
+template <int MODE, bool COUNT, int LDS>
+__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __restrict__ in, uint32_t n,
+                                                  const uint32_t* __restrict__ perm,
+                                                  typename Output<MODE>::type* __restrict__ out,
+                                                  uint32_t* __restrict__ counts, uint32_t* __restrict__ next_ray,
+                                                  const uint32_t* __restrict__ n_ptr, uint32_t refill) {
+    constexpr int kSpill = kStackSize - LDS;
+    __shared__ uint32_t s_ref[LDS][kWave];
+    __shared__ float s_t[LDS][kWave];
     uint32_t spill_ref[kSpill];
     float spill_t[kSpill];
     const uint32_t lane = threadIdx.x;
     // A ray count that earlier kernels on the stream produced (queue length).
-    if (n_ptr) n = __builtin_amdgcn_readfirstlane(*n_ptr);
+    if (n_ptr) {
+        const uint32_t m = __builtin_amdgcn_readfirstlane(*n_ptr);
+        n = m < n ? m : n;
+    }
 
     bool active = false;
     bool exhausted = false;
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
                 t = spill_t[sp];
                 base = sp;
             } else {
-                ref = s_ref[sp & (kLdsStack - 1)][lane];
-                t = s_t[sp & (kLdsStack - 1)][lane];
+                ref = s_ref[sp & (LDS - 1)][lane];
+                t = s_t[sp & (LDS - 1)][lane];
             }
             if (t < closest) {
                 cur = ref;
@@ -226,16 +229,19 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             const int nidle = __popcll(idle);
-            if (nidle >= kRefill) {
+            if ((uint32_t)nidle >= refill) {
                 uint32_t first = 0;
                 if (lane == 0) first = atomicAdd(next_ray, (uint32_t)nidle);
                 first = __shfl(first, 0);
                 if (first + (uint32_t)nidle >= n) exhausted = true;
                 if (!active) {
                     const uint32_t idx = first + (uint32_t)__popcll(idle & lanes_below());
-                    if (idx < n) {
-                        ray = idx;
-                        const float4 a = in[2 * (size_t)idx], b = in[2 * (size_t)idx + 1];
+                    // A permutation entry outside [0, n) is skipped (never
+                    // dereferenced), so a faulty sort shows as missing results.
+                    const uint32_t pick = idx < n ? (perm ? perm[idx] : idx) : n;
+                    if (pick < n) {
+                        ray = pick;
+                        const float4 a = in[2 * (size_t)ray], b = in[2 * (size_t)ray + 1];
                         best = kNone;
                         sp = 0;
                         base = 0;
@@ -301,13 +307,13 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
                     finish(false, true);
                     break;
                 }
-                if (sp - base == kLdsStack) {
-                    spill_ref[base] = s_ref[base & (kLdsStack - 1)][lane];
-                    spill_t[base] = s_t[base & (kLdsStack - 1)][lane];
+                if (sp - base == LDS) {
+                    spill_ref[base] = s_ref[base & (LDS - 1)][lane];
+                    spill_t[base] = s_t[base & (LDS - 1)][lane];
                     ++base;
                 }
-                s_ref[sp & (kLdsStack - 1)][lane] = far_ref;
-                s_t[sp & (kLdsStack - 1)][lane] = far_t;
+                s_ref[sp & (LDS - 1)][lane] = far_ref;
+                s_t[sp & (LDS - 1)][lane] = far_t;
                 ++sp;
             }
             if (near_ok && near_t < closest) {
@@ -351,59 +357,93 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
 
 __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
-// n: the ray count, or (with n_ptr) an upper bound of the count stored at
-// *n_ptr by earlier work on the stream.
-template <int MODE, bool COUNT>
-hipError_t launch_trace(const SceneView& sv, const void* in, uint32_t n, const uint32_t* n_ptr,
-                        void* out, uint32_t* counts, hipStream_t s) {
-    static int grid = 0;
-    if (grid == 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<MODE, COUNT>, kWave, 0);
-        if (e != hipSuccess) return e;
-        grid = cus * (per_cu > 0 ? per_cu : 1);
+template <int MODE, bool COUNT, int LDS>
+int occupancy() {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_trace<MODE, COUNT, LDS>, kWave, 0) != hipSuccess)
+            v = 1;
+        per_cu = v > 0 ? v : 1;
     }
+    return per_cu;
+}
+
+int num_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            cus = v;
+        else
+            return 1;
+    }
+    return cus;
+}
+
+template <int MODE, bool COUNT, int LDS>
+hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, const uint32_t* perm,
+                      void* out, uint32_t* counts, hipStream_t s) {
+    int per_cu = occupancy<MODE, COUNT, LDS>();
+    if (ts.tu.waves_per_cu > 0 && (int)ts.tu.waves_per_cu < per_cu) per_cu = (int)ts.tu.waves_per_cu;
+    const uint32_t grid = (uint32_t)(num_cus() * per_cu);
     const uint32_t needed = (n + kWave - 1) / kWave;
-    const uint32_t blocks = needed < (uint32_t)grid ? needed : (uint32_t)grid;
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, sv.next_ray);
-    hipLaunchKernelGGL((k_trace<MODE, COUNT>), dim3(blocks), dim3(kWave), 0, s, sv,
-                       reinterpret_cast<const float4*>(in), n,
-                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, sv.next_ray, n_ptr);
+    const uint32_t blocks = needed < grid ? needed : grid;
+    const uint32_t refill = ts.tu.refill < 1 ? 1u : (ts.tu.refill > 64 ? 64u : ts.tu.refill);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);
+    hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
+    hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS>), dim3(blocks), dim3(kWave), 0, s, ts.sv,
+                       reinterpret_cast<const float4*>(in), n, perm,
+                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, ts.sv.next_ray, n_ptr, refill);
+    if (t0) timing_end(ts.timing, MODE == kClosest ? GR_KERNEL_TRACE_CLOSEST : GR_KERNEL_TRACE_SHADOW, t0, s);
     return hipGetLastError();
+}
+
+template <int MODE, bool COUNT>
+hipError_t launch_trace(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, void* out,
+                        uint32_t* counts, hipStream_t s) {
+    const uint32_t* perm = nullptr;
+    if (ts.tu.sort_bits != 0) {
+        if (!ts.sort) return hipErrorInvalidValue;
+        hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
+        const hipError_t e = sort_rays(ts.sv, reinterpret_cast<const float4*>(in), n, n_ptr, MODE != kClosest,
+                                       ts.tu.sort_bits, *ts.sort, &perm, s);
+        if (t0) timing_end(ts.timing, GR_KERNEL_RAY_SORT, t0, s);
+        if (e != hipSuccess) return e;
+    }
+    switch (ts.tu.lds_stack) {
+        case 8: return launch_lds<MODE, COUNT, 8>(ts, in, n, n_ptr, perm, out, counts, s);
+        case 32: return launch_lds<MODE, COUNT, 32>(ts, in, n, n_ptr, perm, out, counts, s);
+        default: return launch_lds<MODE, COUNT, 16>(ts, in, n, n_ptr, perm, out, counts, s);
+    }
 }
 
 }  // namespace
 
-hipError_t launch_intersect(const SceneView& sv, const gr_ray* rays, uint32_t n, gr_hit* hits,
-                            uint32_t* counts, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    return counts ? launch_trace<kClosest, true>(sv, rays, n, nullptr, hits, counts, s)
-                  : launch_trace<kClosest, false>(sv, rays, n, nullptr, hits, counts, s);
+int trace_occupancy(uint32_t lds_stack) {
+    switch (lds_stack) {
+        case 8: return occupancy<kClosest, false, 8>();
+        case 32: return occupancy<kClosest, false, 32>();
+        default: return occupancy<kClosest, false, 16>();
+    }
 }
 
-hipError_t launch_occluded(const SceneView& sv, const gr_segment* segs, uint32_t n,
+hipError_t launch_intersect(const TraceSetup& ts, const gr_ray* rays, uint32_t n, const uint32_t* n_ptr,
+                            gr_hit* hits, uint32_t* counts, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    return counts ? launch_trace<kClosest, true>(ts, rays, n, n_ptr, hits, counts, s)
+                  : launch_trace<kClosest, false>(ts, rays, n, n_ptr, hits, counts, s);
+}
+
+hipError_t launch_occluded(const TraceSetup& ts, const gr_segment* segs, uint32_t n, const uint32_t* n_ptr,
                            uint32_t* visible, int mode, uint32_t* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (mode == GR_OCCLUSION_ANYHIT)
-        return counts ? launch_trace<kOcclusionAnyHit, true>(sv, segs, n, nullptr, visible, counts, s)
-                      : launch_trace<kOcclusionAnyHit, false>(sv, segs, n, nullptr, visible, counts, s);
-    return counts ? launch_trace<kOcclusionExact, true>(sv, segs, n, nullptr, visible, counts, s)
-                  : launch_trace<kOcclusionExact, false>(sv, segs, n, nullptr, visible, counts, s);
-}
-
-hipError_t launch_intersect_queue(const SceneView& sv, const gr_ray* rays, uint32_t n_max,
-                                  const uint32_t* n_ptr, gr_hit* hits, hipStream_t s) {
-    if (n_max == 0) return hipSuccess;
-    return launch_trace<kClosest, false>(sv, rays, n_max, n_ptr, hits, nullptr, s);
-}
-
-hipError_t launch_occluded_queue(const SceneView& sv, const gr_segment* segs, uint32_t n_max,
-                                 const uint32_t* n_ptr, uint32_t* visible, hipStream_t s) {
-    if (n_max == 0) return hipSuccess;
-    return launch_trace<kOcclusionExact, false>(sv, segs, n_max, n_ptr, visible, nullptr, s);
+        return counts ? launch_trace<kOcclusionAnyHit, true>(ts, segs, n, n_ptr, visible, counts, s)
+                      : launch_trace<kOcclusionAnyHit, false>(ts, segs, n, n_ptr, visible, counts, s);
+    return counts ? launch_trace<kOcclusionExact, true>(ts, segs, n, n_ptr, visible, counts, s)
+                  : launch_trace<kOcclusionExact, false>(ts, segs, n, n_ptr, visible, counts, s);
 }
 
 }  // namespace gr

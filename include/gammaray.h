@@ -122,6 +122,39 @@ int gr_occluded(gr_scene *scene, const gr_segment *d_segs, uint32_t n, uint32_t 
  * prim = GR_NONE with t = -1 (closest hit) or not-visible (visibility). */
 int gr_scene_take_overflow_count(gr_scene *scene, uint64_t *out);
 
+/* ---- traversal tuning (no reference counterpart: the reference traverses one
+ * ray at a time on the CPU, src/geom/bvh.rs:406-477) ------------------------
+ * Applies to every traversal of the scene: the batched calls above and the
+ * rays gr_trace casts.  No setting changes a result, only speed. */
+typedef struct {
+    uint32_t sort_bits;    /* 0: rays run in input order; else rays are sorted by a key of
+                              this many bits (6..30, rounded down to 3 + 3m): direction
+                              octant above the m-bit-per-axis Morton code of the origin */
+    uint32_t lds_stack;    /* traversal stack entries per lane kept in LDS: 8, 16 or 32 */
+    uint32_t waves_per_cu; /* resident traversal waves per CU; 0 = occupancy limit */
+    uint32_t refill;       /* idle lanes (1..64) at which a wave fetches new rays */
+} gr_tuning;
+int gr_scene_set_tuning(gr_scene *scene, const gr_tuning *tuning);
+int gr_scene_get_tuning(gr_scene *scene, gr_tuning *out);
+/* Times candidate settings on n rays (device memory; closest hit) and keeps the
+ * fastest: sort bits, then LDS stack depth, then waves per CU, then refill.
+ * d_hits: scratch for n hits.  The chosen setting is returned in *out. */
+int gr_scene_autotune(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
+                      gr_tuning *out);
+
+/* ---- kernel timing (observability; no reference counterpart) -------------
+ * When enabled, every traversal and ray-sort launch on the context is
+ * bracketed by HIP events on the stream it runs on (do not enable while a
+ * caller captures the stream into a graph). */
+#define GR_KERNEL_TRACE_CLOSEST 0
+#define GR_KERNEL_TRACE_SHADOW 1
+#define GR_KERNEL_RAY_SORT 2
+#define GR_KERNEL_KINDS 3
+int gr_ctx_set_kernel_timing(gr_ctx *ctx, int enable);
+/* Waits for the context's stream; then, per kind, the summed milliseconds and
+ * the number of launches since the previous call (either pointer may be NULL). */
+int gr_ctx_take_kernel_times(gr_ctx *ctx, double *ms, uint64_t *launches);
+
 /* ---- rendering (Stage::trace, src/render/stage.rs:23-36) ---------------- */
 /* Camera::new arguments (src/core/camera.rs:42-61); f_stop is carried but, as
  * in the reference, unused (pinhole camera). */

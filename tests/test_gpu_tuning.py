@@ -1,0 +1,137 @@
+"""Traversal settings -- coherence sort of the rays, LDS stack depth, resident
+waves per CU, refill threshold -- change the order in which rays run, never a
+result.  Every setting must reproduce the oracle's hits bit for bit, write
+every output slot exactly once, do the same reference-order work per ray, and
+leave rendered films bit-identical.
+"""
+import numpy as np
+import pytest
+
+from gammaray_amd import scenes
+from gammaray_amd.api import GR_OCCLUSION_EXACT, Film, Stage, hits_to_numpy, trace
+from oracle.oracle import NONE, OracleScene
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [
+    dict(sort_bits=6),
+    dict(sort_bits=12),
+    dict(sort_bits=18, lds_stack=8),
+    dict(sort_bits=30, lds_stack=32, refill=1),
+    dict(lds_stack=8, waves_per_cu=1, refill=64),
+    dict(sort_bits=24, waves_per_cu=2, refill=4),
+]
+SENTINEL = 0x3F3F3F3F
+
+
+def _id(tu):
+    return "-".join("%s%d" % (k[0], v) for k, v in sorted(tu.items()))
+
+
+def _scene():
+    s = scenes.showcase_scene(n_spheres=80, grid=(30, 30))
+    for _ in range(20):  # a coincident-centroid cluster: a multi-component leaf
+        s.add_sphere((0.1, 1.0, -1.0), 0.3, scenes.disney())
+    return s
+
+
+def _closest(stage, rays8):
+    import torch
+    n = len(rays8)
+    r = torch.from_numpy(rays8).cuda()
+    hits = torch.full((n, 4), SENTINEL, dtype=torch.int32, device="cuda")
+    cnt = torch.full((n, 3), SENTINEL, dtype=torch.int32, device="cuda")
+    stage.intersect_ptr(r.data_ptr(), n, hits.data_ptr(), cnt.data_ptr())
+    torch.cuda.synchronize()
+    out = hits_to_numpy(hits)
+    out["counts"] = cnt.cpu().numpy().view(np.uint32)
+    out["raw"] = hits.cpu().numpy()
+    return out
+
+
+@pytest.mark.parametrize("tu", TUNINGS, ids=_id)
+def test_tuned_closest_hit_bit_exact(tu, gpu_ctx):
+    print("case tuned closest", tu, flush=True)
+    spec = _scene()
+    stage = Stage(spec, ctx=gpu_ctx)
+    got = stage.set_tuning(**tu)
+    assert got["lds_stack"] == tu.get("lds_stack", 16)
+    o = OracleScene(spec)
+    n = 300_001  # not a multiple of the sort tile or the wave
+    rays = scenes.ray_batch(spec, n, seed=31)
+    g = _closest(stage, rays)
+    r = o.intersect(scenes.rays6(rays), counts=True)
+    miss = r["prim"] == NONE
+    assert not np.any(g["raw"][miss][:, 3] == SENTINEL)
+    assert np.array_equal(g["prim"], r["prim"])
+    hit = ~miss
+    assert np.array_equal(g["comp"][hit], r["comp"][hit])
+    assert np.array_equal(g["dist"][hit].view(np.uint32), r["dist"][hit].view(np.uint32))
+    assert np.array_equal(g["counts"], r["counts"])
+
+
+@pytest.mark.parametrize("tu", TUNINGS[1:4], ids=_id)
+def test_tuned_visibility_exact(tu, gpu_ctx):
+    import torch
+    print("case tuned visibility", tu, flush=True)
+    spec = _scene()
+    stage = Stage(spec, ctx=gpu_ctx)
+    stage.set_tuning(**tu)
+    o = OracleScene(spec)
+    n = 200_003
+    segs = scenes.segment_batch(spec, n, seed=32)
+    s = torch.from_numpy(segs).cuda()
+    vis = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    stage.occluded_ptr(s.data_ptr(), n, vis.data_ptr(), GR_OCCLUSION_EXACT)
+    torch.cuda.synchronize()
+    want = o.visibility(scenes.rays6(segs))["visible"]
+    assert np.array_equal(vis.cpu().numpy(), want.astype(np.int32))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4097])
+def test_sorted_small_batches(n, gpu_ctx):
+    print("case sorted small", n, flush=True)
+    spec = scenes.box_scene()
+    stage = Stage(spec, ctx=gpu_ctx)
+    stage.set_tuning(sort_bits=30)
+    o = OracleScene(spec)
+    rays = scenes.ray_batch(spec, n, seed=n)
+    g = _closest(stage, rays)
+    r = o.intersect(scenes.rays6(rays))
+    assert np.array_equal(g["prim"], r["prim"])
+    hit = r["prim"] != NONE
+    assert np.array_equal(g["dist"][hit].view(np.uint32), r["dist"][hit].view(np.uint32))
+
+
+@pytest.mark.parametrize("integrator", ["pt_nee", "bdpt"])
+def test_sorted_render_is_bit_identical(integrator, gpu_ctx):
+    """The wavefront renderer's queues sorted on the device (count read from
+    the queue) give the same film as unsorted queues."""
+    print("case sorted render", integrator, flush=True)
+    spec = scenes.showcase_scene(n_spheres=60, grid=(40, 40))
+    W, H = 64, 48
+    films = []
+    for tu in [dict(), dict(sort_bits=18, lds_stack=8, refill=8)]:
+        stage = Stage(spec, ctx=gpu_ctx)
+        stage.set_tuning(**tu)
+        film = Film(gpu_ctx, W, H)
+        trace(stage, film, integrator, spp=3, max_depth=6, light_depth=5, seed=41)
+        films.append(film.readback())
+    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
+    assert np.isfinite(films[0]).all() and np.abs(films[0][..., :3]).sum() > 0
+
+
+def test_autotune_picks_a_valid_setting(gpu_ctx):
+    import torch
+    print("case autotune", flush=True)
+    spec = scenes.icosphere_scene(level=5)
+    stage = Stage(spec, ctx=gpu_ctx)
+    rays = scenes.ray_batch(spec, 1 << 18, seed=33)
+    tu = stage.autotune(torch.from_numpy(rays).cuda())
+    assert tu == stage.get_tuning()
+    assert tu["lds_stack"] in (8, 16, 32) and 1 <= tu["refill"] <= 64
+    assert tu["sort_bits"] in (0, 12, 18, 24, 30)
+    o = OracleScene(spec)
+    g = _closest(stage, rays)
+    r = o.intersect(scenes.rays6(rays))
+    assert np.array_equal(g["prim"], r["prim"])

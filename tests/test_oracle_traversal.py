@@ -294,3 +294,21 @@ def test_invalid_mesh_rejected(host_ctx):
     s.add_mesh(np.zeros((3, 3), np.float32), [[0, 1, 5]], scenes.disney())
     with pytest.raises(GrError):
         Stage(s, ctx=host_ctx)
+
+
+def test_tuning_validation_and_rounding(host_ctx):
+    """Sort widths round down to 3 + 3m bits (octant + m bits per axis);
+    settings outside the documented ranges are rejected."""
+    from gammaray_amd.api import Stage
+    from gammaray_amd._lib import GrError
+    st = Stage(scenes.box_scene(), ctx=host_ctx)
+    assert st.get_tuning() == dict(sort_bits=0, lds_stack=16, waves_per_cu=0, refill=16)
+    for want, got in [(0, 0), (5, 0), (6, 6), (8, 6), (20, 18), (30, 30)]:
+        assert st.set_tuning(sort_bits=want)["sort_bits"] == got
+    for bad in [dict(sort_bits=31), dict(lds_stack=12), dict(refill=0), dict(refill=65), dict(waves_per_cu=65)]:
+        with pytest.raises(GrError):
+            st.set_tuning(**bad)
+    assert st.get_tuning()["sort_bits"] == 30
+    with pytest.raises(GrError):
+        import torch
+        st.autotune(torch.zeros((4, 8)))
