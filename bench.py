@@ -167,6 +167,17 @@ def main():
     hits = torch.empty((n, 4), dtype=torch.int32, device=dev)
     vis = torch.empty((n,), dtype=torch.int32, device=dev)
 
+    # Traversal settings: chosen before the timed region.
+    if args.tuning == "auto":
+        t0 = time.time()
+        tuning = stage.autotune(rays)
+        log("autotuned %s (%.1f s)" % (tuning, time.time() - t0))
+    elif args.tuning == "default":
+        tuning = stage.get_tuning()
+    else:
+        kv = dict(item.split("=") for item in args.tuning.split(","))
+        tuning = stage.set_tuning(**{k: int(v) for k, v in kv.items()})
+
     if args.traffic_probe:
         src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -190,17 +201,6 @@ def main():
     log("per ray: closest %.1f box %.2f tri; shadow %.1f box %.2f tri" % (
         cnt_c[:, 0].mean(), cnt_c[:, 1].mean(), cnt_s[:, 0].mean(), cnt_s[:, 1].mean()))
     del cnt_c, cnt_s
-
-    # Traversal settings: chosen before the timed region.
-    if args.tuning == "auto":
-        t0 = time.time()
-        tuning = stage.autotune(rays)
-        log("autotuned %s (%.1f s)" % (tuning, time.time() - t0))
-    elif args.tuning == "default":
-        tuning = stage.get_tuning()
-    else:
-        kv = dict(item.split("=") for item in args.tuning.split(","))
-        tuning = stage.set_tuning(**{k: int(v) for k, v in kv.items()})
 
     def step(ev=None):
         if ev is not None:

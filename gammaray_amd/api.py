@@ -156,8 +156,8 @@ class Stage:
         check(lib().gr_scene_get_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "get_tuning")
         return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
-    def set_tuning(self, sort_bits=0, lds_stack=16, waves_per_cu=0, refill=16):
-        t = _lib.Tuning(sort_bits, lds_stack, waves_per_cu, refill)
+    def set_tuning(self, sort_bits=0, lds_stack=16, waves_per_cu=0, refill=16, xcd_ranges=0):
+        t = _lib.Tuning(sort_bits, lds_stack, waves_per_cu, refill, xcd_ranges)
         check(lib().gr_scene_set_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "set_tuning")
         return self.get_tuning()
 

@@ -307,7 +307,7 @@ int gr_scene_commit(gr_scene* s) {
     if (e == hipSuccess) e = upload(&s->d_comp_ids, lay.comp_ids, ctx->stream);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_overflow), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemsetAsync(s->d_overflow, 0, sizeof(unsigned long long), ctx->stream);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_next_ray), sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_next_ray), gr::kRayCounters * sizeof(uint32_t));
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         free_device(s);
@@ -394,7 +394,7 @@ int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_v
 
 static bool valid_tuning(const gr_tuning& t) {
     return t.sort_bits <= 30 && (t.lds_stack == 8 || t.lds_stack == 16 || t.lds_stack == 32) && t.refill >= 1 &&
-           t.refill <= 64 && t.waves_per_cu <= 64;
+           t.refill <= 64 && t.waves_per_cu <= 64 && t.xcd_ranges <= 1;
 }
 
 int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
@@ -404,6 +404,7 @@ int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
     s->tuning.lds_stack = t->lds_stack;
     s->tuning.waves_per_cu = t->waves_per_cu;
     s->tuning.refill = t->refill;
+    s->tuning.xcd_ranges = t->xcd_ranges;
     return GR_OK;
 }
 
@@ -413,6 +414,7 @@ int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
     out->lds_stack = s->tuning.lds_stack;
     out->waves_per_cu = s->tuning.waves_per_cu;
     out->refill = s->tuning.refill;
+    out->xcd_ranges = s->tuning.xcd_ranges;
     return GR_OK;
 }
 
@@ -462,6 +464,11 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
     for (uint32_t bits : {12u, 18u, 24u, 30u}) {
         gr::TraceTuning c = best;
         c.sort_bits = bits;
+        consider(c);
+    }
+    {
+        gr::TraceTuning c = best;
+        c.xcd_ranges = 1;
         consider(c);
     }
     for (uint32_t lds : {8u, 32u}) {

@@ -83,7 +83,7 @@ struct SceneView {
     float root_lo[3], root_hi[3];
     float pad;  // 1 + 2 gamma(3)
     unsigned long long* overflow;
-    uint32_t* next_ray;  // work counter of the persistent traversal kernels
+    uint32_t* next_ray;  // kRayCounters work counters of the persistent traversal kernels
 };
 
 struct GpuLayout {
@@ -121,7 +121,10 @@ struct TraceTuning {
     uint32_t lds_stack = 16;
     uint32_t waves_per_cu = 0;
     uint32_t refill = 16;
+    uint32_t xcd_ranges = 0;
 };
+// Ray counters of the persistent traversal: one per XCD range.
+constexpr uint32_t kRayCounters = 8;
 
 // Everything a traversal launch needs besides its rays.
 struct TraceSetup {

@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
                                                   const uint32_t* __restrict__ perm,
                                                   typename Output<MODE>::type* __restrict__ out,
                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ next_ray,
-                                                  const uint32_t* __restrict__ n_ptr, uint32_t refill) {
+                                                  const uint32_t* __restrict__ n_ptr, uint32_t refill,
+                                                  uint32_t parts) {
     constexpr int kSpill = kStackSize - LDS;
     __shared__ uint32_t s_ref[LDS][kWave];
     __shared__ float s_t[LDS][kWave];
@@ -170,6 +171,11 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
         n = m < n ? m : n;
     }
 
+    // With parts > 1 the batch is cut into `parts` contiguous ranges, one per
+    // XCD (workgroups are dealt to the XCDs round-robin); a wave drains its own
+    // XCD's range first, then helps with the following ones.
+    uint32_t part = blockIdx.x % parts;
+    uint32_t drained = 0;
     bool active = false;
     bool exhausted = false;
     uint32_t ray = 0, cur = 0, best = kNone;
@@ -230,15 +236,22 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
             const uint64_t idle = __ballot(!active);
             const int nidle = __popcll(idle);
             if ((uint32_t)nidle >= refill) {
+                const uint32_t lo = (uint32_t)(((uint64_t)n * part) / parts);
+                const uint32_t len = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - lo;
                 uint32_t first = 0;
-                if (lane == 0) first = atomicAdd(next_ray, (uint32_t)nidle);
+                if (lane == 0) first = atomicAdd(next_ray + part, (uint32_t)nidle);
                 first = __shfl(first, 0);
-                if (first + (uint32_t)nidle >= n) exhausted = true;
+                if (first + (uint32_t)nidle >= len) {
+                    // this range is drained (counters only grow): go to the next
+                    if (++drained == parts) exhausted = true;
+                    part = part + 1 == parts ? 0u : part + 1;
+                }
                 if (!active) {
-                    const uint32_t idx = first + (uint32_t)__popcll(idle & lanes_below());
+                    const uint32_t k = first + (uint32_t)__popcll(idle & lanes_below());
+                    const uint32_t idx = lo + k;
                     // A permutation entry outside [0, n) is skipped (never
                     // dereferenced), so a faulty sort shows as missing results.
-                    const uint32_t pick = idx < n ? (perm ? perm[idx] : idx) : n;
+                    const uint32_t pick = k < len ? (perm ? perm[idx] : idx) : n;
                     if (pick < n) {
                         ray = pick;
                         const float4 a = in[2 * (size_t)ray], b = in[2 * (size_t)ray + 1];
@@ -355,7 +368,9 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
     }
 }
 
-__global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
+__global__ void k_reset_counter(uint32_t* c) {
+    for (uint32_t i = 0; i < kRayCounters; ++i) c[i] = 0u;
+}
 
 template <int MODE, bool COUNT, int LDS>
 int occupancy() {
@@ -391,11 +406,13 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     const uint32_t needed = (n + kWave - 1) / kWave;
     const uint32_t blocks = needed < grid ? needed : grid;
     const uint32_t refill = ts.tu.refill < 1 ? 1u : (ts.tu.refill > 64 ? 64u : ts.tu.refill);
+    const uint32_t parts = ts.tu.xcd_ranges ? kRayCounters : 1u;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);
     hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
     hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS>), dim3(blocks), dim3(kWave), 0, s, ts.sv,
                        reinterpret_cast<const float4*>(in), n, perm,
-                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, ts.sv.next_ray, n_ptr, refill);
+                       reinterpret_cast<typename Output<MODE>::type*>(out), counts, ts.sv.next_ray, n_ptr, refill,
+                       parts);
     if (t0) timing_end(ts.timing, MODE == kClosest ? GR_KERNEL_TRACE_CLOSEST : GR_KERNEL_TRACE_SHADOW, t0, s);
     return hipGetLastError();
 }

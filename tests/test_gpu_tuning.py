@@ -20,6 +20,8 @@ TUNINGS = [
     dict(sort_bits=30, lds_stack=32, refill=1),
     dict(lds_stack=8, waves_per_cu=1, refill=64),
     dict(sort_bits=24, waves_per_cu=2, refill=4),
+    dict(sort_bits=18, xcd_ranges=1),
+    dict(xcd_ranges=1, refill=1, waves_per_cu=1),
 ]
 SENTINEL = 0x3F3F3F3F
 
@@ -70,7 +72,7 @@ def test_tuned_closest_hit_bit_exact(tu, gpu_ctx):
     assert np.array_equal(g["counts"], r["counts"])
 
 
-@pytest.mark.parametrize("tu", TUNINGS[1:4], ids=_id)
+@pytest.mark.parametrize("tu", TUNINGS[1:4] + TUNINGS[6:], ids=_id)
 def test_tuned_visibility_exact(tu, gpu_ctx):
     import torch
     print("case tuned visibility", tu, flush=True)
@@ -88,12 +90,13 @@ def test_tuned_visibility_exact(tu, gpu_ctx):
     assert np.array_equal(vis.cpu().numpy(), want.astype(np.int32))
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4097])
-def test_sorted_small_batches(n, gpu_ctx):
-    print("case sorted small", n, flush=True)
+@pytest.mark.parametrize("n", [1, 7, 63, 64, 4095, 4097])
+@pytest.mark.parametrize("xcd", [0, 1])
+def test_sorted_small_batches(n, xcd, gpu_ctx):
+    print("case sorted small", n, xcd, flush=True)
     spec = scenes.box_scene()
     stage = Stage(spec, ctx=gpu_ctx)
-    stage.set_tuning(sort_bits=30)
+    stage.set_tuning(sort_bits=30, xcd_ranges=xcd)
     o = OracleScene(spec)
     rays = scenes.ray_batch(spec, n, seed=n)
     g = _closest(stage, rays)
@@ -111,7 +114,7 @@ def test_sorted_render_is_bit_identical(integrator, gpu_ctx):
     spec = scenes.showcase_scene(n_spheres=60, grid=(40, 40))
     W, H = 64, 48
     films = []
-    for tu in [dict(), dict(sort_bits=18, lds_stack=8, refill=8)]:
+    for tu in [dict(), dict(sort_bits=18, lds_stack=8, refill=8, xcd_ranges=1)]:
         stage = Stage(spec, ctx=gpu_ctx)
         stage.set_tuning(**tu)
         film = Film(gpu_ctx, W, H)
@@ -130,7 +133,7 @@ def test_autotune_picks_a_valid_setting(gpu_ctx):
     tu = stage.autotune(torch.from_numpy(rays).cuda())
     assert tu == stage.get_tuning()
     assert tu["lds_stack"] in (8, 16, 32) and 1 <= tu["refill"] <= 64
-    assert tu["sort_bits"] in (0, 12, 18, 24, 30)
+    assert tu["sort_bits"] in (0, 12, 18, 24, 30) and tu["xcd_ranges"] in (0, 1)
     o = OracleScene(spec)
     g = _closest(stage, rays)
     r = o.intersect(scenes.rays6(rays))
