@@ -45,10 +45,16 @@ class Tuning(ctypes.Structure):
 class TraceParams(ctypes.Structure):
     _fields_ = [("integrator", _u32), ("spp", _u32), ("max_depth", _u32), ("light_depth", _u32),
                 ("seed", ctypes.c_uint64), ("sample_offset", ctypes.c_uint64), ("batch", _u32),
-                ("reserved", _u32), ("d_sample_radiance", _vp)]
+                ("reserved", _u32), ("d_sample_radiance", _vp), ("guide", _vp)]
 
 
-GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT = 0, 1, 2
+class GuideParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("alpha", ctypes.c_float),
+                ("batch", _u32), ("steps_per_pass", _u32), ("train_paths", _u32), ("guided_depth", _u32),
+                ("warmup_steps", _u32), ("reserved", _u32)]
+
+
+GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED = 0, 1, 2, 3
 
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.
 SIGNATURES = {
@@ -78,6 +84,16 @@ SIGNATURES = {
     "gr_occluded": (_i, [_vp, _vp, _u32, _vp, _i, _vp]),
     "gr_scene_take_overflow_count": (_i, [_vp, _u64p]),
     "gr_scene_set_tuning": (_i, [_vp, _vp]),
+    "gr_guide_create": (_i, [_vp, ctypes.POINTER(GuideParams), _vp]),
+    "gr_guide_destroy": (None, [_vp]),
+    "gr_guide_get_params": (_i, [_vp, _vp, _u32]),
+    "gr_guide_set_params": (_i, [_vp, _vp, _u32]),
+    "gr_guide_steps": (_i, [_vp, _u64p]),
+    "gr_guide_infer": (_i, [_vp, _vp, _u32, _vp]),
+    "gr_guide_gradient": (_i, [_vp, _vp, _u32, _vp]),
+    "gr_guide_train": (_i, [_vp, _vp, _u32, _u32]),
+    "gr_guide_mixture_sample": (_i, [_vp, _vp, _vp, _u32, _vp]),
+    "gr_guide_mixture_pdf": (_i, [_vp, _vp, _vp, _u32, _vp]),
     "gr_ctx_set_kernel_timing": (_i, [_vp, _i]),
     "gr_ctx_take_kernel_times": (_i, [_vp, _vp, _vp]),
     "gr_scene_get_tuning": (_i, [_vp, _vp]),

@@ -201,7 +201,7 @@ def hits_to_numpy(hits):
 # Rendering: Film (src/render/film.rs) and Stage::trace (src/render/stage.rs)
 # ---------------------------------------------------------------------------
 INTEGRATORS = {"pt": _lib.GR_INTEGRATOR_PT, "pt_nee": _lib.GR_INTEGRATOR_PT_NEE,
-               "bdpt": _lib.GR_INTEGRATOR_BDPT}
+               "bdpt": _lib.GR_INTEGRATOR_BDPT, "guided": _lib.GR_INTEGRATOR_GUIDED}
 
 
 def make_camera(spec):
@@ -264,10 +264,98 @@ class Film:
             pass
 
 
+GUIDE_PARAMS = 27936
+GUIDE_MIX_FLOATS = 40
+GUIDE_RECORD_FLOATS = 12
+
+
+class Guide:
+    """The learned path guide (an MLP on the matrix cores, trained online)."""
+
+    def __init__(self, ctx, seed=1, learning_rate=5e-3, alpha=0.5, batch=16384, steps_per_pass=4,
+                 train_paths=1 << 16, guided_depth=0, warmup_steps=8):
+        self.ctx = ctx
+        p = _lib.GuideParams(seed, learning_rate, alpha, batch, steps_per_pass, train_paths, guided_depth,
+                             warmup_steps, 0)
+        self._h = ctypes.c_void_p()
+        check(lib().gr_guide_create(ctx.handle, ctypes.byref(p), ctypes.byref(self._h)), ctx.handle,
+              "guide_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self):
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        check(lib().gr_guide_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p), GUIDE_PARAMS),
+              self.ctx.handle, "guide_get_params")
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == GUIDE_PARAMS
+        check(lib().gr_guide_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p), GUIDE_PARAMS),
+              self.ctx.handle, "guide_set_params")
+
+    def steps(self):
+        v = ctypes.c_uint64()
+        check(lib().gr_guide_steps(self._h, ctypes.byref(v)), self.ctx.handle, "guide_steps")
+        return v.value
+
+    def infer(self, raw):
+        """raw: (n, 8) float32 device tensor -> (n, 40) mixtures."""
+        import torch
+        out = torch.empty((raw.shape[0], GUIDE_MIX_FLOATS), dtype=torch.float32, device=raw.device)
+        check(lib().gr_guide_infer(self._h, ctypes.c_void_p(raw.data_ptr()), raw.shape[0],
+                                   ctypes.c_void_p(out.data_ptr())), self.ctx.handle, "guide_infer")
+        return out
+
+    def gradient(self, records):
+        import torch
+        out = torch.empty((GUIDE_PARAMS,), dtype=torch.float32, device=records.device)
+        check(lib().gr_guide_gradient(self._h, ctypes.c_void_p(records.data_ptr()), records.shape[0],
+                                      ctypes.c_void_p(out.data_ptr())), self.ctx.handle, "guide_gradient")
+        return out
+
+    def train(self, records, steps):
+        check(lib().gr_guide_train(self._h, ctypes.c_void_p(records.data_ptr()), records.shape[0], steps),
+              self.ctx.handle, "guide_train")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gr_guide_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mixture_sample(ctx, mix, u):
+    """Directions (n, 4: xyz, pdf) drawn from mixtures (n, 40) with uniforms (n, 3) (device tensors)."""
+    import torch
+    out = torch.empty((mix.shape[0], 4), dtype=torch.float32, device=mix.device)
+    check(lib().gr_guide_mixture_sample(ctx.handle, ctypes.c_void_p(mix.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+                                        mix.shape[0], ctypes.c_void_p(out.data_ptr())), ctx.handle,
+          "mixture_sample")
+    return out
+
+
+def mixture_pdf(ctx, mix, dirs):
+    import torch
+    out = torch.empty((mix.shape[0],), dtype=torch.float32, device=mix.device)
+    check(lib().gr_guide_mixture_pdf(ctx.handle, ctypes.c_void_p(mix.data_ptr()), ctypes.c_void_p(dirs.data_ptr()),
+                                     mix.shape[0], ctypes.c_void_p(out.data_ptr())), ctx.handle, "mixture_pdf")
+    return out
+
+
 def trace(stage, film, integrator="pt", spp=1, max_depth=0, light_depth=0, seed=1, sample_offset=0,
-          batch=0, sample_radiance_ptr=None, camera=None):
+          batch=0, sample_radiance_ptr=None, camera=None, guide=None):
     """spp iterations of Stage::trace(&camera, &integrator, &mut film)."""
     p = _lib.TraceParams()
+    p.guide = guide.handle if guide is not None else None
     p.integrator = INTEGRATORS[integrator] if isinstance(integrator, str) else integrator
     p.spp = spp
     p.max_depth = max_depth

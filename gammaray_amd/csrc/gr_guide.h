@@ -1,0 +1,212 @@
+// gr_guide.h -- device side of the learned path guide: input encoding,
+// parameter layout, MFMA fragment layout of the fp16 weights, and the
+// von Mises-Fisher mixture the network outputs.
+//
+// Network: 8 raw inputs (position in the scene box, outgoing direction and
+// shading normal as (theta/pi, phi/2pi + 1/2), roughness), each encoded as
+// sin/cos(2^k pi x) for k = 0..11 (192 features) plus the 8 raw values, padded
+// to 208; then 208 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 32 outputs = 8 lobes x
+// (weight logit, log kappa, theta, phi).  27,936 parameters.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "gr_math.h"
+
+namespace gr {
+
+constexpr int kGuideRaw = 8;
+constexpr int kGuideFreqs = 12;
+constexpr int kGuideEncoded = kGuideRaw * 2 * kGuideFreqs;  // 192
+constexpr int kGuideIn = 208;                               // 192 + 8 raw + 8 zero
+constexpr int kGuideWidth = 64;
+constexpr int kGuideOut = 32;
+constexpr int kGuideLobes = 8;
+constexpr int kMixFloats = 5 * kGuideLobes;  // per query: lobe i -> (w, kappa, mu.x, mu.y, mu.z)
+constexpr int kRecordFloats = 12;            // training record: raw inputs(8), direction(3), value
+
+// fp32 parameter vector: W_l row-major [in][out], then b_l.
+constexpr int kW1 = 0;
+constexpr int kB1 = kW1 + kGuideIn * kGuideWidth;
+constexpr int kW2 = kB1 + kGuideWidth;
+constexpr int kB2 = kW2 + kGuideWidth * kGuideWidth;
+constexpr int kW3 = kB2 + kGuideWidth;
+constexpr int kB3 = kW3 + kGuideWidth * kGuideWidth;
+constexpr int kW4 = kB3 + kGuideWidth;
+constexpr int kB4 = kW4 + kGuideWidth * kGuideWidth;
+constexpr int kW5 = kB4 + kGuideWidth;
+constexpr int kB5 = kW5 + kGuideWidth * kGuideOut;
+constexpr int kGuideParams = kB5 + kGuideOut;  // 27,936
+
+// MFMA fragments (v_mfma_f32_16x16x16_f16): a 16x16 operand tile is 64 lanes x
+// 4 halves.  A operand: lane L holds row L%16, k = 4(L/16)..+3.  B operand:
+// lane L holds column L%16, k = 4(L/16)..+3.  C/D: lane L holds column L%16,
+// rows 4(L/16)..+3.  Activations are kept transposed (rows = features,
+// columns = 16 queries), so a layer's C tiles are the next layer's B tiles.
+constexpr int kFragHalves = 256;
+// Forward fragments: A = W^T tiles (rows = out, k = in), per layer [out tile][in tile];
+// layer bases 0, 52, 68, 84, 100.
+constexpr int kFwdTiles = 108;
+__host__ __device__ constexpr int guide_fwd_base(int l) { return l == 0 ? 0 : 52 + 16 * (l - 1); }
+// Backward fragments: A = W tiles (rows = in, k = out), layers 2..5, per layer
+// [in tile][out tile]; layer bases 0, 16, 32, 48.
+constexpr int kBwdTiles = 56;
+__host__ __device__ constexpr int guide_bwd_base(int l) { return 16 * (l - 1); }
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int guide_layer_in(int l) { return l == 0 ? kGuideIn : kGuideWidth; }
+__host__ __device__ inline int guide_layer_out(int l) { return l == 4 ? kGuideOut : kGuideWidth; }
+__host__ __device__ inline int guide_w_off(int l) {
+    return l == 0 ? kW1 : (l == 1 ? kW2 : (l == 2 ? kW3 : (l == 3 ? kW4 : kW5)));
+}
+__host__ __device__ inline int guide_b_off(int l) {
+    return l == 0 ? kB1 : (l == 1 ? kB2 : (l == 2 ? kB3 : (l == 3 ? kB4 : kB5)));
+}
+
+// Parameter index -> (layer, in, out) for weights; false for biases.
+__host__ __device__ inline bool guide_weight_coords(int p, int* layer, int* in, int* out) {
+    for (int l = 0; l < 5; ++l) {
+        const int w = guide_w_off(l), n = guide_layer_out(l);
+        if (p >= w && p < guide_b_off(l)) {
+            *layer = l;
+            *in = (p - w) / n;
+            *out = (p - w) % n;
+            return true;
+        }
+    }
+    return false;
+}
+
+// Half index of W_l[in][out] in the forward / backward fragment arrays.
+__host__ __device__ inline int guide_fwd_index(int l, int in, int out) {
+    const int kb_count = guide_layer_in(l) / 16;
+    const int tile = guide_fwd_base(l) + (out / 16) * kb_count + in / 16;
+    const int lane = ((in % 16) / 4) * 16 + out % 16;
+    return tile * kFragHalves + lane * 4 + in % 4;
+}
+__host__ __device__ inline int guide_bwd_index(int l, int in, int out) {
+    const int ob_count = guide_layer_out(l) / 16;
+    const int tile = guide_bwd_base(l) + (in / 16) * ob_count + out / 16;
+    const int lane = ((out % 16) / 4) * 16 + in % 16;
+    return tile * kFragHalves + lane * 4 + out % 4;
+}
+
+__device__ __forceinline__ float guide_pick8(const float x[8], uint32_t d) {
+    float v = x[0];
+    v = d == 1 ? x[1] : v;
+    v = d == 2 ? x[2] : v;
+    v = d == 3 ? x[3] : v;
+    v = d == 4 ? x[4] : v;
+    v = d == 5 ? x[5] : v;
+    v = d == 6 ? x[6] : v;
+    v = d == 7 ? x[7] : v;
+    return v;
+}
+
+// sin(pi a), cos(pi a) of 2^k x, reduced exactly to [0, 2) first.
+__device__ __forceinline__ void guide_sincos(float x, int k, float* s, float* c) {
+    float a = ldexpf(x, k);
+    a = a - 2.0f * floorf(a * 0.5f);
+    *s = sinpif(a);
+    *c = cospif(a);
+}
+
+// Encoded features f0..f0+3 (f0 a multiple of 4) of one query.
+__device__ __forceinline__ half4_t guide_features4(const float x[8], uint32_t f0) {
+    half4_t h;
+    if (f0 >= (uint32_t)(kGuideEncoded + kGuideRaw)) {
+        h[0] = h[1] = h[2] = h[3] = (_Float16)0.0f;
+    } else if (f0 >= (uint32_t)kGuideEncoded) {
+        const uint32_t d = f0 - kGuideEncoded;  // 0 or 4
+        h[0] = (_Float16)guide_pick8(x, d);
+        h[1] = (_Float16)guide_pick8(x, d + 1);
+        h[2] = (_Float16)guide_pick8(x, d + 2);
+        h[3] = (_Float16)guide_pick8(x, d + 3);
+    } else {
+        // feature f = 24 d + 2 k + (0: sin, 1: cos)
+        const uint32_t d = f0 / 24, k = (f0 % 24) / 2;
+        const float xd = guide_pick8(x, d);
+        float s0, c0, s1, c1;
+        guide_sincos(xd, (int)k, &s0, &c0);
+        guide_sincos(xd, (int)k + 1, &s1, &c1);
+        h[0] = (_Float16)s0;
+        h[1] = (_Float16)c0;
+        h[2] = (_Float16)s1;
+        h[3] = (_Float16)c1;
+    }
+    return h;
+}
+
+// Raw inputs of a surface vertex.  lo/inv_ext: scene box.
+__device__ __forceinline__ float guide_angle01(float y, float x) {
+    return atan2f(y, x) * (0.5f * kInvPi) + 0.5f;
+}
+__device__ __forceinline__ void guide_raw_inputs(vec3 p, vec3 wo, vec3 n, float roughness, const float lo[3],
+                                                 const float inv_ext[3], float x[8]) {
+    x[0] = (p.x - lo[0]) * inv_ext[0];
+    x[1] = (p.y - lo[1]) * inv_ext[1];
+    x[2] = (p.z - lo[2]) * inv_ext[2];
+    x[3] = acosf(fmaxf(-1.0f, fminf(1.0f, wo.z))) * kInvPi;
+    x[4] = guide_angle01(wo.y, wo.x);
+    x[5] = acosf(fmaxf(-1.0f, fminf(1.0f, n.z))) * kInvPi;
+    x[6] = guide_angle01(n.y, n.x);
+    x[7] = roughness;
+}
+
+// ---- output head ------------------------------------------------------------
+constexpr float kLogKappaLo = -4.0f, kLogKappaHi = 7.0f;
+
+__device__ __forceinline__ float guide_sigmoid(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+__device__ __forceinline__ vec3 guide_mu(float theta, float phi) {
+    const float st = sinf(theta);
+    return mk(st * cosf(phi), st * sinf(phi), cosf(theta));
+}
+
+// vMF density kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1)).
+__device__ __forceinline__ float vmf_pdf(float kappa, vec3 mu, vec3 w) {
+    const float t = -expm1f(-2.0f * kappa);
+    return kappa / (2.0f * kPi * t) * expf(kappa * (dot(mu, w) - 1.0f));
+}
+
+__device__ __forceinline__ float mixture_pdf(const float* mix, vec3 w) {
+    float p = 0.0f;
+    for (int i = 0; i < kGuideLobes; ++i) {
+        const float* l = mix + 5 * i;
+        p += l[0] * vmf_pdf(l[1], mk(l[2], l[3], l[4]), w);
+    }
+    return p;
+}
+
+// Lobe by u0 (inverse CDF of the weights), then the vMF by inversion of its
+// cosine distribution with a uniform azimuth.
+__device__ __forceinline__ vec3 mixture_sample(const float* mix, float u0, float u1, float u2) {
+    int sel = kGuideLobes - 1;
+    float acc = 0.0f;
+    for (int i = 0; i < kGuideLobes; ++i) {
+        acc += mix[5 * i];
+        if (u0 < acc) {
+            sel = i;
+            break;
+        }
+    }
+    const float* l = mix + 5 * sel;
+    const float kappa = l[1];
+    const vec3 mu = mk(l[2], l[3], l[4]);
+    const float e = expf(-2.0f * kappa);
+    float c = 1.0f + logf(e + u1 * (1.0f - e)) / kappa;
+    c = fmaxf(-1.0f, fminf(1.0f, c));
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    const float ph = 2.0f * kPi * u2;
+    vec3 t, b;
+    // orthonormal basis around mu (Duff et al. 2017)
+    const float sg = mu.z >= 0.0f ? 1.0f : -1.0f;
+    const float aa = -1.0f / (sg + mu.z);
+    const float bb = mu.x * mu.y * aa;
+    t = mk(1.0f + sg * mu.x * mu.x * aa, sg * bb, -sg * mu.x);
+    b = mk(bb, sg + mu.y * mu.y * aa, -mu.y);
+    return mu * c + (t * cosf(ph) + b * sinf(ph)) * s;
+}
+
+}  // namespace gr

@@ -206,6 +206,39 @@ struct gr_scene {
     gr::ShadeView shade{};
 };
 
+struct gr_guide {
+    gr_ctx* ctx = nullptr;
+    gr_guide_params prm{};
+    void* blob = nullptr;  // parameters, Adam moments, gradient, fp16 fragments, counters
+    float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    uint16_t *fwd = nullptr, *bwd = nullptr;  // fp16 MFMA fragments of the weights
+    float* stats = nullptr;                   // [0] sum of record values
+    uint32_t* count = nullptr;                // records available for training
+    uint64_t steps = 0;                       // Adam steps taken
+    // dW operands of one training batch, grown on demand
+    void* work = nullptr;
+    uint32_t work_tiles = 0;
+    uint16_t *at = nullptr, *dz = nullptr;
+    // training records of one traced batch (compacted), grown on demand
+    void* recs_blob = nullptr;
+    size_t recs_cap = 0;
+    float* recs = nullptr;
+};
+
+namespace gr {
+// gr_guide.hip
+// Mixture (kMixFloats each) of the first n (= *n_ptr if given, <= n_max) raw inputs.
+hipError_t guide_infer(gr_guide* g, const float* raw, uint32_t n_max, const uint32_t* n_ptr, float* mix,
+                       hipStream_t s);
+// Sum of the values of the first n records into g->stats[0] (which the caller zeroes).
+hipError_t guide_record_stats(gr_guide* g, const float* recs, uint32_t n_max, const uint32_t* n_ptr, hipStream_t s);
+hipError_t guide_zero_stats(gr_guide* g, hipStream_t s);
+// `steps` Adam steps on batches of g->prm.batch records drawn from the first n records.
+hipError_t guide_adam_steps(gr_guide* g, const float* recs, uint32_t n_max, const uint32_t* n_ptr, uint32_t steps,
+                            hipStream_t s);
+hipError_t guide_reserve_records(gr_guide* g, size_t n);
+}  // namespace gr
+
 // Per-film wavefront buffers, grown on demand by gr_trace.
 struct RenderBuffers {
     size_t capacity = 0;  // paths
@@ -223,6 +256,12 @@ struct RenderBuffers {
     // BDPT vertex storage
     size_t bdpt_capacity = 0;
     void* bdpt_blob = nullptr;
+    // guided tracing: queries, mixtures, per-path query slots, training vertex records
+    size_t guide_capacity = 0, guide_vrecs = 0;
+    void* guide_blob = nullptr;
+    float *graw = nullptr, *gmix = nullptr;
+    uint32_t* gslot = nullptr;
+    float4* vrec = nullptr;
 };
 
 struct gr_film {

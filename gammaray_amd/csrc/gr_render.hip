@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "gr_guide.h"
 #include "gr_internal.h"
 #include "gr_shading.h"
 
@@ -68,6 +69,7 @@ GpuMaterial build_material(const float rec[GR_MAT_FLOATS]) {
     }
     if (kind == 2 || kind == 3) {  // Material::diffuse / mirror
         m.nlobes = 1;
+        m.roughness = kind == 2 ? 1.0f : 0.0f;
         m.lobes[0].type = kind == 2 ? kLobePerfectDiffuse : kLobePerfectMirror;
         return m;
     }
@@ -76,6 +78,7 @@ GpuMaterial build_material(const float rec[GR_MAT_FLOATS]) {
     const float roughness = rec[4], anisotropic = rec[5], ior = rec[6], metallic = rec[7];
     const float specular_trans = rec[8], specular_tint = rec[9], sheen = rec[10], sheen_tint = rec[11];
     const float clearcoat = rec[12], clearcoat_gloss = rec[13];
+    m.roughness = roughness;
     const float diffuse_weight = (1.0f - metallic) * (1.0f - specular_trans);
     const float trans_weight = (1.0f - metallic) * specular_trans;
     if (diffuse_weight > 0.0f) {  // DisneyDiffuseRefl::new, lobes.rs:83-91
@@ -233,7 +236,9 @@ enum { kStreamCamera = 0, kStreamLight = 1, kStreamFilm = 2 };
 // Slots within one bounce's draws: block 0 = slots 0..3, block 1 = 4..7,
 // block 2 = 8..11.
 enum { kSlotLobe = 0, kSlotU1 = 1, kSlotU2 = 2, kSlotRR = 3, kSlotLight = 4, kSlotLightTri = 5,
-       kSlotLightU1 = 6, kSlotLightU2 = 7, kSlotLightDir1 = 8, kSlotLightDir2 = 9 };
+       kSlotLightU1 = 6, kSlotLightU2 = 7, kSlotLightDir1 = 8, kSlotLightDir2 = 9,
+       // guided vertices, block 3: technique choice, mixture lobe, vMF cosine, azimuth
+       kSlotGuideSel = 12, kSlotGuideLobe = 13, kSlotGuideU1 = 14, kSlotGuideU2 = 15 };
 
 // Philox4x32-10 (Salmon et al., SC'11).
 __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
@@ -404,6 +409,22 @@ __device__ __forceinline__ uint32_t wave_append(bool alive, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & lanes_below());
 }
 
+// State of a guided trace (GR_INTEGRATOR_GUIDED).
+constexpr uint32_t kGuideNoQuery = 0xfffffffeu;  // eligible vertex, guide not sampled yet
+constexpr uint32_t kVrecFloat4 = 4;              // vertex record: raw(8) | dir, Lb.x | Lb.yz, T lum, valid
+
+struct GuideArgs {
+    float* raw;        // per query: 8 raw inputs
+    float* mix;        // per query: kMixFloats
+    uint32_t* gslot;   // per path: query index, kGuideNoQuery or kNone
+    float4* vrec;      // per training path and depth
+    float* recs;       // compacted training records
+    uint32_t* rec_count;
+    uint32_t rec_cap, train_stride, train_paths, drec, guided_depth, active;
+    float alpha;
+    float lo[3], inv_ext[3];
+};
+
 struct RenderArgs {
     SceneView sv;
     ShadeView sh;
@@ -412,7 +433,32 @@ struct RenderArgs {
     float cam[16];
     float wx, wy;
     RenderBuffers rb;
+    GuideArgs gd;
 };
+
+// Lobes whose reported pdf is the density their sampling draws from: the
+// guide is mixed with these only.  The clearcoat pdf lacks the cos(theta_h)
+// Jacobian factor (src/material/util.rs:266-308) and the transmission pdf
+// uses an unnormalised half vector (src/material/lobes.rs:337-339); those
+// lobes, and the delta mirror, keep the reference's own sampling.
+__device__ __forceinline__ bool lobe_consistent(const GpuLobe& l) {
+    return l.type == kLobeDisneyDiffuse || l.type == kLobeDisneySpecular ||
+           l.type == kLobePerfectDiffuse;
+}
+
+__device__ __forceinline__ uint32_t consistent_lobes(const GpuMaterial& m) {
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < m.nlobes; ++k) c += lobe_consistent(m.lobes[k]) ? 1u : 0u;
+    return c;
+}
+
+// Vertex-record slot of path p at this depth, or kNone when p records no
+// training data there.
+__device__ __forceinline__ uint32_t vrec_slot(const GuideArgs& g, uint32_t p, uint32_t depth) {
+    if (g.train_stride == 0 || p % g.train_stride != 0 || depth >= g.drec) return kNone;
+    const uint32_t r = p / g.train_stride;
+    return r < g.train_paths ? r * g.drec + depth : kNone;
+}
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -456,8 +502,21 @@ __global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
 
 __global__ void k_reset(uint32_t* c, uint32_t which) { c[which] = 0u; }
 
-// One bounce of PathTracerIntegrator::integrate (src/render/integrators.rs:42-99).
-template <bool NEE>
+// One bounce of PathTracerIntegrator::integrate (src/render/integrators.rs:42-99);
+// with NEE, light sampling and power-heuristic MIS.
+//
+// With GUIDED, the reference's BSDF estimator -- pick one of the n lobes
+// uniformly, sample it, weight f / ((1/n) sum_k pdf_k) -- is split by the
+// picked lobe.  A pick among the nc consistent lobes (see lobe_consistent)
+// estimates the integrand f * S_C / S_all (S_C, S_all: sums of the consistent
+// and of all reported lobe pdfs); there the direction comes from the guide's
+// mixture q with probability alpha and from the picked lobe otherwise, weighted
+// by the one-sample MIS density (nc/n)(alpha q + (1 - alpha) S_C / nc).  Any
+// other pick is the reference's sample as is.  Both parts keep their means, so
+// the guided estimator converges to the plain estimator's image, including the
+// reference's pdf quirks; the NEE and emission MIS weights use the reported
+// BSDF pdf exactly as without the guide.
+template <bool NEE, bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.rb.counts[cur];
@@ -499,6 +558,13 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
                         L = L + mul(T, le) * power_heuristic(aux.x, pl);
                     }
                 }
+                uint32_t gs = kNone;
+                const float* mix = nullptr;
+                if (GUIDED) {
+                    gs = a.gd.gslot[p];
+                    if (gs < kGuideNoQuery) mix = a.gd.mix + (size_t)gs * kMixFloats;
+                }
+                const float alpha = mix ? a.gd.alpha : 0.0f;
                 if (m.nlobes != 0) {
                     if (a.sh.num_lights > 0 && dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0) {
                         const LightPoint lp = sample_light_point(a.sh, draws(a.seed, path, kStreamCamera, depth, 1));
@@ -520,11 +586,58 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
                             }
                         }
                     }
-                    const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
-                    T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
-                    a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
-                    new_o = hp + smp.outgoing * kRayPush;
-                    new_d = smp.outgoing;
+                    const vec3 T_in = T;
+                    vec3 out_dir;
+                    // the lobe the reference's sample_world picks with this draw
+                    const uint32_t pick = (uint32_t)(((uint64_t)r0.x * m.nlobes) >> 32);
+                    const bool split = GUIDED && gs != kNone && lobe_consistent(m.lobes[pick]);
+                    if (!split) {
+                        const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                        T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
+                        a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
+                        out_dir = smp.outgoing;
+                    } else {
+                        const uint4 rg = draws(a.seed, path, kStreamCamera, depth, kSlotGuideSel / 4);
+                        if (mix && u01(rg.x) < alpha) {
+                            out_dir = mixture_sample(mix, u01(rg.y), u01(rg.z), u01(rg.w));
+                        } else {
+                            out_dir = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z)).outgoing;
+                        }
+                        const vec3 il = dev::to_local(wi, f), ol = dev::to_local(out_dir, f);
+                        float s_c = 0.0f, s_all = 0.0f;
+                        uint32_t nc = 0;
+                        if (il.z != 0.0f) {
+                            for (uint32_t k = 0; k < m.nlobes; ++k) {
+                                const float pk = dev::lobe_pdf(m.lobes[k], il, ol);
+                                s_all += pk;
+                                if (lobe_consistent(m.lobes[k])) {
+                                    s_c += pk;
+                                    ++nc;
+                                }
+                            }
+                        } else {
+                            nc = consistent_lobes(m);
+                        }
+                        const float q = mix ? mixture_pdf(mix, out_dir) : 0.0f;
+                        const float dens = ((float)nc / (float)m.nlobes) * (alpha * q + (1.0f - alpha) * (s_c / (float)nc));
+                        vec3 g = mk(0.0f, 0.0f, 0.0f);
+                        if (s_all > 0.0f && dens > 0.0f)
+                            g = dev::f_world(m, wi, out_dir, f, true) * ((s_c / s_all) * fabsf(dot(f.n, out_dir)) / dens);
+                        T = mul(T, g);
+                        a.rb.aux[p] = make_float4(s_all / (float)m.nlobes, 0.0f, 0.0f, 0.0f);
+                    }
+                    if (split) {
+                        // training record: the direction and the radiance so far (this
+                        // vertex's NEE is added when its shadow ray resolves)
+                        const uint32_t vs = vrec_slot(a.gd, p, depth);
+                        if (vs != kNone) {
+                            float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
+                            vr[2] = make_float4(out_dir.x, out_dir.y, out_dir.z, L.x);
+                            vr[3] = make_float4(L.y, L.z, dev::luminance(T_in), 1.0f);
+                        }
+                    }
+                    new_o = hp + out_dir * kRayPush;
+                    new_d = out_dir;
                     if (depth > 10 || dev::nearly_zero3(T)) russian_roulette(T, depth, u01(r0.w));
                     alive = !dev::is_zero(T);
                 }
@@ -552,7 +665,8 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
     }
 }
 
-__global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a) {
+template <bool GUIDED>
+__global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a, uint32_t depth) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.rb.counts[2] || a.rb.vis[j] == 0) return;
     const uint32_t p = a.rb.seg_path[j];
@@ -562,6 +676,93 @@ __global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a) {
     L.y = L.y + c.y;
     L.z = L.z + c.z;
     a.rb.L[p] = L;
+    if (GUIDED) {
+        // this vertex's own light sample is not part of the radiance its
+        // sampled direction brings in
+        const uint32_t vs = vrec_slot(a.gd, p, depth);
+        if (vs != kNone) {
+            float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
+            if (vr[3].w != 0.0f) {
+                vr[2].w = vr[2].w + c.x;
+                vr[3].x = vr[3].x + c.y;
+                vr[3].y = vr[3].y + c.z;
+            }
+        }
+    }
+}
+
+// Guide inputs of the vertices the guide covers (a material with a consistent
+// lobe, depth < guided_depth): queries when the guide is sampled, vertex
+// records for training paths.
+__global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t depth, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.rb.counts[cur];
+    bool eligible = false;
+    uint32_t p = 0;
+    float x[8];
+    if (i < n) {
+        p = a.rb.queue[cur][i];
+        const uint4 hv = a.rb.hits[i];
+        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
+        if (h.prim != kNone && (a.gd.guided_depth == 0 || depth < a.gd.guided_depth)) {
+            const GpuMaterial& m = a.sh.mats[h.prim];
+            if (consistent_lobes(m) != 0) {
+                eligible = true;
+                const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
+                const Frame f = hit_frame(a.sv, a.sh, h, o, d);
+                guide_raw_inputs(o + d * h.t, -d, f.n, m.roughness, a.gd.lo, a.gd.inv_ext, x);
+            }
+        }
+    }
+    const bool query = eligible && a.gd.active;
+    const uint32_t qi = wave_append(query, &a.rb.counts[3]);
+    if (i < n) a.gd.gslot[p] = query ? qi : (eligible ? kGuideNoQuery : kNone);
+    if (query) {
+        float4* r = reinterpret_cast<float4*>(a.gd.raw + (size_t)qi * 8);
+        r[0] = make_float4(x[0], x[1], x[2], x[3]);
+        r[1] = make_float4(x[4], x[5], x[6], x[7]);
+    }
+    if (eligible) {
+        const uint32_t vs = vrec_slot(a.gd, p, depth);
+        if (vs != kNone) {
+            float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
+            vr[0] = make_float4(x[0], x[1], x[2], x[3]);
+            vr[1] = make_float4(x[4], x[5], x[6], x[7]);
+        }
+    }
+}
+
+// Vertex records -> training records (raw, direction, f cos Li / pdf as
+// luminance: the radiance the path gathered after the vertex, divided by the
+// throughput it arrived with).  Clears the records for the next batch.
+__global__ __launch_bounds__(256) void k_guide_finalize(RenderArgs a, uint32_t nslots) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    float4 r0, r1, r2, r3;
+    float v = 0.0f;
+    if (s < nslots) {
+        float4* vr = a.gd.vrec + (size_t)s * kVrecFloat4;
+        r3 = vr[3];
+        if (r3.w != 0.0f) {
+            r0 = vr[0];
+            r1 = vr[1];
+            r2 = vr[2];
+            vr[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const uint32_t p = (s / a.gd.drec) * a.gd.train_stride;
+            const float4 L = p < a.npaths ? a.rb.L[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const vec3 dl = mk(L.x - r2.w, L.y - r3.x, L.z - r3.y);
+            const float tl = r3.z;
+            v = tl > 0.0f ? fmaxf(dev::luminance(dl), 0.0f) / tl : 0.0f;
+            ok = (v == v) && v < 3.0e38f;
+        }
+    }
+    const uint32_t k = wave_append(ok, a.gd.rec_count);
+    if (ok && k < a.gd.rec_cap) {
+        float* o = a.gd.recs + (size_t)k * kRecordFloats;
+        o[0] = r0.x, o[1] = r0.y, o[2] = r0.z, o[3] = r0.w;
+        o[4] = r1.x, o[5] = r1.y, o[6] = r1.z, o[7] = r1.w;
+        o[8] = r2.x, o[9] = r2.y, o[10] = r2.z, o[11] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_export_samples(RenderArgs a, float4* out) {
@@ -1097,12 +1298,52 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
+namespace {
+// Guided-trace buffers of a film: per path a query (raw inputs, mixture) and
+// a slot, plus the training vertex records (zeroed once; each batch's
+// finalize clears what it read).
+hipError_t reserve_guide(RenderBuffers& rb, size_t paths, size_t vrecs) {
+    if (rb.guide_capacity >= paths && rb.guide_vrecs >= vrecs) return hipSuccess;
+    paths = std::max(paths, rb.guide_capacity);
+    vrecs = std::max(vrecs, rb.guide_vrecs);
+    (void)hipFree(rb.guide_blob);
+    rb.guide_blob = nullptr;
+    rb.guide_capacity = rb.guide_vrecs = 0;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_raw = off; off = align(off + paths * 8 * 4);
+    const size_t o_mix = off; off = align(off + paths * kMixFloats * 4);
+    const size_t o_slot = off; off = align(off + paths * 4);
+    const size_t o_vrec = off; off = align(off + vrecs * kVrecFloat4 * 16);
+    char* b = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b), off);
+    if (e != hipSuccess) return e;
+    e = hipMemset(b + o_vrec, 0, vrecs * kVrecFloat4 * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(b);
+        return e;
+    }
+    rb.guide_blob = b;
+    rb.guide_capacity = paths;
+    rb.guide_vrecs = vrecs;
+    rb.graw = reinterpret_cast<float*>(b + o_raw);
+    rb.gmix = reinterpret_cast<float*>(b + o_mix);
+    rb.gslot = reinterpret_cast<uint32_t*>(b + o_slot);
+    rb.vrec = reinterpret_cast<float4*>(b + o_vrec);
+    return hipSuccess;
+}
+}  // namespace
+
 hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_params& prm, gr_film* film) {
     hipStream_t s = sc->ctx->stream;
     const uint32_t W = film->width, H = film->height;
     const size_t npx = (size_t)W * H;
+    const bool guided = prm.integrator == GR_INTEGRATOR_GUIDED;
+    gr_guide* guide = guided ? prm.guide : nullptr;
+    if (guided && !guide) return hipErrorInvalidValue;
     uint32_t batch = prm.batch;
-    if (batch == 0) batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, (1u << 24) / npx));
+    if (batch == 0)
+        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, (guided ? (1u << 21) : (1u << 24)) / npx));
     batch = std::min(batch, prm.spp);
     const size_t paths = npx * batch;
     if (paths > 0x7fffffffull) return hipErrorInvalidValue;
@@ -1121,7 +1362,34 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     a.wx = cam.horizontal_aperture / (cam.focal_length * 2.0f);  // window_max, camera.rs:76-79
     a.wy = cam.vertical_aperture / (cam.focal_length * 2.0f);
     a.rb = film->rb;
-    const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
+    std::memset(&a.gd, 0, sizeof(a.gd));
+    const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE || guided;
+    uint32_t nslots = 0;
+    if (guided) {
+        const gr_guide_params& gp = guide->prm;
+        const uint32_t drec = prm.max_depth == 0 ? 16u : std::min(prm.max_depth, 16u);
+        nslots = gp.train_paths * drec;
+        e = reserve_guide(film->rb, paths, nslots);
+        if (e == hipSuccess) e = guide_reserve_records(guide, nslots);
+        if (e != hipSuccess) return e;
+        a.rb = film->rb;
+        a.gd.raw = film->rb.graw;
+        a.gd.mix = film->rb.gmix;
+        a.gd.gslot = film->rb.gslot;
+        a.gd.vrec = film->rb.vrec;
+        a.gd.recs = guide->recs;
+        a.gd.rec_count = guide->count;
+        a.gd.rec_cap = (uint32_t)guide->recs_cap;
+        a.gd.train_paths = gp.train_paths;
+        a.gd.drec = drec;
+        a.gd.guided_depth = gp.guided_depth;
+        a.gd.alpha = gp.alpha;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = sc->view.root_hi[k] - sc->view.root_lo[k];
+            a.gd.lo[k] = sc->view.root_lo[k];
+            a.gd.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+        }
+    }
     if (prm.integrator == GR_INTEGRATOR_BDPT) {
         const uint32_t cmax = (prm.max_depth == 0 || prm.max_depth > kBdptMaxDepth) ? kBdptMaxDepth : prm.max_depth;
         const uint32_t lmax =
@@ -1146,6 +1414,10 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         const uint32_t nb = std::min(batch, prm.spp - done);
         a.npaths = (uint32_t)(npx * nb);
         a.path_base = (prm.sample_offset + done) * (uint64_t)npx;
+        if (guided) {
+            a.gd.active = guide->steps >= guide->prm.warmup_steps ? 1u : 0u;
+            a.gd.train_stride = std::max<uint32_t>(1u, a.npaths / std::max<uint32_t>(1u, guide->prm.train_paths));
+        }
         hipLaunchKernelGGL(k_camera, blocks_for(a.npaths), dim3(256), 0, s, a);
         uint32_t cur = 0;
         for (uint32_t depth = 0; prm.max_depth == 0 || depth < prm.max_depth; ++depth) {
@@ -1162,18 +1434,41 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, cur ^ 1u);
             if (nee) {
                 hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 2u);
-                hipLaunchKernelGGL(k_shade<true>, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                if (guided) {
+                    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 3u);
+                    hipLaunchKernelGGL(k_guide_query, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                    if (a.gd.active) e = guide_infer(guide, a.gd.raw, a.npaths, film->rb.counts + 3, a.gd.mix, s);
+                    if (e != hipSuccess) break;
+                    hipLaunchKernelGGL((k_shade<true, true>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                } else {
+                    hipLaunchKernelGGL((k_shade<true, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                }
                 e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
                                     film->rb.counts + 2, film->rb.vis, GR_OCCLUSION_EXACT, nullptr, s);
                 if (e != hipSuccess) break;
-                hipLaunchKernelGGL(k_shadow_accum, blocks_for(a.npaths), dim3(256), 0, s, a);
+                if (guided)
+                    hipLaunchKernelGGL((k_shadow_accum<true>), blocks_for(a.npaths), dim3(256), 0, s, a, depth);
+                else
+                    hipLaunchKernelGGL((k_shadow_accum<false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth);
             } else {
-                hipLaunchKernelGGL(k_shade<false>, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                hipLaunchKernelGGL((k_shade<false, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
             }
             e = hipGetLastError();
             cur ^= 1u;
         }
         if (e != hipSuccess) break;
+        if (guided) {
+            // train on this batch's records before the next batch samples the guide
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, guide->count, 0u);
+            e = guide_zero_stats(guide, s);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(k_guide_finalize, blocks_for(nslots), dim3(256), 0, s, a, nslots);
+            e = guide_record_stats(guide, guide->recs, (uint32_t)guide->recs_cap, guide->count, s);
+            if (e == hipSuccess)
+                e = guide_adam_steps(guide, guide->recs, (uint32_t)guide->recs_cap, guide->count,
+                                     guide->prm.steps_per_pass, s);
+            if (e != hipSuccess) break;
+        }
         if (prm.d_sample_radiance)
             hipLaunchKernelGGL(k_export_samples, blocks_for(a.npaths), dim3(256), 0, s, a,
                                reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx);
@@ -1192,6 +1487,7 @@ hipError_t material_query(gr_scene* sc, uint32_t prim, const float* in, uint32_t
 void release_render_buffers(RenderBuffers& rb) {
     (void)hipFree(rb.blob);
     (void)hipFree(rb.bdpt_blob);
+    (void)hipFree(rb.guide_blob);
     rb = RenderBuffers{};
 }
 

@@ -44,7 +44,9 @@ struct GpuLobe {
 };
 
 struct GpuMaterial {
-    uint32_t nlobes, has_light, pad0, pad1;
+    uint32_t nlobes, has_light;
+    float roughness;  // the builder's roughness (guide input): Disney roughness, 1 diffuse, 0 mirror
+    uint32_t pad1;
     float light[4];
     GpuLobe lobes[4];
 };

@@ -193,26 +193,56 @@ def _box_quads(lo, hi, inward):
 # --------------------------------------------------------------------------
 # 256x256 box scene (plain path tracing)
 # --------------------------------------------------------------------------
-def box_scene(seed=1):
+def box_scene(seed=1, consistent=False, full=False):
+    """consistent=True zeroes clearcoat, the one lobe of these materials whose
+    reported pdf is not its sampling density (GTR1, src/material/util.rs:
+    266-308); full=True draws every Disney parameter (adds transmission)."""
     rng = np.random.default_rng(seed)
     s = SceneSpec(name="box256")
+
+    def mat():
+        m = random_disney(rng, full=full)
+        if consistent:
+            m[12] = 0.0
+        return m
     for verts, faces in _box_quads((-1.0, 0.0, -1.0), (1.0, 2.0, 1.0), inward=True):
-        s.add_mesh(verts, faces, random_disney(rng))
+        s.add_mesh(verts, faces, mat())
     bv, bf = [], []
     for k, (verts, faces) in enumerate(_box_quads((-0.7, 0.002, -0.7), (-0.15, 0.7, -0.2), inward=False)):
         bv.append(verts)
         bf.append(faces + 4 * k)
-    s.add_mesh(np.concatenate(bv), np.concatenate(bf), random_disney(rng))
+    s.add_mesh(np.concatenate(bv), np.concatenate(bf), mat())
     for _ in range(3):
         r = 0.15 + 0.2 * rng.random()
         c = np.array([rng.uniform(-0.9 + r, 0.9 - r), r + rng.uniform(0.0, 1.0),
                       rng.uniform(-0.9 + r, 0.2)])
-        s.add_sphere(c, r, random_disney(rng))
+        s.add_sphere(c, r, mat())
     lv, lf = quad((0.0, 1.98, 0.0), (0.3, 0.0, 0.0), (0.0, 0.0, 0.3), (0.0, -1.0, 0.0))
     s.add_mesh(lv, lf, diffuse_light((15.0, 15.0, 15.0)))
     # Square film: equal apertures; ~63 degree field of view looking down -z.
     s.camera = CameraSpec(focal_length=0.8, horizontal_aperture=1.0, vertical_aperture=1.0,
                           translate=(0.0, 1.0, 0.98))
+    return s
+
+
+def indirect_room_scene(seed=5):
+    """A room lit only indirectly: the area light under the ceiling emits
+    upwards, so every surface below it is lit by the bright patch the light
+    makes on the ceiling -- light sampling finds nothing from below and BSDF
+    sampling rarely hits the patch, the case a learned guide is for.
+    Materials are rough Disney dielectrics without clearcoat."""
+    rng = np.random.default_rng(seed)
+    s = SceneSpec(name="indirect_room")
+    for verts, faces in _box_quads((-1.0, 0.0, -1.0), (1.0, 1.0, 1.0), inward=True):
+        s.add_mesh(verts, faces, disney(base_color=0.4 + 0.5 * rng.random(3), roughness=0.9))
+    lv, lf = quad((0.4, 0.9, 0.4), (0.15, 0.0, 0.0), (0.0, 0.0, 0.15), (0.0, 1.0, 0.0))
+    s.add_mesh(lv, lf, diffuse_light((40.0, 40.0, 40.0)))
+    for _ in range(2):
+        r = 0.12 + 0.1 * rng.random()
+        s.add_sphere((rng.uniform(-0.6, 0.6), r, rng.uniform(-0.6, 0.6)), r,
+                     disney(base_color=rng.random(3), roughness=0.6 + 0.4 * rng.random()))
+    s.camera = CameraSpec(focal_length=0.8, horizontal_aperture=1.0, vertical_aperture=1.0,
+                          translate=(0.0, 0.5, 0.98))
     return s
 
 

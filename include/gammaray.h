@@ -169,6 +169,10 @@ typedef struct {
 #define GR_INTEGRATOR_PT 0      /* PathTracerIntegrator, src/render/integrators.rs:39-100 */
 #define GR_INTEGRATOR_PT_NEE 1  /* path tracing + next-event estimation with MIS (no reference counterpart) */
 #define GR_INTEGRATOR_BDPT 2    /* BdptIntegrator, src/render/integrators.rs:102-418 */
+#define GR_INTEGRATOR_GUIDED 3  /* PT_NEE whose BSDF sampling is mixed (one-sample MIS) with a
+                                   learned guide trained online (no reference counterpart) */
+
+typedef struct gr_guide gr_guide;
 
 typedef struct {
     uint32_t integrator;
@@ -180,6 +184,7 @@ typedef struct {
     uint32_t batch;          /* iterations traced together; 0 = automatic */
     uint32_t reserved;
     float *d_sample_radiance; /* optional device buffer: spp*W*H float4 per-sample radiance */
+    gr_guide *guide;         /* GR_INTEGRATOR_GUIDED: the guide to sample and train */
 } gr_trace_params;
 
 /* FilmPixel grid (src/render/film.rs:27-53): per pixel (accum rgb, weight). */
@@ -196,6 +201,47 @@ int gr_trace(gr_scene *scene, const gr_camera *camera, const gr_trace_params *pa
 /* ExrWriter::update + write (src/render/exr.rs:168-199): scanline EXR, B,G,R
  * float channels, rows flipped, value = accum / weight. */
 int gr_film_write_exr(gr_film *film, const char *path);
+
+/* ---- learned path guide (no reference counterpart) -----------------------
+ * A 5-layer MLP (fp16 weights, fp32 accumulation on the matrix cores) maps a
+ * surface vertex (position, outgoing direction, shading normal, roughness) to
+ * a mixture of 8 von Mises-Fisher lobes over the incident direction.  During a
+ * GR_INTEGRATOR_GUIDED trace, guided vertices draw their direction from the
+ * mixture with probability alpha and from the BSDF otherwise, weighted by the
+ * one-sample MIS density alpha*q + (1-alpha)*p_bsdf (unbiased for any network);
+ * after each traced batch the network takes Adam steps on that batch's
+ * (vertex, direction, f*cos*Li/pdf) records, minimising the KL divergence from
+ * the product of BSDF and incident radiance to the mixture. */
+typedef struct {
+    uint64_t seed;            /* weight initialisation and training-batch draws */
+    float learning_rate;      /* Adam step size */
+    float alpha;              /* probability of sampling the guide at a guided vertex */
+    uint32_t batch;           /* training records per Adam step (multiple of 16) */
+    uint32_t steps_per_pass;  /* Adam steps after each traced batch of iterations */
+    uint32_t train_paths;     /* paths per traced batch that record training data */
+    uint32_t guided_depth;    /* vertices at depth < guided_depth are guided; 0 = all */
+    uint32_t warmup_steps;    /* Adam steps taken before the guide is sampled */
+    uint32_t reserved;
+} gr_guide_params;
+#define GR_GUIDE_PARAMS 27936  /* fp32 parameters: W1[208][64] b1 W2[64][64] b2 W3 b3 W4 b4 W5[64][32] b5 */
+#define GR_GUIDE_MIX_FLOATS 40 /* per query: 8 lobes x (weight, kappa, mean x, y, z) */
+#define GR_GUIDE_RECORD_FLOATS 12 /* training record: 8 raw inputs, direction (3), value */
+int gr_guide_create(gr_ctx *ctx, const gr_guide_params *params, gr_guide **out);
+void gr_guide_destroy(gr_guide *guide);
+/* Host copies of the fp32 parameters (n = GR_GUIDE_PARAMS); set also resets Adam. */
+int gr_guide_get_params(gr_guide *guide, float *host, uint32_t n);
+int gr_guide_set_params(gr_guide *guide, const float *host, uint32_t n);
+int gr_guide_steps(gr_guide *guide, uint64_t *adam_steps);
+/* Mixture of n vertices from their 8 raw inputs each (device memory). */
+int gr_guide_infer(gr_guide *guide, const float *d_raw, uint32_t n, float *d_mix);
+/* Mean over n records (device) of the gradient of -(v / mean v) log q(direction). */
+int gr_guide_gradient(gr_guide *guide, const float *d_records, uint32_t n, float *d_grad);
+/* `steps` Adam steps on batches drawn with replacement from n records (device). */
+int gr_guide_train(gr_guide *guide, const float *d_records, uint32_t n, uint32_t steps);
+/* Test hooks: draw directions from n mixtures with 3 uniforms each (out: x, y, z,
+ * pdf), and evaluate mixture densities of given directions. */
+int gr_guide_mixture_sample(gr_ctx *ctx, const float *d_mix, const float *d_u, uint32_t n, float *d_out);
+int gr_guide_mixture_pdf(gr_ctx *ctx, const float *d_mix, const float *d_dirs, uint32_t n, float *d_pdf);
 
 /* Per-function material evaluation on the device (test hook): n items of 24
  * floats in d_in -- wi(3), wo(3), normal, tangent, binormal, geom_normal (12),
