@@ -48,6 +48,16 @@ class TraceParams(ctypes.Structure):
                 ("reserved", _u32), ("d_sample_radiance", _vp), ("guide", _vp)]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("scene", _vp), ("film", _vp), ("camera", Camera), ("spp", _u32), ("max_depth", _u32),
+                ("seed", ctypes.c_uint64), ("sample_offset", ctypes.c_uint64)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("integrator", _u32), ("path_pool", _u32), ("steps_per_poll", _u32), ("use_graph", _u32),
+                ("tuning", Tuning)]
+
+
 class GuideParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("alpha", ctypes.c_float),
                 ("batch", _u32), ("steps_per_pass", _u32), ("train_paths", _u32), ("guided_depth", _u32),
@@ -84,6 +94,7 @@ SIGNATURES = {
     "gr_occluded": (_i, [_vp, _vp, _u32, _vp, _i, _vp]),
     "gr_scene_take_overflow_count": (_i, [_vp, _u64p]),
     "gr_scene_set_tuning": (_i, [_vp, _vp]),
+    "gr_render_jobs": (_i, [_vp, _vp, _u32, ctypes.POINTER(BatchParams), _vp]),
     "gr_guide_create": (_i, [_vp, ctypes.POINTER(GuideParams), _vp]),
     "gr_guide_destroy": (None, [_vp]),
     "gr_guide_get_params": (_i, [_vp, _vp, _u32]),

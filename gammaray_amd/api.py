@@ -369,6 +369,35 @@ def trace(stage, film, integrator="pt", spp=1, max_depth=0, light_depth=0, seed=
           stage.ctx.handle, "trace")
 
 
+def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, use_graph=True, tuning=None):
+    """Render many jobs through one shared pool of in-flight paths.  jobs: list
+    of dicts with stage, film, spp, max_depth, seed and optionally
+    sample_offset and camera (a CameraSpec; default: the stage's).  Returns the
+    per-job latency in ms (batch start to film final)."""
+    n = len(jobs)
+    arr = (_lib.Job * n)()
+    for i, j in enumerate(jobs):
+        arr[i].scene = j["stage"].handle
+        arr[i].film = j["film"].handle
+        arr[i].camera = make_camera(j.get("camera") or j["stage"].spec.camera)
+        arr[i].spp = j["spp"]
+        arr[i].max_depth = j.get("max_depth", 0)
+        arr[i].seed = j.get("seed", 1)
+        arr[i].sample_offset = j.get("sample_offset", 0)
+    p = _lib.BatchParams()
+    p.integrator = INTEGRATORS[integrator] if isinstance(integrator, str) else integrator
+    p.path_pool = path_pool
+    p.steps_per_poll = steps_per_poll
+    p.use_graph = 1 if use_graph else 0
+    if tuning:
+        t = dict(sort_bits=0, lds_stack=16, waves_per_cu=0, refill=16, xcd_ranges=0)
+        t.update(tuning)
+        p.tuning = _lib.Tuning(**t)
+    ms = (ctypes.c_double * n)()
+    check(lib().gr_render_jobs(ctx.handle, arr, n, ctypes.byref(p), ms), ctx.handle, "render_jobs")
+    return list(ms)
+
+
 def material_query(stage, prim, items):
     """Device evaluation of f_world / pdf_world / light_world / sample_world
     (layout documented at gr_material_query)."""

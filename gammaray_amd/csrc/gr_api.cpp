@@ -566,6 +566,32 @@ int gr_trace(gr_scene* s, const gr_camera* cam, const gr_trace_params* prm, gr_f
     return e == hipSuccess ? GR_OK : hip_fail(ctx, e, "trace");
 }
 
+int gr_render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_batch_params* prm, double* job_ms) {
+    if (!ctx || !prm || (njobs && !jobs)) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    if (njobs == 0) return GR_OK;
+    if (prm->integrator != GR_INTEGRATOR_PT && prm->integrator != GR_INTEGRATOR_PT_NEE)
+        return fail(ctx, GR_ERR_INVALID, "batched jobs use GR_INTEGRATOR_PT or GR_INTEGRATOR_PT_NEE%s");
+    if (prm->tuning.lds_stack != 0 && prm->tuning.lds_stack != 8 && prm->tuning.lds_stack != 16 &&
+        prm->tuning.lds_stack != 32)
+        return fail(ctx, GR_ERR_INVALID, "tuning out of range%s");
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const gr_job& jb = jobs[j];
+        if (!jb.scene || !jb.film) return fail(ctx, GR_ERR_INVALID, "job without scene or film%s");
+        if (jb.scene->ctx != ctx || jb.film->ctx != ctx)
+            return fail(ctx, GR_ERR_INVALID, "job scene or film belongs to another context%s");
+        if (!jb.scene->committed) return fail(ctx, GR_ERR_STATE, "scene not committed%s");
+        if (prm->integrator == GR_INTEGRATOR_PT_NEE && jb.scene->shade.num_lights == 0)
+            return fail(ctx, GR_ERR_INVALID, "integrator samples lights but a scene has none%s");
+        if (!(jb.camera.focal_length > 0.0f)) return fail(ctx, GR_ERR_INVALID, "camera focal length must be positive%s");
+        if ((uint64_t)jb.film->width * jb.film->height * jb.spp > 0xffffffffull)
+            return fail(ctx, GR_ERR_INVALID, "a job has more than 2^32 samples%s");
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = render_jobs(ctx, jobs, njobs, *prm, job_ms);
+    return e == hipSuccess ? GR_OK : hip_fail(ctx, e, "batched render");
+}
+
 int gr_film_write_exr(gr_film* f, const char* path) {
     if (!f || !path) return GR_ERR_INVALID;
     std::vector<float> px((size_t)f->width * f->height * 4);

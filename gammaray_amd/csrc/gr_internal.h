@@ -95,6 +95,15 @@ struct GpuLayout {
 };
 GpuLayout make_gpu_layout(const FlatTree& t, const std::vector<Prim>& prims);
 
+// One scene of a multi-scene traversal: the per-scene part of SceneView.
+struct SceneRef {
+    const GpuNode* nodes;
+    const GpuComp* comps;
+    const uint2* comp_ids;
+    uint32_t root_ref;
+    float root_lo[3], root_hi[3];
+};
+
 // ---- ray sort (gr_sort.hip) -------------------------------------------------
 struct SortScratch {
     size_t cap = 0;  // keys
@@ -113,6 +122,11 @@ void release_sort(SortScratch& ss);
 // sorted order, or nullptr when bits == 0.  ss must hold n_max keys.
 hipError_t sort_rays(const SceneView& sv, const float4* in, uint32_t n_max, const uint32_t* n_ptr, bool segments,
                      uint32_t bits, SortScratch& ss, const uint32_t** perm, hipStream_t s);
+// Multi-scene form: the key's top bits are the ray's scene, the origin is
+// quantised to that scene's box (device table of nscenes entries).
+hipError_t sort_rays_multi(const SceneRef* scenes, uint32_t nscenes, const float4* in, uint32_t n_max,
+                           const uint32_t* n_ptr, bool segments, uint32_t bits, SortScratch& ss,
+                           const uint32_t** perm, hipStream_t s);
 
 // ---- launchers (gr_traverse.hip) -------------------------------------------
 // Settings of the traversal launch (gr_tuning).
@@ -126,12 +140,17 @@ struct TraceTuning {
 // Ray counters of the persistent traversal: one per XCD range.
 constexpr uint32_t kRayCounters = 8;
 
-// Everything a traversal launch needs besides its rays.
+// Everything a traversal launch needs besides its rays.  With `scenes` set,
+// every ray names its scene in the pad word after its origin (an index into
+// `scenes`, nscenes entries); sv then only supplies the work counter, the
+// overflow counter and the slab padding.
 struct TraceSetup {
     SceneView sv;
     TraceTuning tu;
     SortScratch* sort;       // holds >= n_max keys when tu.sort_bits != 0
     gr_ctx* timing = nullptr;  // set when kernel timing is on
+    const SceneRef* scenes = nullptr;
+    uint32_t nscenes = 0;
 };
 
 // Kernel timing (gr_api.cpp): brackets launches with events on stream s.
@@ -278,6 +297,7 @@ inline TraceSetup trace_setup(gr_scene* s) {
 // gr_render.hip
 hipError_t upload_shading(gr_scene* s, std::vector<GpuPrim>* prims_out);
 hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_params& prm, gr_film* film);
+hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_batch_params& prm, double* job_ms);
 hipError_t material_query(gr_scene* sc, uint32_t prim, const float* in, uint32_t n, float* out);
 void release_render_buffers(RenderBuffers& rb);
 // gr_exr.cpp

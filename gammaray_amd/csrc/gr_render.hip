@@ -464,31 +464,44 @@ __device__ __forceinline__ uint32_t vrec_slot(const GuideArgs& g, uint32_t p, ui
 // Kernels
 // ---------------------------------------------------------------------------
 
-// Sample point (film.rs:55-76) and camera ray (camera.rs:87-97) per path.
-__global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.npaths) return;
-    const uint32_t npx = a.width * a.height;
-    const uint32_t i = p % npx;
-    const uint32_t row = i / a.width, col = i % a.width;
-    const uint64_t path = a.path_base + p;
-    const uint4 r = draws(a.seed, path, kStreamFilm, 0, 0);
+struct CameraSample {
+    float2 st;
+    vec3 o, d;
+};
+
+// Sample point (film.rs:55-76) and camera ray (camera.rs:87-97) of one path.
+__device__ __forceinline__ CameraSample camera_sample(const float* m, float wx, float wy, uint32_t width,
+                                                      uint32_t height, uint64_t seed, uint64_t path, uint32_t row,
+                                                      uint32_t col) {
+    CameraSample c;
+    const uint4 r = draws(seed, path, kStreamFilm, 0, 0);
     const float rj = (0.5f + (float)row) + (-2.0f + 2.0f * 2.0f * u01(r.x));
     const float cj = (0.5f + (float)col) + (-2.0f + 2.0f * 2.0f * u01(r.y));
-    const float s = dev::lerp(-1.0f, 1.0f, cj / (float)a.width);
-    const float t = dev::lerp(-1.0f, 1.0f, rj / (float)a.height);
-    a.rb.st[p] = make_float2(s, t);
-    const float* m = a.cam;
-    const vec3 dir = normalized(mk(a.wx * s, a.wy * t, -1.0f));
+    const float s = dev::lerp(-1.0f, 1.0f, cj / (float)width);
+    const float t = dev::lerp(-1.0f, 1.0f, rj / (float)height);
+    c.st = make_float2(s, t);
+    const vec3 dir = normalized(mk(wx * s, wy * t, -1.0f));
     const float z = 0.0f;
     const float x = z * m[0] + z * m[4] + z * m[8] + m[12];
     const float y = z * m[1] + z * m[5] + z * m[9] + m[13];
     const float zz = z * m[2] + z * m[6] + z * m[10] + m[14];
     const float w = z * m[3] + z * m[7] + z * m[11] + m[15];
-    const vec3 d = mk(dir.x * m[0] + dir.y * m[4] + dir.z * m[8], dir.x * m[1] + dir.y * m[5] + dir.z * m[9],
-                      dir.x * m[2] + dir.y * m[6] + dir.z * m[10]);
-    a.rb.rays[0][2 * (size_t)p] = make_float4(x / w, y / w, zz / w, 0.0f);
-    a.rb.rays[0][2 * (size_t)p + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    c.o = mk(x / w, y / w, zz / w);
+    c.d = mk(dir.x * m[0] + dir.y * m[4] + dir.z * m[8], dir.x * m[1] + dir.y * m[5] + dir.z * m[9],
+             dir.x * m[2] + dir.y * m[6] + dir.z * m[10]);
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npaths) return;
+    const uint32_t npx = a.width * a.height;
+    const uint32_t i = p % npx;
+    const CameraSample c =
+        camera_sample(a.cam, a.wx, a.wy, a.width, a.height, a.seed, a.path_base + p, i / a.width, i % a.width);
+    a.rb.st[p] = c.st;
+    a.rb.rays[0][2 * (size_t)p] = make_float4(c.o.x, c.o.y, c.o.z, 0.0f);
+    a.rb.rays[0][2 * (size_t)p + 1] = make_float4(c.d.x, c.d.y, c.d.z, 0.0f);
     a.rb.queue[0][p] = p;
     a.rb.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     a.rb.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -516,29 +529,30 @@ __global__ void k_reset(uint32_t* c, uint32_t which) { c[which] = 0u; }
 // the guided estimator converges to the plain estimator's image, including the
 // reference's pdf quirks; the NEE and emission MIS weights use the reported
 // BSDF pdf exactly as without the guide.
+struct ShadeOut {
+    bool alive, shadow;
+    vec3 new_o, new_d, seg_to, c_nee, hp;
+};
+
+// One bounce of path `path` (state slot p: T, L, aux) at hit h of ray (o, d)
+// in scene (sv, sh).  Shared by the per-film wavefront and the batch queue so
+// both compute the same arithmetic.
 template <bool NEE, bool GUIDED>
-__global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uint32_t cur) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = a.rb.counts[cur];
+__device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView& sv, const ShadeView& sh,
+                                           uint64_t seed, uint32_t max_depth, uint64_t path, uint32_t p,
+                                           uint32_t depth, const gr_hit& h, vec3 o, vec3 d, ShadeOut& out) {
     bool alive = false;
     bool shadow = false;
-    uint32_t p = 0;
     vec3 new_o = mk(0.0f, 0.0f, 0.0f), new_d = new_o, seg_to = new_o, c_nee = new_o, hp = new_o;
-    if (i < n) {
-        p = a.rb.queue[cur][i];
-        const uint4 hv = a.rb.hits[i];
-        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
-        const float4 ro = a.rb.rays[cur][2 * (size_t)i], rd = a.rb.rays[cur][2 * (size_t)i + 1];
-        const vec3 o = ld3(ro), d = ld3(rd);
+    {
         vec3 T = ld3(a.rb.T[p]);
         vec3 L = ld3(a.rb.L[p]);
-        const uint64_t path = a.path_base + p;
         if (h.prim != kNone) {
-            const Frame f = hit_frame(a.sv, a.sh, h, o, d);
-            const GpuMaterial& m = a.sh.mats[h.prim];
+            const Frame f = hit_frame(sv, sh, h, o, d);
+            const GpuMaterial& m = sh.mats[h.prim];
             const vec3 wi = -d;
             hp = o + d * h.t;
-            const uint4 r0 = draws(a.seed, path, kStreamCamera, depth, 0);
+            const uint4 r0 = draws(seed, path, kStreamCamera, depth, 0);
             if (!NEE) {
                 const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
                 L = L + mul(T, smp.emission);
@@ -554,7 +568,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
                     if (aux.y != 0.0f) {
                         L = L + mul(T, le);
                     } else {
-                        const float pl = light_area_pdf(a.sh, h.prim) * (h.t * h.t) / fabsf(dot(f.gn, wi));
+                        const float pl = light_area_pdf(sh, h.prim) * (h.t * h.t) / fabsf(dot(f.gn, wi));
                         L = L + mul(T, le) * power_heuristic(aux.x, pl);
                     }
                 }
@@ -566,13 +580,13 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
                 }
                 const float alpha = mix ? a.gd.alpha : 0.0f;
                 if (m.nlobes != 0) {
-                    if (a.sh.num_lights > 0 && dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0) {
-                        const LightPoint lp = sample_light_point(a.sh, draws(a.seed, path, kStreamCamera, depth, 1));
+                    if (sh.num_lights > 0 && dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0) {
+                        const LightPoint lp = sample_light_point(sh, draws(seed, path, kStreamCamera, depth, 1));
                         const vec3 to = lp.p - hp;
                         const float d2 = dot(to, to);
                         const float dist = sqrtf(d2);
                         const vec3 wo = to / dist;
-                        const vec3 le_l = dev::light_world(a.sh.mats[lp.prim], -wo, lp.f);
+                        const vec3 le_l = dev::light_world(sh.mats[lp.prim], -wo, lp.f);
                         const float cos_l = fabsf(dot(lp.f.gn, wo));
                         if (!dev::is_zero(le_l) && cos_l > 0.0f && d2 > 0.0f) {
                             const vec3 fr = dev::f_world(m, wi, wo, f, true);
@@ -597,7 +611,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
                         a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
                         out_dir = smp.outgoing;
                     } else {
-                        const uint4 rg = draws(a.seed, path, kStreamCamera, depth, kSlotGuideSel / 4);
+                        const uint4 rg = draws(seed, path, kStreamCamera, depth, kSlotGuideSel / 4);
                         if (mix && u01(rg.x) < alpha) {
                             out_dir = mixture_sample(mix, u01(rg.y), u01(rg.z), u01(rg.w));
                         } else {
@@ -645,21 +659,44 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
         }
         a.rb.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
         a.rb.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
-        if (a.max_depth != 0 && depth + 1 >= a.max_depth) alive = false;
+        if (max_depth != 0 && depth + 1 >= max_depth) alive = false;
+    }
+    out.alive = alive;
+    out.shadow = shadow;
+    out.new_o = new_o;
+    out.new_d = new_d;
+    out.seg_to = seg_to;
+    out.c_nee = c_nee;
+    out.hp = hp;
+}
+
+template <bool NEE, bool GUIDED>
+__global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.rb.counts[cur];
+    ShadeOut out;
+    out.alive = out.shadow = false;
+    uint32_t p = 0;
+    if (i < n) {
+        p = a.rb.queue[cur][i];
+        const uint4 hv = a.rb.hits[i];
+        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
+        const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
+        shade_path<NEE, GUIDED>(a, a.sv, a.sh, a.seed, a.max_depth, a.path_base + p, p, depth, h, o, d, out);
     }
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t slot = wave_append(alive, &a.rb.counts[nxt]);
-    if (alive) {
+    const uint32_t slot = wave_append(out.alive, &a.rb.counts[nxt]);
+    if (out.alive) {
         a.rb.queue[nxt][slot] = p;
-        a.rb.rays[nxt][2 * (size_t)slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
-        a.rb.rays[nxt][2 * (size_t)slot + 1] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+        a.rb.rays[nxt][2 * (size_t)slot] = make_float4(out.new_o.x, out.new_o.y, out.new_o.z, 0.0f);
+        a.rb.rays[nxt][2 * (size_t)slot + 1] = make_float4(out.new_d.x, out.new_d.y, out.new_d.z, 0.0f);
     }
     if (NEE) {
-        const uint32_t ss = wave_append(shadow, &a.rb.counts[2]);
-        if (shadow) {
-            a.rb.seg[2 * (size_t)ss] = make_float4(hp.x, hp.y, hp.z, 0.0f);
-            a.rb.seg[2 * (size_t)ss + 1] = make_float4(seg_to.x, seg_to.y, seg_to.z, 0.0f);
-            a.rb.contrib[ss] = make_float4(c_nee.x, c_nee.y, c_nee.z, 0.0f);
+        const uint32_t ss = wave_append(out.shadow, &a.rb.counts[2]);
+        if (out.shadow) {
+            a.rb.seg[2 * (size_t)ss] = make_float4(out.hp.x, out.hp.y, out.hp.z, 0.0f);
+            a.rb.seg[2 * (size_t)ss + 1] = make_float4(out.seg_to.x, out.seg_to.y, out.seg_to.z, 0.0f);
+            a.rb.contrib[ss] = make_float4(out.c_nee.x, out.c_nee.y, out.c_nee.z, 0.0f);
             a.rb.seg_path[ss] = p;
         }
     }
@@ -1489,6 +1526,391 @@ void release_render_buffers(RenderBuffers& rb) {
     (void)hipFree(rb.bdpt_blob);
     (void)hipFree(rb.guide_blob);
     rb = RenderBuffers{};
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched rendering: many jobs through one pool of in-flight paths
+// ---------------------------------------------------------------------------
+namespace {
+
+struct JobDev {
+    SceneView sv;
+    ShadeView sh;
+    float cam[16];
+    float wx, wy;
+    uint32_t width, height, max_depth, scene;
+    uint64_t seed, path_base0;  // path id of the job's sample 0: sample_offset * width * height
+    uint64_t first;             // index of the job's first sample among all jobs' samples
+    uint32_t samples, pad;
+    float4* L;                  // per sample: radiance (iteration-major, as gr_trace's batches)
+    float2* st;                 // per sample: film position
+};
+
+struct BatchDev {
+    const JobDev* jobs;
+    uint32_t njobs, pool;
+    unsigned long long total;   // samples of all jobs
+    uint32_t *slot_job, *slot_sample, *slot_depth;
+    float2* slot_st;
+    uint32_t* free_list;
+    uint32_t* retire;
+    // [0] free slots, [1] paths started this step, [2] first free-list entry they take,
+    // [3] their first queue position, [4] paths retired this step
+    uint32_t* ctr;
+    unsigned long long* issued;  // [0] samples started so far, [1] first sample of this step
+    uint32_t* done;              // per job: samples finished
+};
+
+__global__ void k_batch_init(RenderArgs a, BatchDev b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.pool) b.free_list[i] = i;
+    if (i < b.njobs) b.done[i] = 0u;
+    if (i == 0) {
+        b.ctr[0] = b.pool;
+        b.ctr[1] = b.ctr[2] = b.ctr[3] = b.ctr[4] = 0u;
+        b.issued[0] = b.issued[1] = 0ull;
+        a.rb.counts[0] = a.rb.counts[1] = a.rb.counts[2] = 0u;
+    }
+}
+
+// How many new paths this step starts (free slots, samples left) and where
+// they go; clears the step's other queues.
+__global__ void k_batch_plan(RenderArgs a, BatchDev b, uint32_t cur) {
+    if (threadIdx.x != 0) return;
+    const uint32_t freec = b.ctr[0];
+    const unsigned long long issued = b.issued[0];
+    const unsigned long long rem = b.total - issued;
+    const uint32_t k = rem < (unsigned long long)freec ? (uint32_t)rem : freec;
+    const uint32_t q = a.rb.counts[cur];
+    b.ctr[1] = k;
+    b.ctr[2] = freec - k;
+    b.ctr[3] = q;
+    b.ctr[0] = freec - k;
+    b.ctr[4] = 0u;
+    b.issued[1] = issued;
+    b.issued[0] = issued + k;
+    a.rb.counts[cur] = q + k;
+    a.rb.counts[cur ^ 1u] = 0u;
+    a.rb.counts[2] = 0u;
+}
+
+// New camera paths in free slots, appended after the continuing rays.
+__global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.ctr[1]) return;
+    const uint32_t slot = b.free_list[b.ctr[2] + i];
+    const unsigned long long sidx = b.issued[1] + i;
+    uint32_t lo = 0, hi = b.njobs - 1;  // last job whose first sample <= sidx
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (b.jobs[mid].first <= sidx) lo = mid;
+        else hi = mid - 1;
+    }
+    const JobDev& J = b.jobs[lo];
+    const uint32_t local = (uint32_t)(sidx - J.first);
+    const uint32_t px = local % (J.width * J.height);
+    const CameraSample c =
+        camera_sample(J.cam, J.wx, J.wy, J.width, J.height, J.seed, J.path_base0 + local, px / J.width, px % J.width);
+    const uint32_t q = b.ctr[3] + i;
+    a.rb.queue[cur][q] = slot;
+    a.rb.rays[cur][2 * (size_t)q] = make_float4(c.o.x, c.o.y, c.o.z, __uint_as_float(J.scene));
+    a.rb.rays[cur][2 * (size_t)q + 1] = make_float4(c.d.x, c.d.y, c.d.z, 0.0f);
+    a.rb.T[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.rb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a.rb.aux[slot] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+    b.slot_job[slot] = lo;
+    b.slot_sample[slot] = local;
+    b.slot_depth[slot] = 0u;
+    b.slot_st[slot] = c.st;
+}
+
+// One bounce of every queued path, each with its own job's scene, seed and
+// depth; the shading itself is shade_path, as in the per-film wavefront.
+template <bool NEE>
+__global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, uint32_t cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = a.rb.counts[cur];
+    ShadeOut out;
+    out.alive = out.shadow = false;
+    uint32_t slot = 0, scene = 0;
+    bool retire = false;
+    if (i < n) {
+        slot = a.rb.queue[cur][i];
+        const uint4 hv = a.rb.hits[i];
+        const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
+        const float4 ro = a.rb.rays[cur][2 * (size_t)i];
+        const vec3 o = ld3(ro), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
+        scene = __float_as_uint(ro.w);
+        const JobDev& J = b.jobs[b.slot_job[slot]];
+        const uint32_t depth = b.slot_depth[slot];
+        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + b.slot_sample[slot], slot, depth,
+                               h, o, d, out);
+        if (out.alive) b.slot_depth[slot] = depth + 1u;
+        retire = !out.alive;
+    }
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t q = wave_append(out.alive, &a.rb.counts[nxt]);
+    if (out.alive) {
+        a.rb.queue[nxt][q] = slot;
+        a.rb.rays[nxt][2 * (size_t)q] = make_float4(out.new_o.x, out.new_o.y, out.new_o.z, __uint_as_float(scene));
+        a.rb.rays[nxt][2 * (size_t)q + 1] = make_float4(out.new_d.x, out.new_d.y, out.new_d.z, 0.0f);
+    }
+    const uint32_t r = wave_append(retire, &b.ctr[4]);
+    if (retire) b.retire[r] = slot;
+    if (NEE) {
+        const uint32_t ss = wave_append(out.shadow, &a.rb.counts[2]);
+        if (out.shadow) {
+            a.rb.seg[2 * (size_t)ss] = make_float4(out.hp.x, out.hp.y, out.hp.z, __uint_as_float(scene));
+            a.rb.seg[2 * (size_t)ss + 1] = make_float4(out.seg_to.x, out.seg_to.y, out.seg_to.z, 0.0f);
+            a.rb.contrib[ss] = make_float4(out.c_nee.x, out.c_nee.y, out.c_nee.z, 0.0f);
+            a.rb.seg_path[ss] = slot;
+        }
+    }
+}
+
+// Finished paths (their last light sample already added): radiance and film
+// position to the job, slot back to the free list.
+__global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = i < b.ctr[4];
+    uint32_t slot = 0;
+    if (ok) {
+        slot = b.retire[i];
+        const uint32_t j = b.slot_job[slot];
+        const JobDev& J = b.jobs[j];
+        const uint32_t local = b.slot_sample[slot];
+        J.L[local] = a.rb.L[slot];
+        J.st[local] = b.slot_st[slot];
+        atomicAdd(&b.done[j], 1u);
+    }
+    const uint32_t k = wave_append(ok, &b.ctr[0]);
+    if (ok) b.free_list[k] = slot;
+}
+
+}  // namespace
+
+hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_batch_params& prm, double* job_ms) {
+    const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
+    const uint32_t pool = prm.path_pool ? prm.path_pool : (1u << 22);
+    const uint32_t poll = prm.steps_per_poll ? prm.steps_per_poll : 16u;
+    // distinct scenes, job table
+    std::vector<gr_scene*> scenes;
+    std::vector<JobDev> hj(njobs);
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const gr_job& jb = jobs[j];
+        uint32_t si = 0;
+        while (si < scenes.size() && scenes[si] != jb.scene) ++si;
+        if (si == scenes.size()) scenes.push_back(jb.scene);
+        JobDev& d = hj[j];
+        std::memset(&d, 0, sizeof(d));
+        d.sv = jb.scene->view;
+        d.sh = jb.scene->shade;
+        camera_matrix(jb.camera, d.cam);
+        d.wx = jb.camera.horizontal_aperture / (jb.camera.focal_length * 2.0f);  // window_max, camera.rs:76-79
+        d.wy = jb.camera.vertical_aperture / (jb.camera.focal_length * 2.0f);
+        d.width = jb.film->width;
+        d.height = jb.film->height;
+        d.max_depth = jb.max_depth;
+        d.scene = si;
+        d.seed = jb.seed;
+        const uint64_t npx = (uint64_t)d.width * d.height;
+        d.path_base0 = jb.sample_offset * npx;
+        d.first = total;
+        d.samples = (uint32_t)(npx * jb.spp);
+        total += d.samples;
+    }
+    std::vector<SceneRef> hs(scenes.size());
+    for (size_t k = 0; k < scenes.size(); ++k) {
+        const SceneView& v = scenes[k]->view;
+        hs[k].nodes = v.nodes;
+        hs[k].comps = v.comps;
+        hs[k].comp_ids = v.comp_ids;
+        hs[k].root_ref = v.root_ref;
+        for (int c = 0; c < 3; ++c) {
+            hs[k].root_lo[c] = v.root_lo[c];
+            hs[k].root_hi[c] = v.root_hi[c];
+        }
+    }
+    TraceTuning tu = scenes[0]->tuning;
+    if (prm.tuning.lds_stack != 0) {
+        tu.sort_bits = sort_key_bits(prm.tuning.sort_bits);
+        tu.lds_stack = prm.tuning.lds_stack;
+        tu.waves_per_cu = prm.tuning.waves_per_cu;
+        tu.refill = prm.tuning.refill;
+        tu.xcd_ranges = prm.tuning.xcd_ranges;
+    }
+
+    // device memory: job and scene tables, per-sample outputs, slot state, counters
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_jobs = off; off = align(off + njobs * sizeof(JobDev));
+    const size_t o_scenes = off; off = align(off + scenes.size() * sizeof(SceneRef));
+    const size_t o_L = off; off = align(off + total * sizeof(float4));
+    const size_t o_st = off; off = align(off + total * sizeof(float2));
+    const size_t o_sjob = off; off = align(off + (size_t)pool * 4);
+    const size_t o_ssmp = off; off = align(off + (size_t)pool * 4);
+    const size_t o_sdep = off; off = align(off + (size_t)pool * 4);
+    const size_t o_sst = off; off = align(off + (size_t)pool * 8);
+    const size_t o_free = off; off = align(off + (size_t)pool * 4);
+    const size_t o_ret = off; off = align(off + (size_t)pool * 4);
+    const size_t o_ctr = off; off = align(off + 64);
+    const size_t o_iss = off; off = align(off + 16);
+    const size_t o_done = off; off = align(off + njobs * 4);
+    char* blob = nullptr;
+    RenderBuffers rb;
+    SortScratch ss;
+    uint32_t* h_done = nullptr;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev_start = nullptr, ev_poll = nullptr, ev_ctx = nullptr;
+    std::vector<hipEvent_t> ev_job(njobs, nullptr);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&blob), off);
+    if (e == hipSuccess) e = reserve(rb, pool);
+    if (e == hipSuccess && tu.sort_bits) e = reserve_sort(ss, pool);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), njobs * 4);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&ev_poll);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_ctx, hipEventDisableTiming);
+    for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) e = hipEventCreate(&ev_job[j]);
+    std::vector<float4*> job_L(njobs);
+    std::vector<float2*> job_st(njobs);
+    if (e == hipSuccess) {
+        for (uint32_t j = 0; j < njobs; ++j) {
+            hj[j].L = reinterpret_cast<float4*>(blob + o_L) + hj[j].first;
+            hj[j].st = reinterpret_cast<float2*>(blob + o_st) + hj[j].first;
+            job_L[j] = hj[j].L;
+            job_st[j] = hj[j].st;
+        }
+        // start after the work already on the context's stream
+        e = hipEventRecord(ev_ctx, ctx->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, ev_ctx, 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(blob + o_jobs, hj.data(), njobs * sizeof(JobDev), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(blob + o_scenes, hs.data(), hs.size() * sizeof(SceneRef), hipMemcpyHostToDevice, s);
+    }
+    RenderArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.rb = rb;
+    BatchDev b;
+    b.jobs = reinterpret_cast<const JobDev*>(blob + o_jobs);
+    b.njobs = njobs;
+    b.pool = pool;
+    b.total = total;
+    b.slot_job = reinterpret_cast<uint32_t*>(blob + o_sjob);
+    b.slot_sample = reinterpret_cast<uint32_t*>(blob + o_ssmp);
+    b.slot_depth = reinterpret_cast<uint32_t*>(blob + o_sdep);
+    b.slot_st = reinterpret_cast<float2*>(blob + o_sst);
+    b.free_list = reinterpret_cast<uint32_t*>(blob + o_free);
+    b.retire = reinterpret_cast<uint32_t*>(blob + o_ret);
+    b.ctr = reinterpret_cast<uint32_t*>(blob + o_ctr);
+    b.issued = reinterpret_cast<unsigned long long*>(blob + o_iss);
+    b.done = reinterpret_cast<uint32_t*>(blob + o_done);
+
+    TraceSetup ts{scenes[0]->view, tu, &ss, nullptr, reinterpret_cast<const SceneRef*>(blob + o_scenes),
+                  (uint32_t)scenes.size()};
+    auto step = [&](uint32_t cur) -> hipError_t {
+        hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, s, a, b, cur);
+        hipLaunchKernelGGL(k_batch_regen, blocks_for(pool), dim3(256), 0, s, a, b, cur);
+        hipError_t r = launch_intersect(ts, reinterpret_cast<const gr_ray*>(rb.rays[cur]), pool, rb.counts + cur,
+                                        reinterpret_cast<gr_hit*>(rb.hits), nullptr, s);
+        if (r != hipSuccess) return r;
+        if (nee) {
+            hipLaunchKernelGGL((k_batch_shade<true>), blocks_for(pool), dim3(256), 0, s, a, b, cur);
+            r = launch_occluded(ts, reinterpret_cast<const gr_segment*>(rb.seg), pool, rb.counts + 2, rb.vis,
+                                GR_OCCLUSION_EXACT, nullptr, s);
+            if (r != hipSuccess) return r;
+            hipLaunchKernelGGL((k_shadow_accum<false>), blocks_for(pool), dim3(256), 0, s, a, 0u);
+        } else {
+            hipLaunchKernelGGL((k_batch_shade<false>), blocks_for(pool), dim3(256), 0, s, a, b, cur);
+        }
+        hipLaunchKernelGGL(k_batch_retire, blocks_for(pool), dim3(256), 0, s, a, b);
+        return hipGetLastError();
+    };
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(pool, njobs)), dim3(256), 0, s, a, b);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev_start, s);
+    // two eager steps (they also settle the launch shapes), then the graph
+    if (e == hipSuccess) e = step(0);
+    if (e == hipSuccess) e = step(1);
+    if (e == hipSuccess && prm.use_graph) {
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t e1 = step(0);
+            if (e1 == hipSuccess) e1 = step(1);
+            e = hipStreamEndCapture(s, &graph);
+            if (e == hipSuccess) e = e1;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    std::vector<char> gathered(njobs, 0);
+    uint32_t pending = njobs;
+    uint64_t pairs = 0;
+    // Every path ends: past 20 bounces Russian roulette keeps at most 3/4 of
+    // them per bounce.  A workload needs about (samples / pool) * depth steps;
+    // a count far beyond that means the counters are wrong -- stop with an error.
+    const uint64_t max_pairs = (total / pool + 2) * 4096ull;
+    while (e == hipSuccess && pending > 0) {
+        for (uint32_t k = 0; k < (poll + 1) / 2 && e == hipSuccess; ++k) {
+            if (exec) {
+                e = hipGraphLaunch(exec, s);
+            } else {
+                e = step(0);
+                if (e == hipSuccess) e = step(1);
+            }
+            ++pairs;
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(h_done, b.done, njobs * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(ev_poll, s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev_poll);
+        for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) {
+            if (gathered[j] || h_done[j] < hj[j].samples) continue;
+            RenderArgs g = a;
+            g.width = hj[j].width;
+            g.height = hj[j].height;
+            g.rb.L = job_L[j];
+            g.rb.st = job_st[j];
+            const dim3 grid((g.width + kTile - 1) / kTile, (g.height + kTile - 1) / kTile);
+            hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, s, g, jobs[j].film->d_pixels,
+                               jobs[j].spp);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipEventRecord(ev_job[j], s);
+            gathered[j] = 1;
+            --pending;
+        }
+        if (e == hipSuccess && pairs > max_pairs) e = hipErrorLaunchTimeOut;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (uint32_t j = 0; j < njobs && e == hipSuccess && job_ms; ++j) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, ev_start, ev_job[j]);
+        job_ms[j] = ms;
+    }
+    // the context's stream continues after the batch
+    if (ev_ctx && s) {
+        (void)hipEventRecord(ev_ctx, s);
+        (void)hipStreamWaitEvent(ctx->stream, ev_ctx, 0);
+    }
+    if (s) (void)hipStreamSynchronize(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t ev : ev_job)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_poll) (void)hipEventDestroy(ev_poll);
+    if (ev_ctx) (void)hipEventDestroy(ev_ctx);
+    if (s) (void)hipStreamDestroy(s);
+    if (h_done) (void)hipHostFree(h_done);
+    release_sort(ss);
+    release_render_buffers(rb);
+    (void)hipFree(blob);
+    return e;
 }
 
 }  // namespace gr

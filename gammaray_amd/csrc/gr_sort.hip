@@ -67,6 +67,38 @@ __global__ __launch_bounds__(256) void k_sort_keys(const float4* __restrict__ in
     vals[i] = i;
 }
 
+// Multi-scene keys: scene index (sbits) above octant and Morton code, the
+// origin quantised to its own scene's box.
+__global__ __launch_bounds__(256) void k_sort_keys_multi(const float4* __restrict__ in, uint32_t n_max,
+                                                         const uint32_t* __restrict__ n_ptr, uint32_t segments,
+                                                         const SceneRef* __restrict__ scenes, uint32_t nscenes,
+                                                         uint32_t m, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals) {
+    const uint32_t n = live_count(n_max, n_ptr);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = in[2 * (size_t)i], b = in[2 * (size_t)i + 1];
+    float dx = b.x, dy = b.y, dz = b.z;
+    if (segments) {
+        dx = b.x - a.x;
+        dy = b.y - a.y;
+        dz = b.z - a.z;
+    }
+    uint32_t si = __float_as_uint(a.w);
+    si = si < nscenes ? si : 0u;
+    const SceneRef& sr = scenes[si];
+    const uint32_t oct = (dx < 0.0f ? 4u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 1u : 0u);
+    const uint32_t top = (1u << m) - 1u;
+    uint32_t q[3];
+    const float o[3] = {a.x, a.y, a.z};
+    for (int k = 0; k < 3; ++k) {
+        const float ext = sr.root_hi[k] - sr.root_lo[k];
+        q[k] = quantise(o[k], sr.root_lo[k], ext > 0.0f ? (float)(1u << m) / ext : 0.0f, top);
+    }
+    keys[i] = (si << (3 + 3 * m)) | (oct << (3 * m)) | (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    vals[i] = i;
+}
+
 // Per-block digit histogram, stored digit-major: hist[d * nblocks + block].
 __global__ __launch_bounds__(kThreads) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n_max,
                                                         const uint32_t* __restrict__ n_ptr, uint32_t shift,
@@ -231,6 +263,44 @@ void release_sort(SortScratch& ss) {
     ss = SortScratch{};
 }
 
+namespace {
+// Radix passes over the first `bits` key bits of ss.keys[0] / ss.vals[0].
+hipError_t radix_passes(uint32_t n_max, const uint32_t* n_ptr, uint32_t bits, SortScratch& ss, const uint32_t** perm,
+                        hipStream_t s) {
+    const uint32_t nblocks = ceil_div(n_max, kTile);
+    int src = 0;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        const uint32_t dbits = bits - shift < 8 ? bits - shift : 8;
+        const uint32_t mask = (1u << dbits) - 1u;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kThreads), 0, s, ss.keys[src], n_max, n_ptr, shift,
+                           mask, ss.hist, nblocks);
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kThreads), 0, s, ss.hist, nblocks, ss.totals);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kThreads), 0, s, ss.keys[src], ss.vals[src],
+                           ss.keys[src ^ 1], ss.vals[src ^ 1], n_max, n_ptr, shift, mask, dbits, ss.hist,
+                           ss.totals, nblocks);
+        src ^= 1;
+    }
+    *perm = ss.vals[src];
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t sort_rays_multi(const SceneRef* scenes, uint32_t nscenes, const float4* in, uint32_t n_max,
+                           const uint32_t* n_ptr, bool segments, uint32_t bits, SortScratch& ss,
+                           const uint32_t** perm, hipStream_t s) {
+    bits = sort_key_bits(bits);
+    *perm = nullptr;
+    if (bits == 0 || n_max == 0) return hipSuccess;
+    if (ss.cap < n_max) return hipErrorInvalidValue;
+    uint32_t sbits = 0;
+    while ((1u << sbits) < nscenes) ++sbits;
+    uint32_t m = (bits - 3) / 3;
+    while (m > 0 && sbits + 3 + 3 * m > 32) --m;
+    hipLaunchKernelGGL(k_sort_keys_multi, dim3(ceil_div(n_max, 256)), dim3(256), 0, s, in, n_max, n_ptr,
+                       segments ? 1u : 0u, scenes, nscenes, m, ss.keys[0], ss.vals[0]);
+    return radix_passes(n_max, n_ptr, sbits + 3 + 3 * m, ss, perm, s);
+}
+
 hipError_t sort_rays(const SceneView& sv, const float4* in, uint32_t n_max, const uint32_t* n_ptr, bool segments,
                      uint32_t bits, SortScratch& ss, const uint32_t** perm, hipStream_t s) {
     bits = sort_key_bits(bits);
@@ -249,21 +319,7 @@ hipError_t sort_rays(const SceneView& sv, const float4* in, uint32_t n_max, cons
     lo.w = scale.w = 0.0f;
     hipLaunchKernelGGL(k_sort_keys, dim3(ceil_div(n_max, 256)), dim3(256), 0, s, in, n_max, n_ptr,
                        segments ? 1u : 0u, lo, scale, m, ss.keys[0], ss.vals[0]);
-    const uint32_t nblocks = ceil_div(n_max, kTile);
-    int src = 0;
-    for (uint32_t shift = 0; shift < bits; shift += 8) {
-        const uint32_t dbits = bits - shift < 8 ? bits - shift : 8;
-        const uint32_t mask = (1u << dbits) - 1u;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kThreads), 0, s, ss.keys[src], n_max, n_ptr, shift,
-                           mask, ss.hist, nblocks);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kThreads), 0, s, ss.hist, nblocks, ss.totals);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kThreads), 0, s, ss.keys[src], ss.vals[src],
-                           ss.keys[src ^ 1], ss.vals[src ^ 1], n_max, n_ptr, shift, mask, dbits, ss.hist,
-                           ss.totals, nblocks);
-        src ^= 1;
-    }
-    *perm = ss.vals[src];
-    return hipGetLastError();
+    return radix_passes(n_max, n_ptr, bits, ss, perm, s);
 }
 
 }  // namespace gr

@@ -152,8 +152,9 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <int MODE, bool COUNT, int LDS>
-__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __restrict__ in, uint32_t n,
+template <int MODE, bool COUNT, int LDS, bool MULTI>
+__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const SceneRef* __restrict__ scenes,
+                                                  uint32_t nscenes, const float4* __restrict__ in, uint32_t n,
                                                   const uint32_t* __restrict__ perm,
                                                   typename Output<MODE>::type* __restrict__ out,
                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ next_ray,
@@ -185,6 +186,10 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
     RayFrame r;
     r.o = r.d = r.inv = mk(0.0f, 0.0f, 0.0f);
     r.nx = r.ny = r.nz = false;
+    // the lane's scene (the launch's scene unless MULTI)
+    const GpuNode* nodes = sv.nodes;
+    const GpuComp* comps = sv.comps;
+    const uint2* comp_ids = sv.comp_ids;
 
     auto finish = [&](bool occluded, bool overflow) {
         if (MODE == kClosest) {
@@ -192,7 +197,7 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
             if (overflow) {
                 h.z = __float_as_uint(-1.0f);
             } else if (best != kNone) {
-                const uint2 id = sv.comp_ids[best];
+                const uint2 id = comp_ids[best];
                 h = make_uint4(id.x, id.y, __float_as_uint(closest), best);
             }
             reinterpret_cast<uint4*>(out)[ray] = h;
@@ -282,10 +287,21 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
                         if (start) {
                             float tmin;
                             if (COUNT) c_box = 1;
-                            if (slab(sv.root_lo[0], sv.root_lo[1], sv.root_lo[2], sv.root_hi[0],
-                                     sv.root_hi[1], sv.root_hi[2], r, sv.pad, &tmin) &&
-                                tmin < closest) {
-                                cur = sv.root_ref;
+                            uint32_t root = sv.root_ref;
+                            float lo0 = sv.root_lo[0], lo1 = sv.root_lo[1], lo2 = sv.root_lo[2];
+                            float hi0 = sv.root_hi[0], hi1 = sv.root_hi[1], hi2 = sv.root_hi[2];
+                            if (MULTI) {
+                                const uint32_t si = __float_as_uint(a.w);
+                                const SceneRef& sr = scenes[si < nscenes ? si : 0u];
+                                nodes = sr.nodes;
+                                comps = sr.comps;
+                                comp_ids = sr.comp_ids;
+                                root = sr.root_ref;
+                                lo0 = sr.root_lo[0], lo1 = sr.root_lo[1], lo2 = sr.root_lo[2];
+                                hi0 = sr.root_hi[0], hi1 = sr.root_hi[1], hi2 = sr.root_hi[2];
+                            }
+                            if (slab(lo0, lo1, lo2, hi0, hi1, hi2, r, sv.pad, &tmin) && tmin < closest) {
+                                cur = root;
                                 active = true;
                             } else {
                                 finish(false, false);
@@ -302,7 +318,7 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
 
         // ---- interior nodes until every lane sits on a leaf or is done
         while (active && !(cur & kLeafBit)) {
-            const GpuNode nd = load_node(sv.nodes, cur);
+            const GpuNode nd = load_node(nodes, cur);
             if (COUNT) c_box += 2;
             float t0 = 0.0f, t1 = 0.0f;
             const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
@@ -341,7 +357,7 @@ __global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const float4* __r
             uint32_t slot = cur & ~kLeafBit;
             bool done = false;
             for (;;) {
-                const GpuComp g = load_comp(sv.comps, slot);
+                const GpuComp g = load_comp(comps, slot);
                 const uint32_t tag = __float_as_uint(g.q2.y);
                 float dist;
                 if (tag & kTagSphere) {
@@ -372,12 +388,12 @@ __global__ void k_reset_counter(uint32_t* c) {
     for (uint32_t i = 0; i < kRayCounters; ++i) c[i] = 0u;
 }
 
-template <int MODE, bool COUNT, int LDS>
+template <int MODE, bool COUNT, int LDS, bool MULTI>
 int occupancy() {
     static int per_cu = -1;
     if (per_cu < 0) {
         int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_trace<MODE, COUNT, LDS>, kWave, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_trace<MODE, COUNT, LDS, MULTI>, kWave, 0) != hipSuccess)
             v = 1;
         per_cu = v > 0 ? v : 1;
     }
@@ -397,10 +413,10 @@ int num_cus() {
     return cus;
 }
 
-template <int MODE, bool COUNT, int LDS>
+template <int MODE, bool COUNT, int LDS, bool MULTI>
 hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, const uint32_t* perm,
                       void* out, uint32_t* counts, hipStream_t s) {
-    int per_cu = occupancy<MODE, COUNT, LDS>();
+    int per_cu = occupancy<MODE, COUNT, LDS, MULTI>();
     if (ts.tu.waves_per_cu > 0 && (int)ts.tu.waves_per_cu < per_cu) per_cu = (int)ts.tu.waves_per_cu;
     const uint32_t grid = (uint32_t)(num_cus() * per_cu);
     const uint32_t needed = (n + kWave - 1) / kWave;
@@ -409,7 +425,8 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     const uint32_t parts = ts.tu.xcd_ranges ? kRayCounters : 1u;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);
     hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
-    hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS>), dim3(blocks), dim3(kWave), 0, s, ts.sv,
+    hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS, MULTI>), dim3(blocks), dim3(kWave), 0, s, ts.sv, ts.scenes,
+                       ts.nscenes,
                        reinterpret_cast<const float4*>(in), n, perm,
                        reinterpret_cast<typename Output<MODE>::type*>(out), counts, ts.sv.next_ray, n_ptr, refill,
                        parts);
@@ -417,32 +434,42 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     return hipGetLastError();
 }
 
-template <int MODE, bool COUNT>
-hipError_t launch_trace(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, void* out,
-                        uint32_t* counts, hipStream_t s) {
+template <int MODE, bool COUNT, bool MULTI>
+hipError_t launch_sorted(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, void* out,
+                         uint32_t* counts, hipStream_t s) {
     const uint32_t* perm = nullptr;
     if (ts.tu.sort_bits != 0) {
         if (!ts.sort) return hipErrorInvalidValue;
         hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
-        const hipError_t e = sort_rays(ts.sv, reinterpret_cast<const float4*>(in), n, n_ptr, MODE != kClosest,
-                                       ts.tu.sort_bits, *ts.sort, &perm, s);
+        const hipError_t e =
+            MULTI ? sort_rays_multi(ts.scenes, ts.nscenes, reinterpret_cast<const float4*>(in), n, n_ptr,
+                                    MODE != kClosest, ts.tu.sort_bits, *ts.sort, &perm, s)
+                  : sort_rays(ts.sv, reinterpret_cast<const float4*>(in), n, n_ptr, MODE != kClosest,
+                              ts.tu.sort_bits, *ts.sort, &perm, s);
         if (t0) timing_end(ts.timing, GR_KERNEL_RAY_SORT, t0, s);
         if (e != hipSuccess) return e;
     }
     switch (ts.tu.lds_stack) {
-        case 8: return launch_lds<MODE, COUNT, 8>(ts, in, n, n_ptr, perm, out, counts, s);
-        case 32: return launch_lds<MODE, COUNT, 32>(ts, in, n, n_ptr, perm, out, counts, s);
-        default: return launch_lds<MODE, COUNT, 16>(ts, in, n, n_ptr, perm, out, counts, s);
+        case 8: return launch_lds<MODE, COUNT, 8, MULTI>(ts, in, n, n_ptr, perm, out, counts, s);
+        case 32: return launch_lds<MODE, COUNT, 32, MULTI>(ts, in, n, n_ptr, perm, out, counts, s);
+        default: return launch_lds<MODE, COUNT, 16, MULTI>(ts, in, n, n_ptr, perm, out, counts, s);
     }
+}
+
+template <int MODE, bool COUNT>
+hipError_t launch_trace(const TraceSetup& ts, const void* in, uint32_t n, const uint32_t* n_ptr, void* out,
+                        uint32_t* counts, hipStream_t s) {
+    return ts.scenes ? launch_sorted<MODE, COUNT, true>(ts, in, n, n_ptr, out, counts, s)
+                     : launch_sorted<MODE, COUNT, false>(ts, in, n, n_ptr, out, counts, s);
 }
 
 }  // namespace
 
 int trace_occupancy(uint32_t lds_stack) {
     switch (lds_stack) {
-        case 8: return occupancy<kClosest, false, 8>();
-        case 32: return occupancy<kClosest, false, 32>();
-        default: return occupancy<kClosest, false, 16>();
+        case 8: return occupancy<kClosest, false, 8, false>();
+        case 32: return occupancy<kClosest, false, 32, false>();
+        default: return occupancy<kClosest, false, 16, false>();
     }
 }
 

@@ -202,6 +202,37 @@ int gr_trace(gr_scene *scene, const gr_camera *camera, const gr_trace_params *pa
  * float channels, rows flipped, value = accum / weight. */
 int gr_film_write_exr(gr_film *film, const char *path);
 
+/* ---- batched rendering (no reference counterpart: the reference renders one
+ * job, src/bin/gammaray.rs:85-112) -------------------------------------------
+ * Many jobs -- each a scene, camera, film, spp and depth cap -- run through one
+ * shared pool of in-flight paths.  One bounce step serves every job at once:
+ * free path slots take the next samples from a global counter over all jobs,
+ * one closest-hit launch traverses every job's scene (each ray names its
+ * scene), shading and shadow rays follow, finished paths hand their radiance
+ * to their job and free their slot.  The step is captured once as a hipGraph
+ * and replayed.  A job's film gets the Mitchell splat of its samples, in the
+ * reference's order, as soon as its last sample finishes; it equals what
+ * gr_trace renders for the same camera, seed, spp, depth and sample_offset. */
+typedef struct {
+    gr_scene *scene;
+    gr_film *film;             /* the resolution is the film's */
+    gr_camera camera;
+    uint32_t spp, max_depth;   /* max_depth 0 = unbounded, as gr_trace */
+    uint64_t seed, sample_offset;
+} gr_job;
+typedef struct {
+    uint32_t integrator;       /* GR_INTEGRATOR_PT or GR_INTEGRATOR_PT_NEE, for every job */
+    uint32_t path_pool;        /* paths in flight; 0 = 4,194,304 */
+    uint32_t steps_per_poll;   /* bounce steps between completion checks; 0 = 16 */
+    uint32_t use_graph;        /* 1: replay the captured bounce step as a hipGraph; 0: launch eagerly */
+    gr_tuning tuning;          /* the shared traversal launches; lds_stack 0 = the first job's scene's setting */
+} gr_batch_params;
+/* All scenes and films must belong to ctx.  job_ms (may be NULL, njobs entries)
+ * receives each job's latency: ms from the start of the batch until its film
+ * is final. */
+int gr_render_jobs(gr_ctx *ctx, const gr_job *jobs, uint32_t njobs, const gr_batch_params *params,
+                   double *job_ms);
+
 /* ---- learned path guide (no reference counterpart) -----------------------
  * A 5-layer MLP (fp16 weights, fp32 accumulation on the matrix cores) maps a
  * surface vertex (position, outgoing direction, shading normal, roughness) to

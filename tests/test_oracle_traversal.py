@@ -313,3 +313,12 @@ def test_tuning_validation_and_rounding(host_ctx):
     with pytest.raises(GrError):
         import torch
         st.autotune(torch.zeros((4, 8)))
+
+
+def test_batch_rejects_host_only_context(host_ctx):
+    from gammaray_amd.api import Stage, render_jobs
+    from gammaray_amd._lib import GrError
+    st = Stage(scenes.box_scene(), ctx=host_ctx)
+    no_film = type("NoFilm", (), {"handle": None})()
+    with pytest.raises(GrError):
+        render_jobs(host_ctx, [dict(stage=st, film=no_film, spp=1)])
