@@ -115,6 +115,117 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
     return out
 
 
+def bench_batch(ctx, threads, pool=1 << 22, steps=16):
+    """64 jobs (side log-uniform 128..1024, spp in {4,16,64,256}) over 8
+    displaced-mesh scenes (10k..2M triangles), PT+NEE depth 8, through one
+    path pool with the bounce step as a hipGraph; aggregate Mpaths/s and job
+    latency, next to the same jobs rendered one after another with gr_trace."""
+    import torch
+    from gammaray_amd import scenes
+    from gammaray_amd.api import Film, Stage, render_jobs, trace
+    t0 = time.time()
+    specs, jobs = scenes.batch_workload(seed=4)
+    stages = [Stage(sp, ctx=ctx) for sp in specs]
+    log("batch: 8 scenes %s triangles built (%.1f s)" % ([sp.num_components for sp in specs], time.time() - t0))
+    rays = torch.from_numpy(scenes.ray_batch(specs[-1], 1 << 21, seed=9)).cuda()
+    tuning = stages[-1].autotune(rays)
+    del rays
+    for st in stages:
+        st.set_tuning(**tuning)
+    films = [Film(ctx, j["side"], j["side"]) for j in jobs]
+    jl = [dict(stage=stages[j["scene"]], film=films[k], spp=j["spp"], max_depth=8, seed=j["seed"])
+          for k, j in enumerate(jobs)]
+    total = sum(j["side"] ** 2 * j["spp"] for j in jobs)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lat = render_jobs(ctx, jl, "pt_nee", path_pool=pool, steps_per_poll=steps, use_graph=True, tuning=tuning)
+    wall = time.perf_counter() - t
+    log("batch: %d samples in %.3f s -> %.1f Mpaths/s" % (total, wall, total / wall / 1e6))
+    for f in films:
+        f.clear()
+    ctx.synchronize()
+    seq_lat = []
+    t = time.perf_counter()
+    for job in jl:
+        trace(job["stage"], job["film"], "pt_nee", spp=job["spp"], max_depth=8, seed=job["seed"])
+        ctx.synchronize()
+        seq_lat.append((time.perf_counter() - t) * 1e3)
+    seq = time.perf_counter() - t
+    log("batch: sequential gr_trace %.3f s -> %.1f Mpaths/s" % (seq, total / seq / 1e6))
+    for f in films:
+        f.close()
+    return {"jobs": len(jobs), "scenes": len(specs), "triangles": [sp.num_components for sp in specs],
+            "samples": total, "mpaths_s": round(total / wall / 1e6, 2), "wall_s": round(wall, 4),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "path_pool": pool, "graph": True, "tuning": tuning, "max_depth": 8, "integrator": "pt_nee",
+            "sequential": {"mpaths_s": round(total / seq / 1e6, 2), "wall_s": round(seq, 4),
+                           "latency_ms_p50": round(float(np.percentile(seq_lat, 50)), 2),
+                           "latency_ms_p99": round(float(np.percentile(seq_lat, 99)), 2)}}
+
+
+def _rel_mse_trimmed(img, ref, trim=1e-3):
+    """Mean of (x - ref)^2 / (ref^2 + 0.01) over the pixel channels, dropping the
+    largest `trim` fraction (the reference's own fireflies)."""
+    d = ((img - ref) ** 2 / (ref ** 2 + 1e-2)).ravel()
+    d = d[np.isfinite(d)]
+    return float(np.sort(d)[: int(len(d) * (1 - trim))].mean())
+
+
+def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
+    """Learned path guide on the 1080p showcase scene: wall time for guided
+    rendering (online training included) to reach the relative MSE that plain
+    PT+NEE reaches with 64 spp, against a ref_spp PT+NEE render."""
+    from gammaray_amd import scenes
+    from gammaray_amd.api import Film, Guide, Stage, trace
+    spec = scenes.showcase_scene()
+    st = Stage(spec, ctx=ctx)
+
+    def image(f):
+        px = f.readback()
+        return px[..., :3] / px[..., 3:4]
+
+    def timed(fn):
+        ctx.synchronize()
+        t = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        return time.perf_counter() - t
+
+    ref = Film(ctx, W, H)
+    t_ref = timed(lambda: trace(st, ref, "pt_nee", spp=ref_spp, max_depth=8, seed=1000))
+    R = image(ref)
+    ref.close()
+    plain = Film(ctx, W, H)
+    trace(st, plain, "pt_nee", spp=1, max_depth=8, seed=5)
+    plain.clear()
+    t_plain = timed(lambda: trace(st, plain, "pt_nee", spp=spp, max_depth=8, seed=7))
+    e_plain = _rel_mse_trimmed(image(plain), R)
+    plain.close()
+    cfg = dict(alpha=0.3, guided_depth=2, steps_per_pass=2, batch=16384, train_paths=1 << 16, warmup_steps=8,
+               learning_rate=5e-3)
+    g = Guide(ctx, **cfg)
+    f = Film(ctx, W, H)
+    tot, curve, reached = 0.0, [], None
+    for k in range(0, 2 * spp, 4):
+        tot += timed(lambda: trace(st, f, "guided", spp=4, max_depth=8, seed=9, sample_offset=k, batch=2, guide=g))
+        e = _rel_mse_trimmed(image(f), R)
+        curve.append((k + 4, round(tot, 4), round(e, 6)))
+        if reached is None and e <= e_plain:
+            t0, e0 = (curve[-2][1], curve[-2][2]) if len(curve) > 1 else (0.0, float("inf"))
+            reached = t0 + (tot - t0) * ((e0 - e_plain) / (e0 - e) if np.isfinite(e0) and e0 > e else 1.0)
+    log("guided: plain %d spp %.3f s relMSE %.5f; guided reaches it at %s s" % (
+        spp, t_plain, e_plain, "%.3f" % reached if reached else "-- (not within %d spp)" % (2 * spp)))
+    f.close()
+    g.close()
+    return {"scene": "showcase 1920x1080, PT+NEE depth 8", "reference_spp": ref_spp,
+            "reference_s": round(t_ref, 3), "error": "relMSE, 0.1% largest pixel errors dropped",
+            "plain_spp": spp, "plain_s": round(t_plain, 4), "plain_error": round(e_plain, 6),
+            "guided_config": cfg, "guided_curve": curve,
+            "time_to_equal_error_s": round(reached, 4) if reached else None,
+            "time_ratio": round(reached / t_plain, 3) if reached else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +237,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
+    ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
+    ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the guided comparison")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (autotune on the closest-hit batch before timing), 'default', or "
                          "k=v pairs, e.g. sort_bits=18,lds_stack=16,waves_per_cu=0,refill=16")
@@ -265,6 +379,8 @@ def main():
     paths = None
     if not args.no_paths:
         paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads)
+    batch = None if args.no_batch else bench_batch(ctx, threads)
+    guided = None if args.no_guided else bench_guided(ctx, ref_spp=args.ref_spp)
 
     out = {
         "metric": "Grays/s (closest-hit + shadow rays, incoherent, 1 GPU)",
@@ -286,6 +402,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "paths": paths,
+        "batch": batch,
+        "guided": guided,
     }
     print(json.dumps(out), flush=True)
 

@@ -1559,13 +1559,14 @@ struct BatchDev {
     // [3] their first queue position, [4] paths retired this step
     uint32_t* ctr;
     unsigned long long* issued;  // [0] samples started so far, [1] first sample of this step
-    uint32_t* done;              // per job: samples finished
+    uint32_t* done;              // per job: kDoneSpread counters of finished samples (summed by the host)
 };
+constexpr uint32_t kDoneSpread = 64;  // spreads the counter updates of one job over 64 addresses
 
 __global__ void k_batch_init(RenderArgs a, BatchDev b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < b.pool) b.free_list[i] = i;
-    if (i < b.njobs) b.done[i] = 0u;
+    if (i < b.njobs * kDoneSpread) b.done[i] = 0u;
     if (i == 0) {
         b.ctr[0] = b.pool;
         b.ctr[1] = b.ctr[2] = b.ctr[3] = b.ctr[4] = 0u;
@@ -1674,15 +1675,25 @@ __global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, u
 __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < b.ctr[4];
-    uint32_t slot = 0;
+    uint32_t slot = 0, j = kNone;
     if (ok) {
         slot = b.retire[i];
-        const uint32_t j = b.slot_job[slot];
+        j = b.slot_job[slot];
         const JobDev& J = b.jobs[j];
         const uint32_t local = b.slot_sample[slot];
         J.L[local] = a.rb.L[slot];
         J.st[local] = b.slot_st[slot];
-        atomicAdd(&b.done[j], 1u);
+    }
+    // one counter update per distinct job in the wave (finished paths come in
+    // job order, so a wave usually holds one or two jobs)
+    uint64_t left = __ballot(ok);
+    while (left) {
+        const uint32_t lead = (uint32_t)__builtin_ctzll(left);
+        const uint32_t lj = __shfl(j, (int)lead);
+        const uint64_t same = __ballot(ok && j == lj);
+        if ((threadIdx.x & 63u) == lead)
+            atomicAdd(&b.done[lj * kDoneSpread + blockIdx.x % kDoneSpread], (uint32_t)__popcll(same));
+        left &= ~same;
     }
     const uint32_t k = wave_append(ok, &b.ctr[0]);
     if (ok) b.free_list[k] = slot;
@@ -1757,7 +1768,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     const size_t o_ret = off; off = align(off + (size_t)pool * 4);
     const size_t o_ctr = off; off = align(off + 64);
     const size_t o_iss = off; off = align(off + 16);
-    const size_t o_done = off; off = align(off + njobs * 4);
+    const size_t o_done = off; off = align(off + (size_t)njobs * kDoneSpread * 4);
     char* blob = nullptr;
     RenderBuffers rb;
     SortScratch ss;
@@ -1770,7 +1781,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&blob), off);
     if (e == hipSuccess) e = reserve(rb, pool);
     if (e == hipSuccess && tu.sort_bits) e = reserve_sort(ss, pool);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), njobs * 4);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), (size_t)njobs * kDoneSpread * 4);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ev_poll);
@@ -1832,7 +1843,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         return hipGetLastError();
     };
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(pool, njobs)), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(pool, njobs * kDoneSpread)), dim3(256), 0, s, a, b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev_start, s);
@@ -1866,11 +1877,15 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             }
             ++pairs;
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(h_done, b.done, njobs * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_done, b.done, (size_t)njobs * kDoneSpread * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipEventRecord(ev_poll, s);
         if (e == hipSuccess) e = hipEventSynchronize(ev_poll);
         for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) {
-            if (gathered[j] || h_done[j] < hj[j].samples) continue;
+            if (gathered[j]) continue;
+            uint64_t fin = 0;
+            for (uint32_t k = 0; k < kDoneSpread; ++k) fin += h_done[(size_t)j * kDoneSpread + k];
+            if (fin < hj[j].samples) continue;
             RenderArgs g = a;
             g.width = hj[j].width;
             g.height = hj[j].height;

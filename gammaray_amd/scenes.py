@@ -418,3 +418,40 @@ def showcase_scene(seed=3, n_spheres=200, grid=(250, 200)):
     s.camera = CameraSpec(focal_length=1.0, horizontal_aperture=1.6, vertical_aperture=0.9,
                           translate=(0.0, 1.6, 4.0))
     return s
+
+
+# --------------------------------------------------------------------------
+# Batched-rendering workload: 8 scenes, 64 jobs
+# --------------------------------------------------------------------------
+def terrain_scene(triangles, seed):
+    """A displaced-mesh terrain of about ``triangles`` triangles (a square
+    grid, 2 triangles per cell) with three spheres and a quad light above,
+    Disney materials drawn per object."""
+    rng = np.random.default_rng(seed)
+    n = max(2, int(round(math.sqrt(triangles / 2.0))))
+    s = SceneSpec(name="terrain%d" % (2 * n * n))
+    gv, gf = displaced_grid(n, n, (4.0, 4.0), seed, amplitude=0.08, center=(0.0, 0.0, -2.0))
+    s.add_mesh(gv, gf, random_disney(rng))
+    for _ in range(3):
+        r = rng.uniform(0.15, 0.4)
+        s.add_sphere((rng.uniform(-1.2, 1.2), 0.5 + r, rng.uniform(-3.0, -1.0)), r, random_disney(rng))
+    lv, lf = quad((0.0, 3.0, -2.0), (0.6, 0.0, 0.0), (0.0, 0.0, 0.6), (0.0, -1.0, 0.0))
+    s.add_mesh(lv, lf, diffuse_light((18.0, 17.0, 16.0)))
+    s.camera = CameraSpec(focal_length=1.0, horizontal_aperture=1.0, vertical_aperture=1.0,
+                          rotate=(0.9659258, -0.258819, 0.0, 0.0), translate=(0.0, 1.6, 0.8))
+    return s
+
+
+def batch_workload(seed=4, n_jobs=64, n_scenes=8):
+    """Triangle counts 10k * 200^(i/7) (10k .. 2M), and per job a side length
+    log-uniform in [128, 1024], spp from {4, 16, 64, 256} and a scene, all
+    uniform.  Returns (scene specs, job dicts with scene index, side, spp, seed)."""
+    rng = np.random.default_rng(seed)
+    specs = [terrain_scene(int(10000 * 200.0 ** (i / (n_scenes - 1))), seed * 100 + i) for i in range(n_scenes)]
+    jobs = []
+    for j in range(n_jobs):
+        side = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
+        jobs.append(dict(scene=int(rng.integers(0, n_scenes)), side=side,
+                         spp=int(rng.choice([4, 16, 64, 256])), seed=1000 + j))
+    return specs, jobs
+
