@@ -40,7 +40,9 @@ def algorithmic_bytes(n, counts, out_bytes):
     return n * (RAY_BYTES + out_bytes) + NODE_BYTES * c[0] + TRI_BYTES * c[1] + SPHERE_BYTES * c[2]
 
 
-def cpu_baseline(spec, rays8, segs8, sample, threads):
+def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=3):
+    """The oracle's Bvh::intersect / Bvh::visibility restatement on `sample`
+    rays and segments of the same batches, median of `repeats` passes."""
     from gammaray_amd import scenes
     from oracle.oracle import OracleScene
     t0 = time.time()
@@ -49,15 +51,21 @@ def cpu_baseline(spec, rays8, segs8, sample, threads):
     r6 = scenes.rays6(rays8[:sample])
     s6 = scenes.rays6(segs8[:sample])
     o.intersect(r6[:4096], threads=threads)  # warm
-    t0 = time.perf_counter()
-    o.intersect(r6, threads=threads)
-    t1 = time.perf_counter()
-    o.visibility(s6, threads=threads)
-    t2 = time.perf_counter()
-    return dict(value=2 * sample / (t2 - t0) / 1e9, unit="Grays/s", cores=threads, kind="port",
-                sample="%d closest-hit rays + %d shadow segments of the same batches "
-                       "(oracle Bvh::intersect / Bvh::visibility restatement, %d threads; "
-                       "closest %.2f s, shadow %.2f s)" % (sample, sample, threads, t1 - t0, t2 - t1))
+    tc, ts = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o.intersect(r6, threads=threads)
+        t1 = time.perf_counter()
+        o.visibility(s6, threads=threads)
+        t2 = time.perf_counter()
+        tc.append(t1 - t0)
+        ts.append(t2 - t1)
+    c, sh = float(np.median(tc)), float(np.median(ts))
+    return dict(value=2 * sample / (c + sh) / 1e9, unit="Grays/s", cores=threads, kind="port",
+                sample="%d closest-hit rays + %d shadow segments of the timed batches "
+                       "(oracle Bvh::intersect / Bvh::visibility restatement, %d threads; median of %d passes: "
+                       "closest %.2f s, shadow %.2f s; %.1f s of CPU work in all)" % (
+                           sample, sample, threads, repeats, c, sh, sum(tc) + sum(ts)))
 
 
 PATH_WORKLOADS = [
@@ -172,6 +180,34 @@ def _rel_mse_trimmed(img, ref, trim=1e-3):
     return float(np.sort(d)[: int(len(d) * (1 - trim))].mean())
 
 
+GUIDE_FLOPS_PER_QUERY = 2 * (208 * 64 + 3 * 64 * 64 + 64 * 32)  # the five layers' multiply-adds
+FP16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def guide_inference_rate(ctx, guide, n=1 << 22, reps=5):
+    """Throughput of the guide's MFMA inference kernel on n queries (HIP events
+    on the context's stream around each launch)."""
+    import torch
+    raw = torch.rand((n, 8), dtype=torch.float32, device="cuda")
+    guide.infer(raw)
+    torch.cuda.synchronize()
+    st = torch.cuda.ExternalStream(ctx.stream()) if ctx.stream() else torch.cuda.current_stream()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        guide.infer(raw)
+        e1.record(st)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    tflops = GUIDE_FLOPS_PER_QUERY * n / (ms * 1e-3) / 1e12
+    log("guide inference: %d queries in %.3f ms -> %.1f TFLOP/s" % (n, ms, tflops))
+    return {"queries": n, "ms": round(ms, 4), "flops_per_query": GUIDE_FLOPS_PER_QUERY,
+            "tflops": round(tflops, 2), "peak_tflops": FP16_DENSE_PEAK_TFLOPS,
+            "frac": round(tflops / FP16_DENSE_PEAK_TFLOPS, 4), "bound": "mfma", "dtype": "fp16 x fp16 -> fp32"}
+
+
 def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
     """Learned path guide on the 1080p showcase scene: wall time for guided
     rendering (online training included) to reach the relative MSE that plain
@@ -205,6 +241,7 @@ def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
     cfg = dict(alpha=0.3, guided_depth=2, steps_per_pass=2, batch=16384, train_paths=1 << 16, warmup_steps=8,
                learning_rate=5e-3)
     g = Guide(ctx, **cfg)
+    infer = guide_inference_rate(ctx, g)
     f = Film(ctx, W, H)
     tot, curve, reached = 0.0, [], None
     for k in range(0, 2 * spp, 4):
@@ -221,7 +258,7 @@ def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
     return {"scene": "showcase 1920x1080, PT+NEE depth 8", "reference_spp": ref_spp,
             "reference_s": round(t_ref, 3), "error": "relMSE, 0.1% largest pixel errors dropped",
             "plain_spp": spp, "plain_s": round(t_plain, 4), "plain_error": round(e_plain, 6),
-            "guided_config": cfg, "guided_curve": curve,
+            "guided_config": cfg, "guided_curve": curve, "inference": infer,
             "time_to_equal_error_s": round(reached, 4) if reached else None,
             "time_ratio": round(reached / t_plain, 3) if reached else None}
 
@@ -234,7 +271,7 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--occlusion", choices=["exact", "anyhit"], default="exact")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
