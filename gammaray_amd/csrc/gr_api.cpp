@@ -461,7 +461,7 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
             best_ms = ms;
         }
     };
-    for (uint32_t bits : {12u, 18u, 24u, 30u}) {
+    for (uint32_t bits : {12u, 15u, 18u, 21u, 24u, 30u}) {  // 15 and 18: two and three radix passes
         gr::TraceTuning c = best;
         c.sort_bits = bits;
         consider(c);

@@ -152,8 +152,11 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Eight waves per SIMD keep the most rays in flight (the kernel waits on
+// dependent node fetches); instances with deeper LDS stacks are LDS-limited
+// below that anyway.
 template <int MODE, bool COUNT, int LDS, bool MULTI>
-__global__ __launch_bounds__(kWave) void k_trace(SceneView sv, const SceneRef* __restrict__ scenes,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_trace(SceneView sv, const SceneRef* __restrict__ scenes,
                                                   uint32_t nscenes, const float4* __restrict__ in, uint32_t n,
                                                   const uint32_t* __restrict__ perm,
                                                   typename Output<MODE>::type* __restrict__ out,
