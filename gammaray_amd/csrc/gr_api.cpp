@@ -448,7 +448,7 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    gr::TraceTuning best{};  // sort off, 16 LDS entries, occupancy limit, refill at 16
+    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 32
     float best_ms = 0.0f;
     if (e == hipSuccess) e = time_closest(s, best, d_rays, n, d_hits, a, b, &best_ms);
     // One knob at a time; a candidate must win by more than 1 % to replace the incumbent.
@@ -471,10 +471,10 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
         c.xcd_ranges = 1;
         consider(c);
     }
-    for (uint32_t lds : {8u, 32u}) {
+    for (uint32_t lds : {8u, 16u, 32u}) {
         gr::TraceTuning c = best;
         c.lds_stack = lds;
-        consider(c);
+        if (lds != best.lds_stack) consider(c);
     }
     const int occ = gr::trace_occupancy(best.lds_stack);
     for (int frac : {3, 2, 1}) {
@@ -482,10 +482,10 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
         c.waves_per_cu = (uint32_t)std::max(1, occ * frac / 4);
         if ((int)c.waves_per_cu < occ) consider(c);
     }
-    for (uint32_t rf : {4u, 8u, 32u}) {
+    for (uint32_t rf : {4u, 8u, 16u, 32u}) {
         gr::TraceTuning c = best;
         c.refill = rf;
-        consider(c);
+        if (rf != best.refill) consider(c);
     }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);

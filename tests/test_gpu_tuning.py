@@ -57,7 +57,7 @@ def test_tuned_closest_hit_bit_exact(tu, gpu_ctx):
     spec = _scene()
     stage = Stage(spec, ctx=gpu_ctx)
     got = stage.set_tuning(**tu)
-    assert got["lds_stack"] == tu.get("lds_stack", 16)
+    assert got["lds_stack"] == tu.get("lds_stack", 8)
     o = OracleScene(spec)
     n = 300_001  # not a multiple of the sort tile or the wave
     rays = scenes.ray_batch(spec, n, seed=31)
@@ -114,7 +114,7 @@ def test_sorted_render_is_bit_identical(integrator, gpu_ctx):
     spec = scenes.showcase_scene(n_spheres=60, grid=(40, 40))
     W, H = 64, 48
     films = []
-    for tu in [dict(), dict(sort_bits=18, lds_stack=8, refill=8, xcd_ranges=1)]:
+    for tu in [dict(lds_stack=16, refill=16), dict(sort_bits=18, lds_stack=8, refill=8, xcd_ranges=1)]:
         stage = Stage(spec, ctx=gpu_ctx)
         stage.set_tuning(**tu)
         film = Film(gpu_ctx, W, H)
