@@ -92,18 +92,6 @@ __host__ __device__ inline int guide_bwd_index(int l, int in, int out) {
     return tile * kFragHalves + lane * 4 + out % 4;
 }
 
-__device__ __forceinline__ float guide_pick8(const float x[8], uint32_t d) {
-    float v = x[0];
-    v = d == 1 ? x[1] : v;
-    v = d == 2 ? x[2] : v;
-    v = d == 3 ? x[3] : v;
-    v = d == 4 ? x[4] : v;
-    v = d == 5 ? x[5] : v;
-    v = d == 6 ? x[6] : v;
-    v = d == 7 ? x[7] : v;
-    return v;
-}
-
 // sin(pi a), cos(pi a) of 2^k x, reduced exactly to [0, 2) first.
 __device__ __forceinline__ void guide_sincos(float x, int k, float* s, float* c) {
     float a = ldexpf(x, k);
@@ -112,30 +100,55 @@ __device__ __forceinline__ void guide_sincos(float x, int k, float* s, float* c)
     *c = cospif(a);
 }
 
-// Encoded features f0..f0+3 (f0 a multiple of 4) of one query.
-__device__ __forceinline__ half4_t guide_features4(const float x[8], uint32_t f0) {
-    half4_t h;
-    if (f0 >= (uint32_t)(kGuideEncoded + kGuideRaw)) {
-        h[0] = h[1] = h[2] = h[3] = (_Float16)0.0f;
-    } else if (f0 >= (uint32_t)kGuideEncoded) {
-        const uint32_t d = f0 - kGuideEncoded;  // 0 or 4
-        h[0] = (_Float16)guide_pick8(x, d);
-        h[1] = (_Float16)guide_pick8(x, d + 1);
-        h[2] = (_Float16)guide_pick8(x, d + 2);
-        h[3] = (_Float16)guide_pick8(x, d + 3);
-    } else {
-        // feature f = 24 d + 2 k + (0: sin, 1: cos)
-        const uint32_t d = f0 / 24, k = (f0 % 24) / 2;
-        const float xd = guide_pick8(x, d);
-        float s0, c0, s1, c1;
-        guide_sincos(xd, (int)k, &s0, &c0);
-        guide_sincos(xd, (int)k + 1, &s1, &c1);
-        h[0] = (_Float16)s0;
-        h[1] = (_Float16)c0;
-        h[2] = (_Float16)s1;
-        h[3] = (_Float16)c1;
+// Encoded features of one query, staged in a per-wave LDS tile of 16 queries
+// x kFeatStride halves.  The lane holding query q and group g (lane / 16)
+// writes input dimensions 2g and 2g+1: sin/cos(2^k pi x) for k = 0..11 from
+// exact values at k = 0, 4, 8 and the double-angle recurrence in between (at
+// most three doublings, so the error stays within 8x of the direct values),
+// plus the raw value; group 0 also writes the zero padding.
+constexpr int kFeatStride = 212;  // halves per query row: 208 used, padded against LDS bank conflicts
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// xa, xb: the query's raw inputs 0..3 and 4..7.
+__device__ __forceinline__ void guide_encode_tile(float4 xa, float4 xb, _Float16* tile, uint32_t lane) {
+    const uint32_t q = lane & 15u, g = lane >> 4;
+    _Float16* row = tile + q * kFeatStride;
+    // this lane's two dimensions, 2g and 2g+1
+    const float x0 = g == 0 ? xa.x : (g == 1 ? xa.z : (g == 2 ? xb.x : xb.z));
+    const float x1 = g == 0 ? xa.y : (g == 1 ? xa.w : (g == 2 ? xb.y : xb.w));
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t d = 2 * g + j;
+        const float xd = j == 0 ? x0 : x1;
+        half2_t* pair = reinterpret_cast<half2_t*>(row + 24 * d);
+#pragma unroll
+        for (int a = 0; a < kGuideFreqs; a += 4) {
+            float s, c;
+            guide_sincos(xd, a, &s, &c);
+#pragma unroll
+            for (int k = a; k < a + 4; ++k) {
+                half2_t v;
+                v[0] = (_Float16)s;
+                v[1] = (_Float16)c;
+                pair[k] = v;
+                const float s2 = 2.0f * s * c;
+                const float c2 = (c - s) * (c + s);
+                s = s2;
+                c = c2;
+            }
+        }
     }
-    return h;
+    half2_t raw;
+    raw[0] = (_Float16)x0;
+    raw[1] = (_Float16)x1;
+    reinterpret_cast<half2_t*>(row + kGuideEncoded)[g] = raw;
+    if (g == 0) {
+        half2_t z;
+        z[0] = z[1] = (_Float16)0.0f;
+#pragma unroll
+        for (int k = (kGuideEncoded + kGuideRaw) / 2; k < kGuideIn / 2; ++k) reinterpret_cast<half2_t*>(row)[k] = z;
+    }
 }
 
 // Raw inputs of a surface vertex.  lo/inv_ext: scene box.

@@ -23,7 +23,7 @@
 namespace gr {
 namespace {
 
-constexpr int kInferThreads = 512;
+constexpr int kInferThreads = 768;  // 12 waves: weights (54 KiB) + 12 feature tiles fit the LDS
 constexpr int kTrainThreads = 256;
 constexpr int kAFrags = 29;  // layer-input fragments per 16-query tile: 13 + 4 * 4
 constexpr int kZFrags = 18;  // dZ fragments per tile: 4 * 4 + 2
@@ -40,6 +40,13 @@ __device__ __forceinline__ float4_t zero4() {
     float4_t z;
     z[0] = z[1] = z[2] = z[3] = 0.0f;
     return z;
+}
+
+// Orders one wave's LDS writes before its later reads of other lanes' data.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Biases gathered for the forward pass: b1..b4 (64 each) then b5 (32).
@@ -59,15 +66,16 @@ struct BiasView {
 // activations (B layout) for back-propagation; the encoded inputs are
 // recomputed where needed.  o: the 32 outputs (C layout).
 template <typename WP, typename BP>
-__device__ __forceinline__ void guide_forward(WP W, BP bias, const float x[8], uint32_t lane, half4_t h[4][4],
+__device__ __forceinline__ void guide_forward(WP W, BP bias, const _Float16* ftile, uint32_t lane, half4_t h[4][4],
                                               float4_t o[2]) {
     const uint32_t g = lane >> 4;
+    const _Float16* frow = ftile + (lane & 15u) * kFeatStride + 4 * g;
     float4_t acc[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) acc[ot] = zero4();
-#pragma unroll 1
+#pragma unroll
     for (int kb = 0; kb < 13; ++kb) {
-        const half4_t f = guide_features4(x, (uint32_t)(kb * 16) + 4 * g);
+        const half4_t f = *reinterpret_cast<const half4_t*>(frow + kb * 16);
 #pragma unroll
         for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma16(W[(guide_fwd_base(0) + ot * 13 + kb) * 64 + lane], f, acc[ot]);
     }
@@ -135,13 +143,11 @@ __device__ __forceinline__ void head_weights(const Lobe& a, const Lobe& b, float
     *wb = eb / s;
 }
 
-__device__ __forceinline__ void load_raw(const float* raw, uint32_t q, bool valid, float x[8]) {
+__device__ __forceinline__ void load_raw(const float* raw, uint32_t q, bool valid, float4* xa, float4* xb) {
+    *xa = *xb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
-        const float4 a = reinterpret_cast<const float4*>(raw)[2 * (size_t)q];
-        const float4 b = reinterpret_cast<const float4*>(raw)[2 * (size_t)q + 1];
-        x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
-    } else {
-        for (int i = 0; i < 8; ++i) x[i] = 0.0f;
+        *xa = reinterpret_cast<const float4*>(raw)[2 * (size_t)q];
+        *xb = reinterpret_cast<const float4*>(raw)[2 * (size_t)q + 1];
     }
 }
 
@@ -152,6 +158,7 @@ __global__ __launch_bounds__(kInferThreads) void k_guide_infer(const uint16_t* _
                                                                float* __restrict__ mix) {
     __shared__ half4_t s_w[kFwdTiles * 64];
     __shared__ float s_b[4 * kGuideWidth + kGuideOut];
+    __shared__ __attribute__((aligned(16))) _Float16 s_f[kInferThreads / 64][16 * kFeatStride];
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < (uint32_t)kFwdTiles * 64; i += kInferThreads)
         s_w[i] = reinterpret_cast<const half4_t*>(fwd)[i];
@@ -168,11 +175,15 @@ __global__ __launch_bounds__(kInferThreads) void k_guide_infer(const uint16_t* _
     for (uint32_t tile = blockIdx.x * (kInferThreads / 64) + (tid >> 6); tile * 16 < n; tile += waves) {
         const uint32_t q = tile * 16 + (lane & 15u);
         const bool valid = q < n;
-        float x[8];
-        load_raw(raw, q, valid, x);
+        float4 xa, xb;
+        load_raw(raw, q, valid, &xa, &xb);
+        _Float16* ft = s_f[tid >> 6];
+        wave_lds_sync();  // the previous tile's reads are done
+        guide_encode_tile(xa, xb, ft, lane);
+        wave_lds_sync();
         half4_t h[4][4];
         float4_t o[2];
-        guide_forward(s_w, s_b, x, lane, h, o);
+        guide_forward(s_w, s_b, ft, lane, h, o);
         const Lobe la = head_lobe(o[0]), lb = head_lobe(o[1]);
         float wa, wb;
         head_weights(la, lb, &wa, &wb);
@@ -249,6 +260,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
                                                                uint16_t* __restrict__ at_buf,
                                                                uint16_t* __restrict__ dz_buf) {
     __shared__ _Float16 s_t[kTrainThreads / 64][256];
+    __shared__ __attribute__((aligned(16))) _Float16 s_f[kTrainThreads / 64][16 * kFeatStride];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6, g = lane >> 4;
     const uint32_t tile = blockIdx.x * (kTrainThreads / 64) + wave;
     if (tile * 16 >= batch) return;
@@ -262,21 +274,23 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
     uint32_t ri = 0;
     if (valid) ri = sequential ? i : hash32(draw_key ^ hash32(i)) % n;
     if (sequential && ri >= n) valid = false;
-    float x[8];
     const float* rec = recs + (size_t)ri * kRecordFloats;
+    float4 xa = make_float4(0.0f, 0.0f, 0.0f, 0.0f), xb = xa;
     if (valid) {
-        for (int k = 0; k < 8; ++k) x[k] = rec[k];
-    } else {
-        for (int k = 0; k < 8; ++k) x[k] = 0.0f;
+        xa = make_float4(rec[0], rec[1], rec[2], rec[3]);
+        xb = make_float4(rec[4], rec[5], rec[6], rec[7]);
     }
     const vec3 dir = valid ? mk(rec[8], rec[9], rec[10]) : mk(0.0f, 0.0f, 1.0f);
     float vnorm = 0.0f;
     if (valid && stats[0] > 0.0f) vnorm = fminf(fmaxf(rec[11] * (float)n / stats[0], 0.0f), kValueClip);
     const half4_t* W = reinterpret_cast<const half4_t*>(fwd);
     const half4_t* Wb = reinterpret_cast<const half4_t*>(bwd);
+    _Float16* ft = s_f[wave];
+    guide_encode_tile(xa, xb, ft, lane);
+    wave_lds_sync();
     half4_t h[4][4];
     float4_t o[2];
-    guide_forward(W, BiasView{params}, x, lane, h, o);
+    guide_forward(W, BiasView{params}, ft, lane, h, o);
 
     // ---- output head: d(-vnorm log q)/d outputs
     const Lobe la = head_lobe(o[0]), lb = head_lobe(o[1]);
@@ -357,9 +371,15 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
                 store_transposed(h[l - 1][kb], lds, at + (13 + 4 * (l - 1) + kb) * kFragHalves, lane);
         }
     }
-#pragma unroll 1
-    for (int kb = 0; kb < 13; ++kb)
-        store_transposed(guide_features4(x, (uint32_t)(kb * 16) + 4 * g), lds, at + kb * kFragHalves, lane);
+    // layer-1 inputs as dW A fragments: lane holds feature kb*16 + lane%16 of
+    // queries 4(lane/16)..+3, read straight from the feature tile
+#pragma unroll
+    for (int kb = 0; kb < 13; ++kb) {
+        half4_t r;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r[c] = ft[(4 * g + c) * kFeatStride + kb * 16 + (lane & 15u)];
+        reinterpret_cast<half4_t*>(at + kb * kFragHalves)[lane] = r;
+    }
 }
 
 // dW_l[in][out] += sum over the batch of A_{l-1}[q][in] dZ_l[q][out]: one
@@ -589,7 +609,7 @@ hipError_t guide_infer(gr_guide* g, const float* raw, uint32_t n_max, const uint
     if (n_max == 0) return hipSuccess;
     const uint32_t tiles = (n_max + 15) / 16;
     uint32_t blocks = (tiles + kInferThreads / 64 - 1) / (kInferThreads / 64);
-    const uint32_t cap = (uint32_t)cus_of_device() * 2;  // LDS (55 KiB of weights) allows two blocks per CU
+    const uint32_t cap = (uint32_t)cus_of_device();  // one block per CU: its LDS holds the weights and 12 feature tiles
     if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferThreads), 0, s, g->fwd, g->params, raw, n_max, n_ptr,
                        mix);
