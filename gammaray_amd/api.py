@@ -305,6 +305,8 @@ class Guide:
     def infer(self, raw):
         """raw: (n, 8) float32 device tensor -> (n, 40) mixtures."""
         import torch
+        if raw.dim() != 2 or raw.shape[1] != 8 or raw.dtype != torch.float32 or not raw.is_contiguous():
+            raise ValueError("raw inputs must be a contiguous (n, 8) float32 tensor")
         out = torch.empty((raw.shape[0], GUIDE_MIX_FLOATS), dtype=torch.float32, device=raw.device)
         check(lib().gr_guide_infer(self._h, ctypes.c_void_p(raw.data_ptr()), raw.shape[0],
                                    ctypes.c_void_p(out.data_ptr())), self.ctx.handle, "guide_infer")

@@ -53,7 +53,47 @@ constexpr int kBwdTiles = 56;
 __host__ __device__ constexpr int guide_bwd_base(int l) { return 16 * (l - 1); }
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Inference fragments for v_mfma_f32_16x16x32_f16 (A: lane L holds row L%16,
+// k = 8(L/16)..+7; B: column L%16, same k; C as for 16x16x16).  Layer 1's k
+// runs over 7 blocks of 32 in the order the encoder produces features in
+// registers: lane group G (lane / 16) owns input dimensions 2G and 2G+1, and
+// its 8 values of block kb are features 48G + 8kb .. +7 for kb < 6 (the
+// sin/cos pairs of one dimension at 4 consecutive frequencies) and, for kb =
+// 6, raw inputs 2G, 2G+1 and six zeros.  A hidden layer's 32-wide k block kb
+// is the pair of C tiles 2kb, 2kb+1 as each lane holds them -- k = 8G + i is
+// feature (2kb + i/4)*16 + 4G + i%4 -- so activations move straight from one
+// layer's C tiles into the next layer's B operand.
+constexpr int kFrag32Halves = 512;
+constexpr int kFwd32Tiles = 56;  // 4x7, 4x2, 4x2, 4x2, 2x2
+__host__ __device__ constexpr int guide_fwd32_base(int l) { return l == 0 ? 0 : 28 + 8 * (l - 1); }
+// Fragment position of weight (l, in, out); -1 for layer 1's zero padding rows.
+__host__ __device__ inline int guide_fwd32_index(int l, int in, int out) {
+    const int ot = out / 16;
+    int kb, g, i;
+    if (l == 0) {
+        if (in < kGuideEncoded) {
+            g = in / 48;
+            kb = (in % 48) / 8;
+            i = in % 8;
+        } else if (in < kGuideEncoded + kGuideRaw) {
+            g = (in - kGuideEncoded) / 2;
+            kb = 6;
+            i = (in - kGuideEncoded) % 2;
+        } else {
+            return -1;
+        }
+    } else {
+        const int b16 = in / 16, r = in % 16;
+        kb = b16 / 2;
+        g = r / 4;
+        i = (b16 % 2) * 4 + r % 4;
+    }
+    const int tile = guide_fwd32_base(l) + ot * (l == 0 ? 7 : 2) + kb;
+    return tile * kFrag32Halves + (g * 16 + out % 16) * 8 + i;
+}
 
 __host__ __device__ inline int guide_layer_in(int l) { return l == 0 ? kGuideIn : kGuideWidth; }
 __host__ __device__ inline int guide_layer_out(int l) { return l == 4 ? kGuideOut : kGuideWidth; }
@@ -92,53 +132,68 @@ __host__ __device__ inline int guide_bwd_index(int l, int in, int out) {
     return tile * kFragHalves + lane * 4 + out % 4;
 }
 
-// sin(pi a), cos(pi a) of 2^k x, reduced exactly to [0, 2) first.
+// sin/cos(2^k pi x) = sin/cos(2 pi r) with r = 2^(k-1) x - floor(2^(k-1) x):
+// the scaling and the reduction are exact in float, and v_sin/v_cos take the
+// angle in revolutions (absolute error near 1e-6, far below the fp16
+// rounding the features get).
 __device__ __forceinline__ void guide_sincos(float x, int k, float* s, float* c) {
-    float a = ldexpf(x, k);
-    a = a - 2.0f * floorf(a * 0.5f);
-    *s = sinpif(a);
-    *c = cospif(a);
+    const float a = ldexpf(x, k - 1);
+    const float r = a - floorf(a);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
 }
 
-// Encoded features of one query, staged in a per-wave LDS tile of 16 queries
-// x kFeatStride halves.  The lane holding query q and group g (lane / 16)
-// writes input dimensions 2g and 2g+1: sin/cos(2^k pi x) for k = 0..11 from
-// exact values at k = 0, 4, 8 and the double-angle recurrence in between (at
-// most three doublings, so the error stays within 8x of the direct values),
-// plus the raw value; group 0 also writes the zero padding.
-constexpr int kFeatStride = 212;  // halves per query row: 208 used, padded against LDS bank conflicts
-
+// Input encoding: sin/cos(2^k pi x) for k = 0..11 of each raw input, then
+// the raw inputs themselves, then zeros to 208.  Each run of four frequencies
+// k = a..a+3 (a = 0, 4, 8) starts from the exact values at k = a and takes
+// three double-angle steps, so the error stays within 8x of the direct values.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Features 2a..2a+7 of input x, interleaved (sin, cos) per frequency.
+__device__ __forceinline__ half8_t guide_encode_run(float x, int a) {
+    float s, c;
+    guide_sincos(x, a, &s, &c);
+    half8_t v;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[2 * k] = (_Float16)s;
+        v[2 * k + 1] = (_Float16)c;
+        const float s2 = 2.0f * s * c;
+        const float c2 = (c - s) * (c + s);
+        s = s2;
+        c = c2;
+    }
+    return v;
+}
+
+// Layer 1's K=32 B operand for block kb of the lane's query, whose inputs
+// 2G and 2G+1 (G = lane / 16) are x0 and x1.
+__device__ __forceinline__ half8_t guide_encode_block(float x0, float x1, int kb) {
+    if (kb < 6) return guide_encode_run(kb < 3 ? x0 : x1, 4 * (kb % 3));
+    half8_t v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (_Float16)0.0f;
+    v[0] = (_Float16)x0;
+    v[1] = (_Float16)x1;
+    return v;
+}
+
+// Encoded features of 16 queries, staged in a per-wave LDS tile of 16 x
+// kFeatStride halves (training reads it as K=16 B operands and, transposed,
+// as the layer-1 activations of the weight gradient).  The lane holding query
+// q and group g (lane / 16) writes input dimensions 2g and 2g+1; group 0
+// also writes the zero padding.
+constexpr int kFeatStride = 216;  // halves per query row: 208 used; 16-byte aligned rows
 
 // xa, xb: the query's raw inputs 0..3 and 4..7.
 __device__ __forceinline__ void guide_encode_tile(float4 xa, float4 xb, _Float16* tile, uint32_t lane) {
     const uint32_t q = lane & 15u, g = lane >> 4;
     _Float16* row = tile + q * kFeatStride;
-    // this lane's two dimensions, 2g and 2g+1
     const float x0 = g == 0 ? xa.x : (g == 1 ? xa.z : (g == 2 ? xb.x : xb.z));
     const float x1 = g == 0 ? xa.y : (g == 1 ? xa.w : (g == 2 ? xb.y : xb.w));
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const uint32_t d = 2 * g + j;
-        const float xd = j == 0 ? x0 : x1;
-        half2_t* pair = reinterpret_cast<half2_t*>(row + 24 * d);
-#pragma unroll
-        for (int a = 0; a < kGuideFreqs; a += 4) {
-            float s, c;
-            guide_sincos(xd, a, &s, &c);
-#pragma unroll
-            for (int k = a; k < a + 4; ++k) {
-                half2_t v;
-                v[0] = (_Float16)s;
-                v[1] = (_Float16)c;
-                pair[k] = v;
-                const float s2 = 2.0f * s * c;
-                const float c2 = (c - s) * (c + s);
-                s = s2;
-                c = c2;
-            }
-        }
-    }
+    for (int r = 0; r < 6; ++r)
+        *reinterpret_cast<half8_t*>(row + 48 * g + 8 * r) = guide_encode_block(x0, x1, r);
     half2_t raw;
     raw[0] = (_Float16)x0;
     raw[1] = (_Float16)x1;

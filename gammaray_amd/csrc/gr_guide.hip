@@ -1,13 +1,16 @@
 // gr_guide.hip -- the learned path guide on gfx950: inference, training
 // (forward, backward, weight gradients) and Adam, all on the matrix cores
-// (v_mfma_f32_16x16x16_f16: fp16 operands, fp32 accumulation).
+// (fp16 operands, fp32 accumulation: v_mfma_f32_16x16x32_f16 for inference,
+// v_mfma_f32_16x16x16_f16 for training).
 //
-// Layout: one wave evaluates 16 queries at a time.  Activations are held
+// Layout: one wave evaluates 16-query tiles (inference: two at a time, so each
+// weight fragment read from LDS feeds two MFMAs).  Activations are held
 // transposed (rows = features, columns = queries), so the C tiles a layer
 // produces are, converted to fp16, exactly the B tiles the next layer consumes
 // -- no data movement between layers.  Weights are stored pre-swizzled as A
-// fragments (64 lanes x 4 halves per 16x16 tile): W^T tiles for the forward
-// pass, W tiles for back-propagation.  Weight gradients are a GEMM over the
+// fragments: W^T tiles in K=32 order for inference, W^T tiles (64 lanes x 4
+// halves per 16x16 tile) for the training forward pass, W tiles for
+// back-propagation.  Weight gradients are a GEMM over the
 // batch dimension (dW_l = A_{l-1}^T dZ_l); the training kernel writes both
 // operands transposed into fragment order through a per-wave LDS tile and a
 // split-K kernel reduces them.
@@ -23,7 +26,7 @@
 namespace gr {
 namespace {
 
-constexpr int kInferThreads = 768;  // 12 waves: weights (54 KiB) + 12 feature tiles fit the LDS
+constexpr int kInferThreads = 768;  // 12 waves sharing one LDS copy of the inference weights
 constexpr int kTrainThreads = 256;
 constexpr int kAFrags = 29;  // layer-input fragments per 16-query tile: 13 + 4 * 4
 constexpr int kZFrags = 18;  // dZ fragments per tile: 4 * 4 + 2
@@ -34,6 +37,14 @@ constexpr float kValueClip = 20.0f;
 
 __device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float4_t mfma32(half8_t a, half8_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ half8_t cat8(half4_t lo, half4_t hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 __device__ __forceinline__ float4_t zero4() {
@@ -110,6 +121,68 @@ __device__ __forceinline__ void guide_forward(WP W, BP bias, const _Float16* fti
         for (int c = 0; c < 4; ++c) o[ot][c] = o[ot][c] + bias[4 * kGuideWidth + ot * 16 + 4 * g + c];
 }
 
+// The forward pass with the K=32 instruction (inference) for T 16-query
+// tiles at once, so that each weight fragment read from LDS feeds T MFMAs.
+// x0[t], x1[t]: raw inputs 2G and 2G+1 (G = lane / 16) of the lane's query in
+// tile t; layer 1's operands are encoded in registers just before use.
+template <int T>
+__device__ __forceinline__ void guide_forward32(const half8_t* W, const float* bias, const float* x0,
+                                                const float* x1, uint32_t lane, float4_t o[][2]) {
+    const uint32_t g = lane >> 4;
+    // each accumulator starts from its bias: rows 4g..4g+3 of output tile ot
+    const float4_t* B4 = reinterpret_cast<const float4_t*>(bias);
+    float4_t acc[T][4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+        const float4_t b = B4[ot * 4 + g];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t][ot] = b;
+    }
+#pragma unroll
+    for (int kb = 0; kb < 7; ++kb) {
+        half8_t f[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) f[t] = guide_encode_block(x0[t], x1[t], kb);
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) {
+            const half8_t w = W[(guide_fwd32_base(0) + ot * 7 + kb) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t][ot] = mfma32(w, f[t], acc[t][ot]);
+        }
+    }
+    half4_t h[T][4];
+    const half4_t hz = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+#pragma unroll
+    for (int l = 1; l <= 4; ++l) {
+        // ReLU after the fp16 rounding: the same values as before it, in packed halves
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot)
+                h[t][ot] = __builtin_elementwise_max(__builtin_convertvector(acc[t][ot], half4_t), hz);
+        const int nout = l == 4 ? 2 : 4;
+#pragma unroll
+        for (int ot = 0; ot < nout; ++ot) {
+            const float4_t b = B4[l * 16 + ot * 4 + g];
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t][ot] = b;
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+            for (int ot = 0; ot < nout; ++ot) {
+                const half8_t w = W[(guide_fwd32_base(l) + ot * 2 + kb) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < T; ++t) acc[t][ot] = mfma32(w, cat8(h[t][2 * kb], h[t][2 * kb + 1]), acc[t][ot]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) o[t][ot] = acc[t][ot];
+}
+
 // One lobe's parameters from its 4 outputs (logit, log kappa, theta, phi).
 struct Lobe {
     float logit, lk_raw, kappa, sa, sb, theta, phi;
@@ -143,56 +216,98 @@ __device__ __forceinline__ void head_weights(const Lobe& a, const Lobe& b, float
     *wb = eb / s;
 }
 
-__device__ __forceinline__ void load_raw(const float* raw, uint32_t q, bool valid, float4* xa, float4* xb) {
-    *xa = *xb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (valid) {
-        *xa = reinterpret_cast<const float4*>(raw)[2 * (size_t)q];
-        *xb = reinterpret_cast<const float4*>(raw)[2 * (size_t)q + 1];
-    }
+// The inference head on the hardware exp2, reciprocal, sin and cos (angles in
+// revolutions): theta = pi sa is half a revolution times sa, phi = 2 pi sb is
+// sb revolutions.  The sampler and the pdf both read the mixture it writes,
+// so their approximation errors (a few ulp) cannot bias a guided estimate.
+__device__ __forceinline__ float hw_exp(float v) { return __builtin_amdgcn_exp2f(v * 1.44269504088896341f); }
+__device__ __forceinline__ float hw_sigmoid(float v) { return __builtin_amdgcn_rcpf(1.0f + hw_exp(-v)); }
+
+struct FastLobe {
+    float logit, kappa;
+    vec3 mu;
+};
+
+__device__ __forceinline__ FastLobe head_lobe_fast(float4_t o) {
+    FastLobe L;
+    L.logit = o[0];
+    L.kappa = hw_exp(fminf(fmaxf(o[1], kLogKappaLo), kLogKappaHi));
+    const float ht = 0.5f * hw_sigmoid(o[2]), sb = hw_sigmoid(o[3]);
+    const float st = __builtin_amdgcn_sinf(ht), sp = __builtin_amdgcn_sinf(sb), cp = __builtin_amdgcn_cosf(sb);
+    L.mu = mk(st * cp, st * sp, __builtin_amdgcn_cosf(ht));
+    return L;
 }
 
-__global__ __launch_bounds__(kInferThreads) void k_guide_infer(const uint16_t* __restrict__ fwd,
+__device__ __forceinline__ void head_weights_fast(const FastLobe& a, const FastLobe& b, float* wa, float* wb) {
+    float m = fmaxf(a.logit, b.logit);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    const float ea = hw_exp(a.logit - m), eb = hw_exp(b.logit - m);
+    float s = ea + eb;
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    const float r = __builtin_amdgcn_rcpf(s);
+    *wa = ea * r;
+    *wb = eb * r;
+}
+
+// Tiles of 16 queries per wave and pass: each weight fragment feeds that many MFMAs.
+constexpr int kInferTiles = 2;
+
+// Raw inputs 2G and 2G+1 of query q (G = lane / 16); zeros past the end.
+__device__ __forceinline__ float2 load_pair(const float* raw, size_t q, size_t n, uint32_t g) {
+    return q < n ? reinterpret_cast<const float2*>(raw)[q * 4 + g] : make_float2(0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kInferThreads) void k_guide_infer(const uint16_t* __restrict__ fwd32,
                                                                const float* __restrict__ params,
                                                                const float* __restrict__ raw, uint32_t n_max,
                                                                const uint32_t* __restrict__ n_ptr,
                                                                float* __restrict__ mix) {
-    __shared__ half4_t s_w[kFwdTiles * 64];
-    __shared__ float s_b[4 * kGuideWidth + kGuideOut];
-    __shared__ __attribute__((aligned(16))) _Float16 s_f[kInferThreads / 64][16 * kFeatStride];
+    __shared__ half8_t s_w[kFwd32Tiles * 64];
+    __shared__ __attribute__((aligned(16))) float s_b[4 * kGuideWidth + kGuideOut];
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < (uint32_t)kFwdTiles * 64; i += kInferThreads)
-        s_w[i] = reinterpret_cast<const half4_t*>(fwd)[i];
+    for (uint32_t i = tid; i < (uint32_t)kFwd32Tiles * 64; i += kInferThreads)
+        s_w[i] = reinterpret_cast<const half8_t*>(fwd32)[i];
     for (uint32_t i = tid; i < (uint32_t)(4 * kGuideWidth + kGuideOut); i += kInferThreads)
         s_b[i] = bias_at(params, (int)i);
     __syncthreads();
-    uint32_t n = n_max;
+    size_t n = n_max;
     if (n_ptr) {
         const uint32_t m = *n_ptr;
         n = m < n_max ? m : n_max;
     }
     const uint32_t lane = tid & 63u, g = lane >> 4;
-    const uint32_t waves = gridDim.x * (kInferThreads / 64);
-    for (uint32_t tile = blockIdx.x * (kInferThreads / 64) + (tid >> 6); tile * 16 < n; tile += waves) {
-        const uint32_t q = tile * 16 + (lane & 15u);
-        const bool valid = q < n;
-        float4 xa, xb;
-        load_raw(raw, q, valid, &xa, &xb);
-        _Float16* ft = s_f[tid >> 6];
-        wave_lds_sync();  // the previous tile's reads are done
-        guide_encode_tile(xa, xb, ft, lane);
-        wave_lds_sync();
-        half4_t h[4][4];
-        float4_t o[2];
-        guide_forward(s_w, s_b, ft, lane, h, o);
-        const Lobe la = head_lobe(o[0]), lb = head_lobe(o[1]);
-        float wa, wb;
-        head_weights(la, lb, &wa, &wb);
-        if (valid) {
-            float* out = mix + (size_t)q * kMixFloats;
-            float* pa = out + 5 * g;
-            float* pb = out + 5 * (4 + g);
-            pa[0] = wa, pa[1] = la.kappa, pa[2] = la.mu.x, pa[3] = la.mu.y, pa[4] = la.mu.z;
-            pb[0] = wb, pb[1] = lb.kappa, pb[2] = lb.mu.x, pb[3] = lb.mu.y, pb[4] = lb.mu.z;
+    const size_t span = 16 * kInferTiles;
+    const size_t waves = (size_t)gridDim.x * (kInferThreads / 64);
+    for (size_t base = ((size_t)blockIdx.x * (kInferThreads / 64) + (tid >> 6)) * span; base < n;
+         base += waves * span) {
+        float x0[kInferTiles], x1[kInferTiles];
+#pragma unroll
+        for (int t = 0; t < kInferTiles; ++t) {
+            const float2 v = load_pair(raw, base + 16 * t + (lane & 15u), n, g);
+            x0[t] = v.x;
+            x1[t] = v.y;
+        }
+        // the weight fragments stay in LDS: hiding the lane's offset from the
+        // optimiser keeps it from hoisting all 56 reads out of the loop (and
+        // spilling them to scratch)
+        uint32_t wl = lane;
+        asm volatile("" : "+v"(wl));
+        float4_t o[kInferTiles][2];
+        guide_forward32<kInferTiles>(s_w, s_b, x0, x1, wl, o);
+#pragma unroll
+        for (int t = 0; t < kInferTiles; ++t) {
+            const FastLobe la = head_lobe_fast(o[t][0]), lb = head_lobe_fast(o[t][1]);
+            float wa, wb;
+            head_weights_fast(la, lb, &wa, &wb);
+            const size_t q = base + 16 * t + (lane & 15u);
+            if (q < n) {
+                float* pa = mix + q * kMixFloats + 5 * g;
+                float* pb = pa + 5 * 4;
+                pa[0] = wa, pa[1] = la.kappa, pa[2] = la.mu.x, pa[3] = la.mu.y, pa[4] = la.mu.z;
+                pb[0] = wb, pb[1] = lb.kappa, pb[2] = lb.mu.x, pb[3] = lb.mu.y, pb[4] = lb.mu.z;
+            }
         }
     }
 }
@@ -416,20 +531,22 @@ __global__ __launch_bounds__(64) void k_guide_dw(const uint16_t* __restrict__ at
     }
 }
 
-__device__ __forceinline__ void store_fragments(uint16_t* fwd, uint16_t* bwd, int p, float w) {
+__device__ __forceinline__ void store_fragments(uint16_t* fwd, uint16_t* bwd, uint16_t* fwd32, int p, float w) {
     int l, in, out;
     if (!guide_weight_coords(p, &l, &in, &out)) return;
     const _Float16 h = (_Float16)w;
     uint16_t bits;
     __builtin_memcpy(&bits, &h, 2);
     fwd[guide_fwd_index(l, in, out)] = bits;
+    const int i32 = guide_fwd32_index(l, in, out);
+    if (i32 >= 0) fwd32[i32] = bits;
     if (l >= 1) bwd[guide_bwd_index(l, in, out)] = bits;
 }
 
 __global__ __launch_bounds__(256) void k_guide_pack(const float* __restrict__ params, uint16_t* __restrict__ fwd,
-                                                    uint16_t* __restrict__ bwd) {
+                                                    uint16_t* __restrict__ bwd, uint16_t* __restrict__ fwd32) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < kGuideParams) store_fragments(fwd, bwd, p, params[p]);
+    if (p < kGuideParams) store_fragments(fwd, bwd, fwd32, p, params[p]);
 }
 
 // Adam on the mean gradient of `count` records; clears the gradient.  Skipped
@@ -437,6 +554,7 @@ __global__ __launch_bounds__(256) void k_guide_pack(const float* __restrict__ pa
 __global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ params, float* __restrict__ m,
                                                     float* __restrict__ v, float* __restrict__ grad,
                                                     uint16_t* __restrict__ fwd, uint16_t* __restrict__ bwd,
+                                                    uint16_t* __restrict__ fwd32,
                                                     const uint32_t* __restrict__ n_ptr, uint32_t n_host,
                                                     float inv_batch, float lr, float bc1, float bc2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,7 +569,7 @@ __global__ __launch_bounds__(256) void k_guide_adam(float* __restrict__ params, 
     v[p] = vv;
     const float w = params[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + 1.0e-8f);
     params[p] = w;
-    store_fragments(fwd, bwd, p, w);
+    store_fragments(fwd, bwd, fwd32, p, w);
 }
 
 __global__ __launch_bounds__(256) void k_guide_fill(float* __restrict__ a, uint32_t n, float value) {
@@ -593,7 +711,8 @@ hipError_t upload_params(gr_guide* g, const float* host) {
     if (e == hipSuccess) e = hipMemsetAsync(g->v, 0, bytes, s);
     if (e == hipSuccess) e = hipMemsetAsync(g->grad, 0, bytes, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_guide_pack, dim3(blocks256(kGuideParams)), dim3(256), 0, s, g->params, g->fwd, g->bwd);
+        hipLaunchKernelGGL(k_guide_pack, dim3(blocks256(kGuideParams)), dim3(256), 0, s, g->params, g->fwd, g->bwd,
+                           g->fwd32);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -607,12 +726,13 @@ hipError_t upload_params(gr_guide* g, const float* host) {
 hipError_t guide_infer(gr_guide* g, const float* raw, uint32_t n_max, const uint32_t* n_ptr, float* mix,
                        hipStream_t s) {
     if (n_max == 0) return hipSuccess;
-    const uint32_t tiles = (n_max + 15) / 16;
-    uint32_t blocks = (tiles + kInferThreads / 64 - 1) / (kInferThreads / 64);
-    const uint32_t cap = (uint32_t)cus_of_device();  // one block per CU: its LDS holds the weights and 12 feature tiles
+    const uint32_t spans = (n_max + 16 * kInferTiles - 1) / (16 * kInferTiles);
+    uint32_t blocks = (spans + kInferThreads / 64 - 1) / (kInferThreads / 64);
+    // two blocks per CU: each holds its own copy of the weights (57 KiB of LDS)
+    const uint32_t cap = 2u * (uint32_t)cus_of_device();
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferThreads), 0, s, g->fwd, g->params, raw, n_max, n_ptr,
-                       mix);
+    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferThreads), 0, s, g->fwd32, g->params, raw, n_max,
+                       n_ptr, mix);
     return hipGetLastError();
 }
 
@@ -638,8 +758,8 @@ hipError_t guide_adam_steps(gr_guide* g, const float* recs, uint32_t n_max, cons
         const double t = (double)g->steps;
         const float bc1 = (float)(1.0 - pow(0.9, t)), bc2 = (float)(1.0 - pow(0.999, t));
         hipLaunchKernelGGL(k_guide_adam, dim3(blocks256(kGuideParams)), dim3(256), 0, s, g->params, g->m, g->v,
-                           g->grad, g->fwd, g->bwd, n_ptr, n_max, 1.0f / (float)batch, g->prm.learning_rate, bc1,
-                           bc2);
+                           g->grad, g->fwd, g->bwd, g->fwd32, n_ptr, n_max, 1.0f / (float)batch,
+                           g->prm.learning_rate, bc1, bc2);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -678,9 +798,10 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
     const size_t P = gr::kGuideParams;
     const size_t f32 = ((P * 4 + 255) / 256) * 256;
     const size_t fwd = (size_t)gr::kFwdTiles * gr::kFragHalves * 2, bwd = (size_t)gr::kBwdTiles * gr::kFragHalves * 2;
+    const size_t fwd32 = (size_t)gr::kFwd32Tiles * gr::kFrag32Halves * 2;
     char* p = nullptr;
     hipError_t e = hipSetDevice(ctx->device);
-    if (e == hipSuccess) e = hipMalloc(&p, 4 * f32 + fwd + bwd + 256);
+    if (e == hipSuccess) e = hipMalloc(&p, 4 * f32 + fwd + bwd + 256 + fwd32);
     if (e != hipSuccess) {
         delete g;
         return gr::hip_error(ctx, e, "guide allocation");
@@ -694,7 +815,9 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
     g->bwd = reinterpret_cast<uint16_t*>(p + 4 * f32 + fwd);
     g->stats = reinterpret_cast<float*>(p + 4 * f32 + fwd + bwd);
     g->count = reinterpret_cast<uint32_t*>(p + 4 * f32 + fwd + bwd + 128);
-    e = hipMemsetAsync(p + 4 * f32, 0, fwd + bwd + 256, ctx->stream);
+    g->fwd32 = reinterpret_cast<uint16_t*>(p + 4 * f32 + fwd + bwd + 256);
+    // zero: the fragments' padding (layer 1's inputs 208..223 in fwd32) is never written
+    e = hipMemsetAsync(p + 4 * f32, 0, fwd + bwd + 256 + fwd32, ctx->stream);
     const std::vector<float> init = gr::initial_params(prm->seed);
     if (e == hipSuccess) e = gr::upload_params(g, init.data());
     if (e != hipSuccess) {
@@ -740,6 +863,8 @@ int gr_guide_steps(gr_guide* g, uint64_t* steps) {
 int gr_guide_infer(gr_guide* g, const float* d_raw, uint32_t n, float* d_mix) {
     if (!g) return GR_ERR_INVALID;
     if (n && (!d_raw || !d_mix)) return gr::set_error(g->ctx, GR_ERR_INVALID, "null buffer");
+    if ((reinterpret_cast<uintptr_t>(d_raw) | reinterpret_cast<uintptr_t>(d_mix)) & 15u)
+        return gr::set_error(g->ctx, GR_ERR_INVALID, "raw inputs and mixtures must be 16-byte aligned");
     const hipError_t e = gr::guide_infer(g, d_raw, n, nullptr, d_mix, g->ctx->stream);
     return e == hipSuccess ? GR_OK : gr::hip_error(g->ctx, e, "guide inference");
 }

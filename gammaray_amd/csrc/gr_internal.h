@@ -230,7 +230,8 @@ struct gr_guide {
     gr_guide_params prm{};
     void* blob = nullptr;  // parameters, Adam moments, gradient, fp16 fragments, counters
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
-    uint16_t *fwd = nullptr, *bwd = nullptr;  // fp16 MFMA fragments of the weights
+    uint16_t *fwd = nullptr, *bwd = nullptr;  // fp16 MFMA fragments of the weights (K=16)
+    uint16_t* fwd32 = nullptr;                // inference fragments (K=32)
     float* stats = nullptr;                   // [0] sum of record values
     uint32_t* count = nullptr;                // records available for training
     uint64_t steps = 0;                       // Adam steps taken

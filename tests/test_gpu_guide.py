@@ -69,6 +69,40 @@ def test_inference_matches_oracle(gpu_ctx):
     np.testing.assert_allclose(qg, G.mixture_pdf(wr, kr, mr, d), rtol=6e-2, atol=1e-4)
 
 
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 200_003])
+def test_inference_ragged_counts(n, gpu_ctx):
+    """Counts that are not whole 16-query tiles: every query is written, nothing past n is."""
+    print("case guide inference ragged", n, flush=True)
+    import torch
+    rng = np.random.default_rng(n)
+    g = Guide(gpu_ctx)
+    p = _params(14, 4.0)
+    g.set_params(p)
+    raw = rng.uniform(0.0, 1.0, (n, 8)).astype(np.float32)
+    out = torch.full((n + 16, 40), -7.0, dtype=torch.float32, device="cuda")
+    from gammaray_amd._lib import check, lib
+    import ctypes
+    check(lib().gr_guide_infer(g._h, ctypes.c_void_p(_cuda(raw).data_ptr()), n, ctypes.c_void_p(out.data_ptr())))
+    got = out.cpu().numpy()
+    assert np.all(got[n:] == -7.0)
+    sel = rng.choice(n, min(n, 512), replace=False)
+    ref = G.mixture_of(p.astype(np.float64), raw[sel], f16=True)
+    w, k, mu = G.split_mixture(got[sel])
+    wr, kr, mr = G.split_mixture(ref)
+    np.testing.assert_allclose(w, wr, atol=5e-3)
+    np.testing.assert_allclose(k, kr, rtol=2e-2)
+    np.testing.assert_allclose(mu, mr, atol=1e-2)
+
+
+def test_inference_rejects_misaligned_buffers(gpu_ctx):
+    import torch
+    from gammaray_amd._lib import GrError
+    g = Guide(gpu_ctx)
+    flat = torch.zeros(8 * 32 + 1, dtype=torch.float32, device="cuda")
+    with pytest.raises(GrError):
+        g.infer(flat[1:].view(32, 8))
+
+
 def test_gradient_matches_oracle(gpu_ctx):
     print("case guide gradient", flush=True)
     rng = np.random.default_rng(2)
