@@ -321,7 +321,7 @@ def main():
     # Traversal settings: chosen before the timed region.
     if args.tuning == "auto":
         t0 = time.time()
-        tuning = stage.autotune(rays)
+        tuning = stage.autotune(rays, segs, mode)  # the step's own mix: closest hit + visibility
         log("autotuned %s (%.1f s)" % (tuning, time.time() - t0))
     elif args.tuning == "default":
         tuning = stage.get_tuning()

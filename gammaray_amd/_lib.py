@@ -109,6 +109,7 @@ SIGNATURES = {
     "gr_ctx_take_kernel_times": (_i, [_vp, _vp, _vp]),
     "gr_scene_get_tuning": (_i, [_vp, _vp]),
     "gr_scene_autotune": (_i, [_vp, _vp, _u32, _vp, _vp]),
+    "gr_scene_autotune_mixed": (_i, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _i, _vp]),
 }
 
 _lib = None

@@ -161,16 +161,27 @@ class Stage:
         check(lib().gr_scene_set_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "set_tuning")
         return self.get_tuning()
 
-    def autotune(self, rays):
-        """Times traversal settings on `rays` ((n, 8) float32 device tensor) and
-        keeps the fastest; returns the chosen setting."""
+    def autotune(self, rays, segments=None, mode=GR_OCCLUSION_EXACT):
+        """Times traversal settings on `rays` ((n, 8) float32 device tensor) and,
+        if given, visibility `segments` ((m, 8) float32) together, and keeps the
+        fastest; returns the chosen setting."""
         import torch
-        n = rays.shape[0]
-        hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        n = rays.shape[0] if rays is not None else 0
+        dev = rays.device if rays is not None else segments.device
+        hits = torch.empty((max(n, 1), 4), dtype=torch.int32, device=dev)
         t = _lib.Tuning()
-        check(lib().gr_scene_autotune(self._h, ctypes.c_void_p(rays.data_ptr()), n,
-                                      ctypes.c_void_p(hits.data_ptr()), ctypes.byref(t)),
-              self.ctx.handle, "autotune")
+        if segments is None:
+            check(lib().gr_scene_autotune(self._h, ctypes.c_void_p(rays.data_ptr() if n else 0), n,
+                                          ctypes.c_void_p(hits.data_ptr()), ctypes.byref(t)),
+                  self.ctx.handle, "autotune")
+        else:
+            m = segments.shape[0]
+            vis = torch.empty((max(m, 1),), dtype=torch.int32, device=dev)
+            check(lib().gr_scene_autotune_mixed(self._h, ctypes.c_void_p(rays.data_ptr() if n else 0), n,
+                                                ctypes.c_void_p(hits.data_ptr()),
+                                                ctypes.c_void_p(segments.data_ptr() if m else 0), m,
+                                                ctypes.c_void_p(vis.data_ptr()), mode, ctypes.byref(t)),
+                  self.ctx.handle, "autotune")
         return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
     def take_overflow_count(self):

@@ -419,15 +419,17 @@ int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
 }
 
 namespace {
-// Shortest of three timed closest-hit launches (after one untimed) in ms.
-hipError_t time_closest(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* rays, uint32_t n, gr_hit* hits,
-                        hipEvent_t a, hipEvent_t b, float* best) {
+// Best of three timed passes (after one untimed) of the workload under setting tu.
+hipError_t time_workload(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* rays, uint32_t n, gr_hit* hits,
+                         const gr_segment* segs, uint32_t m, uint32_t* vis, int mode, hipEvent_t a, hipEvent_t b,
+                         float* best) {
     const gr::TraceSetup ts{s->view, tu, &s->sort};
     hipStream_t st = s->ctx->stream;
     *best = 3.0e38f;
     for (int rep = 0; rep < 4; ++rep) {
         hipError_t e = hipEventRecord(a, st);
-        if (e == hipSuccess) e = gr::launch_intersect(ts, rays, n, nullptr, hits, nullptr, st);
+        if (e == hipSuccess && n) e = gr::launch_intersect(ts, rays, n, nullptr, hits, nullptr, st);
+        if (e == hipSuccess && m) e = gr::launch_occluded(ts, segs, m, nullptr, vis, mode, nullptr, st);
         if (e == hipSuccess) e = hipEventRecord(b, st);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float ms = 0.0f;
@@ -439,23 +441,30 @@ hipError_t time_closest(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* ra
 }
 }  // namespace
 
-int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, gr_tuning* out) {
+int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, const gr_segment* d_segs,
+                            uint32_t m, uint32_t* d_visible, int mode, gr_tuning* out) {
     if (!s) return GR_ERR_INVALID;
     if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
     if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
-    if (n == 0 || !d_rays || !d_hits) return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays and a hit buffer%s");
-    hipError_t e = gr::reserve_sort(s->sort, n);
+    if (n + (uint64_t)m == 0 || (n && (!d_rays || !d_hits)) || (m && (!d_segs || !d_visible)))
+        return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays with a hit buffer and/or segments with a result buffer%s");
+    if (mode != GR_OCCLUSION_EXACT && mode != GR_OCCLUSION_ANYHIT)
+        return fail(s->ctx, GR_ERR_INVALID, "unknown occlusion mode%s");
+    hipError_t e = gr::reserve_sort(s->sort, std::max(n, m));
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
     gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 32
     float best_ms = 0.0f;
-    if (e == hipSuccess) e = time_closest(s, best, d_rays, n, d_hits, a, b, &best_ms);
+    auto timed = [&](const gr::TraceTuning& tu, float* ms) {
+        return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
+    };
+    if (e == hipSuccess) e = timed(best, &best_ms);
     // One knob at a time; a candidate must win by more than 1 % to replace the incumbent.
     auto consider = [&](gr::TraceTuning cand) {
         if (e != hipSuccess) return;
         float ms = 0.0f;
-        e = time_closest(s, cand, d_rays, n, d_hits, a, b, &ms);
+        e = timed(cand, &ms);
         if (e == hipSuccess && ms < 0.99f * best_ms) {
             best = cand;
             best_ms = ms;
@@ -492,6 +501,11 @@ int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_h
     if (e != hipSuccess) return hip_fail(s->ctx, e, "autotune");
     s->tuning = best;
     return out ? gr_scene_get_tuning(s, out) : GR_OK;
+}
+
+int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, gr_tuning* out) {
+    if (s && n == 0) return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays and a hit buffer%s");
+    return gr_scene_autotune_mixed(s, d_rays, n, d_hits, nullptr, 0, nullptr, GR_OCCLUSION_EXACT, out);
 }
 
 int gr_film_create(gr_ctx* ctx, uint32_t width, uint32_t height, gr_film** out) {

@@ -400,6 +400,22 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Reserves n consecutive entries per lane with one atomic per wave; returns
+// the lane's first entry.  Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t* counter) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = n;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    uint32_t base = 0;
+    if (lane == 63u && incl) base = atomicAdd(counter, incl);
+    base = __shfl(base, 63);
+    return base + incl - n;
+}
+
 // Appends `alive` lanes to a queue with one atomic per wave; returns the slot.
 __device__ __forceinline__ uint32_t wave_append(bool alive, uint32_t* counter) {
     const uint64_t m = __ballot(alive);
@@ -937,6 +953,7 @@ struct BdptArgs {
     uint32_t* nl;        // light vertex count; bit 31 set = the light sample failed
     uint32_t* mask;      // visibility bits per path, mwords words each
     uint2* entry;        // shadow queue: (path, strategy)
+    uint32_t* list;      // the chunk's valid strategies, as s * chunk + (path - p0)
     float4* slots;       // per (strategy, path): contribution * weight, needs-visibility flag
     uint32_t cmax, lmax, mwords;
 };
@@ -1218,6 +1235,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     const size_t o_seg = off; off = align(off + shadow * 32);
     const size_t o_vis = off; off = align(off + shadow * 4);
     const size_t o_slots = off; off = align(off + nslots * sizeof(float4));
+    const size_t o_list = off; off = align(off + nslots * sizeof(uint32_t));
     if (rb.bdpt_capacity < off) {
         (void)hipFree(rb.bdpt_blob);
         rb.bdpt_blob = nullptr;
@@ -1234,6 +1252,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     b->mask = reinterpret_cast<uint32_t*>(base + o_mask);
     b->entry = reinterpret_cast<uint2*>(base + o_entry);
     b->slots = reinterpret_cast<float4*>(base + o_slots);
+    b->list = reinterpret_cast<uint32_t*>(base + o_list);
     b->cmax = cmax;
     b->lmax = lmax;
     b->mwords = mwords;
@@ -1245,35 +1264,58 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
 }  // namespace
 
 namespace {
-// One thread per (strategy, path) of paths [p0, p1), strategy-major so that a
-// wave reads the vertices of consecutive paths: evaluates connect(c, l) without
-// its visibility test, stores contribution * weight, and queues the shadow
-// segment of every strategy that needs one (wave-compacted).
+// The strategies (c, l) of paths [p0, p1) that connect() evaluates -- c <=
+// nc, l <= nl, c + l <= BDPT_MAX_DEPTH, light sample not failed -- listed
+// path by path as s * chunk + (p - p0), s = (c - 1)(lmax + 1) + l.  Most of
+// the cmax (lmax + 1) strategies of a path are invalid (subpaths end early),
+// so evaluating only the listed ones keeps every lane of a wave busy.
+__global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+    const uint32_t chunk = p1 - p0;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nc = 0, nl = 0, n = 0;
+    if (q < chunk) {
+        nl = b.nl[p0 + q];
+        nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
+        for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) n += min(nl, kBdptMaxDepth - c) + 1;
+    }
+    uint32_t at = wave_reserve(n, &a.rb.counts[4]);
+    for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) {
+        const uint32_t lm = min(nl, kBdptMaxDepth - c);
+        for (uint32_t l = 0; l <= lm; ++l) b.list[at++] = ((c - 1) * (b.lmax + 1) + l) * chunk + q;
+    }
+}
+
+// One lane per listed strategy (grid-stride over the list, whole waves per
+// step): evaluates connect(c, l) without its visibility test, stores
+// contribution * weight in the strategy's slot, and queues the shadow segment
+// of every strategy that needs one (wave-compacted).
 __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t chunk = p1 - p0;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nstrat = b.cmax * (b.lmax + 1);
-    bool emit = false;
-    uint32_t p = 0, s = 0;
-    Connection cn;
-    if (t < (size_t)nstrat * chunk) {
-        s = (uint32_t)(t / chunk);
-        p = p0 + (uint32_t)(t % chunk);
-        const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
-        const uint32_t nl = b.nl[p];
-        if (!(nl & kLightFailed) && c <= b.nc[p] && l <= nl && c + l <= kBdptMaxDepth) {
+    const uint32_t n = a.rb.counts[4];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t j = base + lane;
+        bool emit = false;
+        uint32_t p = 0, s = 0;
+        Connection cn;
+        if (j < n) {
+            const uint32_t t = b.list[j];
+            s = t / chunk;
+            p = p0 + (t - s * chunk);
+            const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
             cn = connect(a, b, p, c, l);
             const float w = 1.0f / (float)(c + l);
             const vec3 cw = cn.contrib * w;
             emit = l > 0 && cn.needs_vis;
             b.slots[t] = make_float4(cw.x, cw.y, cw.z, emit ? 1.0f : 0.0f);
         }
-    }
-    const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
-    if (emit) {
-        a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
-        a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
-        b.entry[slot] = make_uint2(p, s);
+        const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
+        if (emit) {
+            a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
+            a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
+            b.entry[slot] = make_uint2(p, s);
+        }
     }
 }
 
@@ -1282,8 +1324,9 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
                       float4* sample_out, hipStream_t s) {
     const size_t npaths = a.npaths;
     const size_t nstrat = (size_t)cmax * (lmax + 1);
-    // strategies per connection chunk: at most 2^25 (512 MiB of slots)
-    const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 25) / nstrat));
+    // strategies per connection chunk: at most 2^27 (2 GiB of slots, 512 MiB of
+    // strategy list; the shadow queue holds chunk * cmax * lmax segments)
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 27) / nstrat));
     const size_t shadow = chunk * cmax * lmax;
     BdptArgs b;
     float4* seg = nullptr;
@@ -1317,8 +1360,11 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     for (size_t p0 = 0; p0 < npaths && e == hipSuccess; p0 += chunk) {
         const size_t p1 = std::min(npaths, p0 + chunk);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 2u);
-        hipLaunchKernelGGL(k_connect_eval, blocks_for((p1 - p0) * nstrat), dim3(256), 0, s, a, b, (uint32_t)p0,
-                           (uint32_t)p1);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 4u);
+        hipLaunchKernelGGL(k_connect_list, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
+        // grid-stride over the list: enough waves to fill the chip, never more than the list can need
+        const dim3 eval_grid(std::min<unsigned>(blocks_for((p1 - p0) * nstrat).x, 8192u));
+        hipLaunchKernelGGL(k_connect_eval, eval_grid, dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
         e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
                             vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
@@ -1431,9 +1477,10 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         const uint32_t cmax = (prm.max_depth == 0 || prm.max_depth > kBdptMaxDepth) ? kBdptMaxDepth : prm.max_depth;
         const uint32_t lmax =
             (prm.light_depth == 0 || prm.light_depth + 1 > kBdptMaxDepth) ? kBdptMaxDepth : prm.light_depth + 1;
-        // vertex storage bounded to ~8 GiB per batch
+        // vertex storage bounded to 32 GiB per batch (1080p, depths 8 + 6: 8 spp,
+        // 26 GB): larger batches keep the traversal and connection launches full
         const size_t per_iter = npx * (cmax + lmax) * sizeof(BVert);
-        uint32_t bb = (uint32_t)std::max<size_t>(1, ((size_t)8 << 30) / per_iter);
+        uint32_t bb = (uint32_t)std::max<size_t>(1, ((size_t)32 << 30) / per_iter);
         bb = std::min(bb, batch);
         for (uint32_t done = 0; done < prm.spp && e == hipSuccess; done += bb) {
             const uint32_t nb = std::min(bb, prm.spp - done);

@@ -144,6 +144,13 @@ int gr_scene_get_tuning(gr_scene *scene, gr_tuning *out);
  * d_hits: scratch for n hits.  The chosen setting is returned in *out. */
 int gr_scene_autotune(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
                       gr_tuning *out);
+/* The same search on a mixed workload: each candidate is timed on n closest-hit
+ * rays plus m visibility segments (mode as gr_occluded), since one setting
+ * serves both kinds of launch.  d_visible: scratch for m results.  n or m may
+ * be 0 (not both). */
+int gr_scene_autotune_mixed(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
+                            const gr_segment *d_segs, uint32_t m, uint32_t *d_visible, int mode,
+                            gr_tuning *out);
 
 /* ---- kernel timing (observability; no reference counterpart) -------------
  * When enabled, every traversal and ray-sort launch on the context is

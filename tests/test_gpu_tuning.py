@@ -138,3 +138,21 @@ def test_autotune_picks_a_valid_setting(gpu_ctx):
     g = _closest(stage, rays)
     r = o.intersect(scenes.rays6(rays))
     assert np.array_equal(g["prim"], r["prim"])
+
+
+@pytest.mark.parametrize("with_rays", [True, False])
+def test_autotune_on_a_mixed_workload(with_rays, gpu_ctx):
+    """Closest-hit rays and visibility segments timed together (or segments
+    alone): the chosen setting is valid and visibility stays exact under it."""
+    import torch
+    print("case autotune mixed", with_rays, flush=True)
+    spec = scenes.icosphere_scene(level=5)
+    stage = Stage(spec, ctx=gpu_ctx)
+    rays = torch.from_numpy(scenes.ray_batch(spec, 1 << 17, seed=34)).cuda() if with_rays else None
+    segs = scenes.segment_batch(spec, 1 << 17, seed=35)
+    tu = stage.autotune(rays, torch.from_numpy(segs).cuda(), GR_OCCLUSION_EXACT)
+    assert tu == stage.get_tuning()
+    assert tu["sort_bits"] in (0, 12, 15, 18, 21, 24, 30) and tu["lds_stack"] in (8, 16, 32)
+    vis, _ = stage.visibility(torch.from_numpy(segs).cuda(), mode=GR_OCCLUSION_EXACT)
+    ref = OracleScene(spec).visibility(scenes.rays6(segs))["visible"]
+    assert np.array_equal(vis.cpu().numpy().astype(bool), ref.astype(bool))

@@ -313,6 +313,8 @@ def test_tuning_validation_and_rounding(host_ctx):
     with pytest.raises(GrError):
         import torch
         st.autotune(torch.zeros((4, 8)))
+    with pytest.raises(GrError):
+        st.autotune(torch.zeros((4, 8)), torch.zeros((4, 8)))
 
 
 def test_batch_rejects_host_only_context(host_ctx):
