@@ -847,9 +847,14 @@ constexpr int kTile = 16, kApron = 4, kSpan = kTile + 2 * kApron;
 // Film::report_samples as a gather (film.rs:78-114): pixel (x, y) visits the
 // samples of pixels y-4..y+4, x-4..x+4 in row-major order -- the order the
 // serial splat reaches it -- and applies each sample's own footprint test.
+// A sample's continuous position and footprint (the pixel range its splat
+// covers) are computed once, when the tile stages it in LDS, with the same
+// arithmetic the splat uses; each pixel then only compares its coordinates
+// with the staged bounds and weights the samples that cover it.
 __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters) {
-    __shared__ float s_s[kSpan * kSpan], s_t[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
+    __shared__ float s_cd[kSpan * kSpan], s_rd[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
         s_b[kSpan * kSpan];
+    __shared__ int4 s_box[kSpan * kSpan];  // min_col, max_col, min_row, max_row
     const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
     const int x = x0 + tx, y = y0 + ty;
@@ -866,8 +871,15 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
                 const size_t p = (size_t)it * npx + (size_t)sy * W + sx;
                 const float2 st = a.rb.st[p];
                 const float4 L = a.rb.L[p];
-                s_s[k] = st.x;
-                s_t[k] = st.y;
+                const float col_cont = 0.0f + (0.5f * (st.x + 1.0f)) * (wf - 0.0f);
+                const float row_cont = 0.0f + (0.5f * (st.y + 1.0f)) * (hf - 0.0f);
+                const float cd = col_cont - 0.5f, rd = row_cont - 0.5f;
+                s_cd[k] = cd;
+                s_rd[k] = rd;
+                s_box[k] = make_int4(clampi(f2i(ceilf(cd - 2.0f)), 0, last_col),
+                                     clampi(f2i(floorf(cd + 2.0f)), 0, last_col),
+                                     clampi(f2i(ceilf(rd - 2.0f)), 0, last_row),
+                                     clampi(f2i(floorf(rd + 2.0f)), 0, last_row));
                 s_r[k] = L.x;
                 s_g[k] = L.y;
                 s_b[k] = L.z;
@@ -880,15 +892,9 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
                 for (int sx = x - kApron; sx <= x + kApron; ++sx) {
                     if (sx < 0 || sx >= W) continue;
                     const int k = (sy - (y0 - kApron)) * kSpan + (sx - (x0 - kApron));
-                    const float s = s_s[k], t = s_t[k];
-                    const float col_cont = 0.0f + (0.5f * (s + 1.0f)) * (wf - 0.0f);
-                    const float row_cont = 0.0f + (0.5f * (t + 1.0f)) * (hf - 0.0f);
-                    const float cd = col_cont - 0.5f, rd = row_cont - 0.5f;
-                    const int min_col = clampi(f2i(ceilf(cd - 2.0f)), 0, last_col);
-                    const int max_col = clampi(f2i(floorf(cd + 2.0f)), 0, last_col);
-                    const int min_row = clampi(f2i(ceilf(rd - 2.0f)), 0, last_row);
-                    const int max_row = clampi(f2i(floorf(rd + 2.0f)), 0, last_row);
-                    if (x < min_col || x > max_col || y < min_row || y > max_row) continue;
+                    const int4 bx = s_box[k];
+                    if (x < bx.x || x > bx.y || y < bx.z || y > bx.w) continue;
+                    const float cd = s_cd[k], rd = s_rd[k];
                     const float wt = mitchell1(((float)x - cd) / 2.0f) * mitchell1(((float)y - rd) / 2.0f);
                     acc.x = acc.x + s_r[k] * wt;
                     acc.y = acc.y + s_g[k] * wt;
