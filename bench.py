@@ -238,16 +238,18 @@ def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
     t_plain = timed(lambda: trace(st, plain, "pt_nee", spp=spp, max_depth=8, seed=7))
     e_plain = _rel_mse_trimmed(image(plain), R)
     plain.close()
-    cfg = dict(alpha=0.3, guided_depth=2, steps_per_pass=2, batch=16384, train_paths=1 << 16, warmup_steps=8,
+    # the guide samples the first vertex only (the scene's error comes from long paths, which a guide
+    # at later vertices does not reduce); batches of 8 spp like the plain render, 8 Adam steps each
+    cfg = dict(alpha=0.3, guided_depth=1, steps_per_pass=8, batch=16384, train_paths=1 << 16, warmup_steps=8,
                learning_rate=5e-3)
     g = Guide(ctx, **cfg)
     infer = guide_inference_rate(ctx, g)
     f = Film(ctx, W, H)
     tot, curve, reached = 0.0, [], None
-    for k in range(0, 2 * spp, 4):
-        tot += timed(lambda: trace(st, f, "guided", spp=4, max_depth=8, seed=9, sample_offset=k, batch=2, guide=g))
+    for k in range(0, 2 * spp, 8):
+        tot += timed(lambda: trace(st, f, "guided", spp=8, max_depth=8, seed=9, sample_offset=k, batch=8, guide=g))
         e = _rel_mse_trimmed(image(f), R)
-        curve.append((k + 4, round(tot, 4), round(e, 6)))
+        curve.append((k + 8, round(tot, 4), round(e, 6)))
         if reached is None and e <= e_plain:
             t0, e0 = (curve[-2][1], curve[-2][2]) if len(curve) > 1 else (0.0, float("inf"))
             reached = t0 + (tot - t0) * ((e0 - e_plain) / (e0 - e) if np.isfinite(e0) and e0 > e else 1.0)
@@ -277,6 +279,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the guided comparison")
+    ap.add_argument("--guide-inference-only", action="store_true",
+                    help="for profiles: only the guide inference timing (4M queries), printed as JSON")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (autotune on the closest-hit batch before timing), 'default', or "
                          "k=v pairs, e.g. sort_bits=18,lds_stack=16,waves_per_cu=0,refill=16")
@@ -299,6 +303,12 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
+    if args.guide_inference_only:
+        from gammaray_amd.api import Guide
+        ctx = Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        print(json.dumps(guide_inference_rate(ctx, Guide(ctx))), flush=True)
+        return
     t0 = time.time()
     spec = scenes.icosphere_scene(level=args.level, seed=2)
     log("icosphere L%d: %d triangles (%.1f s)" % (args.level, spec.num_components, time.time() - t0))

@@ -1432,7 +1432,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     if (guided && !guide) return hipErrorInvalidValue;
     uint32_t batch = prm.batch;
     if (batch == 0)
-        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, (guided ? (1u << 21) : (1u << 24)) / npx));
+        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(prm.spp, (1u << 24) / npx));
     batch = std::min(batch, prm.spp);
     const size_t paths = npx * batch;
     if (paths > 0x7fffffffull) return hipErrorInvalidValue;
