@@ -212,7 +212,8 @@ def hits_to_numpy(hits):
 # Rendering: Film (src/render/film.rs) and Stage::trace (src/render/stage.rs)
 # ---------------------------------------------------------------------------
 INTEGRATORS = {"pt": _lib.GR_INTEGRATOR_PT, "pt_nee": _lib.GR_INTEGRATOR_PT_NEE,
-               "bdpt": _lib.GR_INTEGRATOR_BDPT, "guided": _lib.GR_INTEGRATOR_GUIDED}
+               "bdpt": _lib.GR_INTEGRATOR_BDPT, "guided": _lib.GR_INTEGRATOR_GUIDED,
+               "display": _lib.GR_INTEGRATOR_DISPLAY}
 
 
 def make_camera(spec):

@@ -568,10 +568,10 @@ int gr_trace(gr_scene* s, const gr_camera* cam, const gr_trace_params* prm, gr_f
     if (!s->committed) return fail(ctx, GR_ERR_STATE, "scene not committed%s");
     if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
     if (film->ctx != ctx) return fail(ctx, GR_ERR_INVALID, "film belongs to another context%s");
-    if (prm->integrator > GR_INTEGRATOR_GUIDED) return fail(ctx, GR_ERR_INVALID, "unknown integrator%s");
+    if (prm->integrator > GR_INTEGRATOR_DISPLAY) return fail(ctx, GR_ERR_INVALID, "unknown integrator%s");
     if (prm->integrator == GR_INTEGRATOR_GUIDED && (!prm->guide || prm->guide->ctx != ctx))
         return fail(ctx, GR_ERR_INVALID, "guided tracing needs a guide of the same context%s");
-    if ((prm->integrator != GR_INTEGRATOR_PT) && s->shade.num_lights == 0)
+    if (prm->integrator != GR_INTEGRATOR_PT && prm->integrator != GR_INTEGRATOR_DISPLAY && s->shade.num_lights == 0)
         return fail(ctx, GR_ERR_INVALID, "integrator samples lights but the scene has none%s");
     if (!(cam->focal_length > 0.0f)) return fail(ctx, GR_ERR_INVALID, "camera focal length must be positive%s");
     if (prm->spp == 0) return GR_OK;

@@ -62,12 +62,13 @@ GpuMaterial build_material(const float rec[GR_MAT_FLOATS]) {
     const vec3 one = mk(1.0f, 1.0f, 1.0f);
     if (kind == 1) {  // Material::diffuse_light
         m.has_light = 1;
-        m.light[0] = rec[1];
-        m.light[1] = rec[2];
-        m.light[2] = rec[3];
+        m.light[0] = m.display[0] = rec[1];
+        m.light[1] = m.display[1] = rec[2];
+        m.light[2] = m.display[2] = rec[3];
         return m;
     }
     if (kind == 2 || kind == 3) {  // Material::diffuse / mirror
+        m.display[0] = m.display[1] = m.display[2] = 1.0f;
         m.nlobes = 1;
         m.roughness = kind == 2 ? 1.0f : 0.0f;
         m.lobes[0].type = kind == 2 ? kLobePerfectDiffuse : kLobePerfectMirror;
@@ -75,6 +76,7 @@ GpuMaterial build_material(const float rec[GR_MAT_FLOATS]) {
     }
     // DisneyMaterialBuilder::build, src/material/material.rs:292-333
     const vec3 base = mk(rec[1], rec[2], rec[3]);
+    set3(m.display, base);  // display: the builder's base colour (material.rs:329)
     const float roughness = rec[4], anisotropic = rec[5], ior = rec[6], metallic = rec[7];
     const float specular_trans = rec[8], specular_tint = rec[9], sheen = rec[10], sheen_tint = rec[11];
     const float clearcoat = rec[12], clearcoat_gloss = rec[13];
@@ -530,6 +532,21 @@ __global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
 }
 
 __global__ void k_reset(uint32_t* c, uint32_t which) { c[which] = 0u; }
+
+// DisplayColorIntegrator::integrate (src/render/integrators.rs:21-34): the
+// display colour of the camera ray's closest hit, black on a miss.
+__global__ __launch_bounds__(256) void k_display(RenderArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npaths) return;
+    const uint32_t p = a.rb.queue[0][i];
+    const uint32_t prim = a.rb.hits[i].x;
+    float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (prim != kNone) {
+        const float* c = a.sh.mats[prim].display;
+        L = make_float4(c[0], c[1], c[2], 0.0f);
+    }
+    a.rb.L[p] = L;
+}
 
 // One bounce of PathTracerIntegrator::integrate (src/render/integrators.rs:42-99);
 // with NEE, light sampling and power-heuristic MIS.
@@ -1510,7 +1527,14 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         }
         hipLaunchKernelGGL(k_camera, blocks_for(a.npaths), dim3(256), 0, s, a);
         uint32_t cur = 0;
-        for (uint32_t depth = 0; prm.max_depth == 0 || depth < prm.max_depth; ++depth) {
+        if (prm.integrator == GR_INTEGRATOR_DISPLAY) {
+            e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[0]), a.npaths,
+                                 nullptr, reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(k_display, blocks_for(a.npaths), dim3(256), 0, s, a);
+        }
+        for (uint32_t depth = 0; prm.integrator != GR_INTEGRATOR_DISPLAY && (prm.max_depth == 0 || depth < prm.max_depth);
+             ++depth) {
             if (prm.max_depth == 0) {
                 // Unbounded paths (the reference's loop): stop once no path is live.
                 uint32_t live = 0;

@@ -48,9 +48,10 @@ struct GpuMaterial {
     float roughness;  // the builder's roughness (guide input): Disney roughness, 1 diffuse, 0 mirror
     uint32_t pad1;
     float light[4];
+    float display[4];  // Material::display_color (material.rs:61-63): incandescence, one, or base colour
     GpuLobe lobes[4];
 };
-static_assert(sizeof(GpuMaterial) == 288, "material layout");
+static_assert(sizeof(GpuMaterial) == 304, "material layout");
 
 // Build the lobe table the reference's constructors and Disney builder make
 // (src/material/material.rs:27-53,292-333).  Host side.

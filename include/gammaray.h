@@ -177,6 +177,8 @@ typedef struct {
 #define GR_INTEGRATOR_PT 0      /* PathTracerIntegrator, src/render/integrators.rs:39-100 */
 #define GR_INTEGRATOR_PT_NEE 1  /* path tracing + next-event estimation with MIS (no reference counterpart) */
 #define GR_INTEGRATOR_BDPT 2    /* BdptIntegrator, src/render/integrators.rs:102-418 */
+#define GR_INTEGRATOR_DISPLAY 4 /* DisplayColorIntegrator, src/render/integrators.rs:18-34: the display
+                                   colour of the first hit's material, black on a miss */
 #define GR_INTEGRATOR_GUIDED 3  /* PT_NEE whose BSDF sampling is mixed (one-sample MIS) with a
                                    learned guide trained online (no reference counterpart) */
 

@@ -186,7 +186,7 @@ class OracleScene:
 
 
 # ---- rendering -------------------------------------------------------------
-PT, PT_NEE, BDPT = 0, 1, 2
+PT, PT_NEE, BDPT, DISPLAY = 0, 1, 2, 4
 
 
 def camera_matrix(cam):

@@ -12,6 +12,7 @@
  *               pdf-consistent materials and by analytic scenes).
  *   ORC_BDPT    BdptIntegrator (integrators.rs:102-418) with caps on camera
  *               vertices (max_depth) and light vertices (1 + light_depth).
+ *   ORC_DISPLAY DisplayColorIntegrator (integrators.rs:18-34).
  */
 #include "gr_oracle.h"
 #include "orc_render.h"
@@ -238,6 +239,15 @@ static float power_heuristic(float f_pdf, float g_pdf)
 {
     float f = 1.0f * f_pdf, g = 1.0f * g_pdf;
     return (f * f) / (f * f + g * g);
+}
+
+/* ---- DisplayColorIntegrator::integrate, integrators.rs:21-34 --------------- */
+
+static ovec display_integrate(const orc_scene *s, ovec o, ovec d)
+{
+    orc_hit h;
+    if (orc_intersect_one(s, o, d, &h, NULL) != 0 || h.prim == ORC_NONE) return v_zero();
+    return s->prims[h.prim].mat.display;
 }
 
 /* ---- path tracer, integrators.rs:42-99 ------------------------------------ */
@@ -510,6 +520,7 @@ int orc_render(const orc_scene *s, const float cam[16], float wx, float wy, uint
             ovec L;
             if (integrator == 0) L = pt_integrate(s, o, d, seed, path, max_depth);
             else if (integrator == 1) L = nee_integrate(s, o, d, seed, path, max_depth);
+            else if (integrator == ORC_DISPLAY) L = display_integrate(s, o, d);
             else L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth);
             col[3 * i] = L.x; col[3 * i + 1] = L.y; col[3 * i + 2] = L.z;
         }

@@ -289,3 +289,38 @@ def test_bdpt_is_unbiased_against_unbounded_pt():
     la, lb = a.sum(2).ravel(), b.sum(2).ravel()
     z = abs(la.mean() - lb.mean()) / math.sqrt(la.var() / la.size + lb.var() / lb.size)
     assert z < 4.0, (la.mean(), lb.mean(), z)
+
+
+@pytest.mark.parametrize("mat,want", [
+    (scenes.disney(base_color=(0.2, 0.5, 0.7)), (0.2, 0.5, 0.7)),   # Disney: the base colour
+    (scenes.diffuse(), (1.0, 1.0, 1.0)),                            # Material::diffuse: one
+    (scenes.mirror(), (1.0, 1.0, 1.0)),                             # Material::mirror: one
+    (scenes.diffuse_light((3.0, 2.0, 0.5)), (3.0, 2.0, 0.5)),       # diffuse_light: incandescence
+])
+def test_display_integrator_inside_a_sphere(mat, want):
+    """DisplayColorIntegrator (integrators.rs:18-34): every camera ray inside a
+    sphere hits it, so every sample -- and the normalised film -- is the
+    sphere's display colour."""
+    s = scenes.SceneSpec(name="inside")
+    s.add_sphere((0.0, 0.0, 0.0), 50.0, mat)
+    s.camera = scenes.CameraSpec(focal_length=0.8, horizontal_aperture=1.0, vertical_aperture=1.0)
+    o = O.OracleScene(s)
+    film, cols = O.render(o, s, 24, 16, O.DISPLAY, spp=2, seed=3, colors=True)
+    assert np.array_equal(cols, np.broadcast_to(np.float32(want), cols.shape))
+    img = film[:, :3] / film[:, 3:4]
+    np.testing.assert_allclose(img, np.broadcast_to(want, img.shape), rtol=1e-5)
+
+
+def test_display_integrator_is_black_on_a_miss_and_follows_the_hit():
+    """Samples whose ray misses everything are black; the others carry the
+    display colour of the primitive the oracle's closest-hit query reports."""
+    spec = scenes.box_scene()
+    s = scenes.SceneSpec(name="one_sphere")
+    s.add_sphere((0.0, 1.0, -2.0), 0.5, scenes.disney(base_color=(0.9, 0.1, 0.3)))
+    s.camera = spec.camera
+    o = O.OracleScene(s)
+    _, cols = O.render(o, s, 32, 32, O.DISPLAY, spp=1, seed=5, colors=True)
+    c = cols[0]
+    hit = np.all(c == np.float32([0.9, 0.1, 0.3]), axis=1)
+    miss = np.all(c == 0.0, axis=1)
+    assert np.all(hit | miss) and 0.02 < hit.mean() < 0.9
