@@ -88,6 +88,11 @@ SIGNATURES = {
     "gr_scene_destroy": (None, [_vp]),
     "gr_scene_add_mesh": (_i, [_vp, _f32p, _u32, _f32p, _u32, _f32p, _u32, _u32p, _u32, _f32p, _u32p]),
     "gr_scene_add_sphere": (_i, [_vp, _f32p, ctypes.c_float, _f32p, _u32p]),
+    "gr_obj_load": (_i, [ctypes.c_char_p, _f32p, _vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "gr_obj_sizes": (_i, [_vp, _u32p, _u32p, _u32p, _u32p]),
+    "gr_obj_read": (_i, [_vp, _f32p, _f32p, _f32p, _u32p]),
+    "gr_obj_destroy": (None, [_vp]),
+    "gr_scene_add_obj": (_i, [_vp, _vp, _f32p, _u32p]),
     "gr_scene_commit": (_i, [_vp]),
     "gr_scene_info_get": (_i, [_vp, ctypes.POINTER(SceneInfo)]),
     "gr_scene_export_bvh": (_i, [_vp, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p]),

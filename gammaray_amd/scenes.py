@@ -136,6 +136,15 @@ class SceneSpec:
                                    uvs=None if uvs is None else np.ascontiguousarray(uvs, np.float32)))
         return len(self.prims) - 1
 
+    def add_obj(self, path, material, xform=None):
+        """Mesh::from_obj(material, xform, path) (src/geom/mesh.rs:61-188): the
+        file parsed by the library's OBJ loader; xform is a 4x4 row-major Mat."""
+        m = load_obj(path, xform)
+        self.prims.append(PrimSpec("mesh", np.asarray(material, np.float32), vertices=m["vertices"],
+                                   tris=m["tris"], normals=m["normals"] if len(m["normals"]) else None,
+                                   uvs=m["uvs"] if len(m["uvs"]) else None))
+        return len(self.prims) - 1
+
     def add_sphere(self, center, radius, material):
         self.prims.append(PrimSpec("sphere", np.asarray(material, np.float32),
                                    center=np.asarray(center, np.float32), radius=float(np.float32(radius))))
@@ -156,6 +165,34 @@ class SceneSpec:
                 lo = np.minimum(lo, p.center - p.radius)
                 hi = np.maximum(hi, p.center + p.radius)
         return lo, hi
+
+
+def load_obj(path, xform=None):
+    """An OBJ file as the arrays gr_scene_add_mesh takes: vertices (V,3),
+    normals (N,3), uvs (U,2) float32 and tris (T,9) uint32 (GR_NONE = geometric
+    normal / zero uv), through the library's loader (host only, no GPU)."""
+    import ctypes
+    from ._lib import GrError, lib
+    L = lib()
+    xf = None if xform is None else np.ascontiguousarray(xform, np.float32).reshape(16)
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    code = L.gr_obj_load(str(path).encode(), None if xf is None else xf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                         ctypes.byref(h), err, len(err))
+    if code != 0:
+        raise GrError("gr_obj_load failed: " + err.value.decode(errors="replace"))
+    try:
+        n = [ctypes.c_uint32() for _ in range(4)]
+        L.gr_obj_sizes(h, *[ctypes.byref(x) for x in n])
+        nv, nn, nt, ntri = (x.value for x in n)
+        out = dict(vertices=np.zeros((nv, 3), np.float32), normals=np.zeros((nn, 3), np.float32),
+                   uvs=np.zeros((nt, 2), np.float32), tris=np.zeros((ntri, 9), np.uint32))
+        f32 = ctypes.POINTER(ctypes.c_float)
+        L.gr_obj_read(h, out["vertices"].ctypes.data_as(f32), out["normals"].ctypes.data_as(f32),
+                      out["uvs"].ctypes.data_as(f32), out["tris"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    finally:
+        L.gr_obj_destroy(h)
+    return out
 
 
 def quad(center, u, v, normal):

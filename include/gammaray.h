@@ -85,6 +85,23 @@ int gr_scene_add_mesh(gr_scene *scene, const float *verts, uint32_t nverts,
 /* Sphere::new(material, translation, radius), src/geom/sphere.rs:17-24. */
 int gr_scene_add_sphere(gr_scene *scene, const float center[3], float radius,
                         const float *material, uint32_t *prim_out);
+/* Wavefront OBJ, parsed on the host into the state Mesh::from_obj builds
+ * (src/geom/mesh.rs:61-188): vertices through Xform::transform, normals
+ * through Xform::transform_normal (Mat::inverted) and normalised unless nearly
+ * zero, n-gons as fans (0, k, k+1).  xf: 16 floats, the row-major Mat
+ * (xf[4 r + c] = mat[r][c]); NULL = identity.  On failure err (if given)
+ * receives the message, e.g. "OBJ parse error at line N: ...". */
+typedef struct gr_obj gr_obj;
+int gr_obj_load(const char *path, const float *xf, gr_obj **out, char *err, size_t err_len);
+int gr_obj_sizes(const gr_obj *obj, uint32_t *nverts, uint32_t *nnormals, uint32_t *nuvs,
+                 uint32_t *ntris);
+/* verts: 3 floats each, normals 3, uvs 2, tris 9 uint32 each (a,b,c vertex;
+ * an,bn,cn normal or GR_NONE = geometric normal; at,bt,ct uv or GR_NONE = zero
+ * uv), the layout gr_scene_add_mesh takes.  Any output may be NULL. */
+int gr_obj_read(const gr_obj *obj, float *verts, float *normals, float *uvs, uint32_t *tris);
+void gr_obj_destroy(gr_obj *obj);
+/* gr_scene_add_mesh with the loaded arrays (Mesh::from_obj + Stage::new). */
+int gr_scene_add_obj(gr_scene *scene, const gr_obj *obj, const float *material, uint32_t *prim_out);
 /* Bvh::build (src/geom/bvh.rs:339-382) on the host -- the reference's SAH
  * builder, so the tree is identical -- then upload of the GPU layout. */
 int gr_scene_commit(gr_scene *scene);

@@ -62,8 +62,24 @@ def _mixed():
     return s
 
 
+def _obj_scene():
+    """The reference's cone.obj and sphere.obj through the OBJ loader, placed
+    with translations and a scale, next to analytic spheres."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "obj")
+    s = scenes.SceneSpec(name="obj")
+    for k, name in enumerate(("cone.obj", "sphere.obj", "cone.obj")):
+        xf = np.eye(4, dtype=np.float32) * np.float32(0.25 + 0.1 * k)
+        xf[3] = (-1.5 + 1.5 * k, 0.3 * k, -0.5 * k, 1.0)
+        s.add_obj(os.path.join(here, name), scenes.disney(), xf)
+    for c in ((0.0, 1.5, 0.0), (1.0, -0.5, 0.5)):
+        s.add_sphere(c, 0.4, scenes.diffuse())
+    return s
+
+
 SCENES = {
     "box": lambda: scenes.box_scene(),
+    "obj": _obj_scene,
     "ico6": lambda: scenes.icosphere_scene(level=6),
     "showcase": lambda: scenes.showcase_scene(),
     "spheres": lambda: _spheres(500, 4),
