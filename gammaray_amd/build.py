@@ -58,7 +58,26 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+CLI_SRC = os.path.join(CSRC, "gr_cli.cpp")
+CLI = os.path.join(HERE, "gammaray")
+
+
+def build_cli():
+    """The progressive render driver (the reference binary's loop), linked
+    against the library next to it."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = CLI + ".tmp"
+    cmd = [gxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-o", tmp, CLI_SRC, "-L" + HERE, "-lgammaray_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cli build failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

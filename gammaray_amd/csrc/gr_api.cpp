@@ -535,6 +535,12 @@ int gr_film_create(gr_ctx* ctx, uint32_t width, uint32_t height, gr_film** out) 
 void gr_film_destroy(gr_film* f) {
     if (!f) return;
     (void)hipStreamSynchronize(f->ctx->stream);
+    if (f->copy_stream) (void)hipStreamSynchronize(f->copy_stream);
+    (void)hipFree(f->d_snapshot);
+    (void)hipHostFree(f->h_pinned);
+    if (f->snap_ready) (void)hipEventDestroy(f->snap_ready);
+    if (f->copy_done) (void)hipEventDestroy(f->copy_done);
+    if (f->copy_stream) (void)hipStreamDestroy(f->copy_stream);
     (void)hipFree(f->d_pixels);
     release_render_buffers(f->rb);
     delete f;
@@ -552,6 +558,47 @@ int gr_film_readback(gr_film* f, float* rgbw) {
                                   hipMemcpyDeviceToHost, f->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
     return e == hipSuccess ? GR_OK : hip_fail(f->ctx, e, "film readback");
+}
+
+int gr_film_readback_begin(gr_film* f) {
+    if (!f) return GR_ERR_INVALID;
+    if (f->copy_pending) return fail(f->ctx, GR_ERR_STATE, "a readback is already in flight%s");
+    const size_t bytes = (size_t)f->width * f->height * sizeof(float4);
+    hipError_t e = hipSuccess;
+    if (!f->copy_stream) {
+        e = hipSetDevice(f->ctx->device);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&f->d_snapshot), bytes);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&f->h_pinned), bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f->snap_ready, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&f->copy_done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&f->copy_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(f->ctx, e, "readback buffers");
+    }
+    // the snapshot is ordered after the work queued so far; the host copy runs
+    // on the side stream while the context's stream goes on rendering
+    e = hipMemcpyAsync(f->d_snapshot, f->d_pixels, bytes, hipMemcpyDeviceToDevice, f->ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(f->snap_ready, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(f->copy_stream, f->snap_ready, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(f->h_pinned, f->d_snapshot, bytes, hipMemcpyDeviceToHost, f->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(f->copy_done, f->copy_stream);
+    if (e != hipSuccess) return hip_fail(f->ctx, e, "readback begin");
+    f->copy_pending = true;
+    return GR_OK;
+}
+
+int gr_film_readback_end(gr_film* f, float* rgbw) {
+    if (!f || !rgbw) return GR_ERR_INVALID;
+    if (!f->copy_pending) return fail(f->ctx, GR_ERR_STATE, "no readback in flight%s");
+    f->copy_pending = false;
+    const hipError_t e = hipEventSynchronize(f->copy_done);
+    if (e != hipSuccess) return hip_fail(f->ctx, e, "readback end");
+    std::memcpy(rgbw, f->h_pinned, (size_t)f->width * f->height * sizeof(float4));
+    return GR_OK;
+}
+
+int gr_exr_write(const float* rgbw, uint32_t width, uint32_t height, const char* path) {
+    if (!rgbw || !path || width == 0 || height == 0) return GR_ERR_INVALID;
+    return write_exr(rgbw, width, height, path) ? GR_OK : GR_ERR_INVALID;
 }
 
 int gr_film_upload(gr_film* f, const float* rgbw) {

@@ -289,6 +289,12 @@ struct gr_film {
     uint32_t width = 0, height = 0;
     float4* d_pixels = nullptr;  // (accum r, g, b, weight)
     RenderBuffers rb;
+    // asynchronous readback: a device snapshot, pinned host copy, a side stream
+    float4* d_snapshot = nullptr;
+    float4* h_pinned = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, copy_done = nullptr;
+    bool copy_pending = false;
 };
 
 namespace gr {

@@ -262,6 +262,34 @@ def box_scene(seed=1, consistent=False, full=False):
     return s
 
 
+def demo_scene(obj_path=None):
+    """The reference binary's scene (src/bin/gammaray.rs:8-81), as the
+    `gammaray` driver builds it: the OBJ mesh (the dragon there; skipped when
+    obj_path is None) scaled by 0.9 and moved to (-4, -5, -100), a light
+    sphere, a large metal sphere, a glass sphere and two rows of 20 spheres,
+    seen by Camera::default."""
+    s = SceneSpec(name="demo")
+    if obj_path is not None:
+        xf = np.eye(4, dtype=np.float32) * np.float32(0.9)
+        xf[3] = (-4.0, -5.0, -100.0, 1.0)
+        s.add_obj(obj_path, disney(base_color=(0.0, 0.5, 1.0), roughness=0.5, metallic=1.0, anisotropic=1.0,
+                                   sheen=1.0), xf)
+    s.add_sphere((12.0, 3.0, -90.0), 5.0, diffuse_light((2.0, 2.0, 2.0)))
+    s.add_sphere((-25.0, 0.0, -50.0), 75.0, disney(base_color=(0.5, 0.9, 0.0), roughness=0.5, metallic=1.0))
+    s.add_sphere((6.0, -10.0, -90.0), 4.0, disney(specular_trans=1.0, roughness=0.2, ior=1.8, metallic=0.0))
+    for row in range(2):
+        for i in range(20):
+            color = (0.9 if i % 2 == 0 else 0.2, 0.9 if i % 3 == 0 else 0.2, 0.9 if i % 5 == 0 else 0.2)
+            if row == 0:
+                m = disney(base_color=color, roughness=0.5, metallic=1.0, anisotropic=0.8, clearcoat=1.0,
+                           clearcoat_gloss=1.0)
+            else:
+                m = disney(base_color=color, roughness=0.2, ior=1.8, metallic=0.0, sheen=1.0)
+            s.add_sphere((-20.0 + i * 4, 12.0 if row == 0 else -12.0, -100.0), 1.8, m)
+    s.camera = CameraSpec()
+    return s
+
+
 def indirect_room_scene(seed=5):
     """A room lit only indirectly: the area light under the ceiling emits
     upwards, so every surface below it is lit by the bright patch the light

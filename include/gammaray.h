@@ -220,6 +220,15 @@ void gr_film_destroy(gr_film *film);
 int gr_film_clear(gr_film *film);
 /* Host copy of the film: width*height*4 floats (r, g, b, weight), row 0 first. */
 int gr_film_readback(gr_film *film, float *rgbw);
+/* Overlapped readback (the progressive loop of src/bin/gammaray.rs:85-112):
+ * begin snapshots the film as it will be after the work queued so far and
+ * starts its copy to host memory on a side stream, so later gr_trace calls
+ * render meanwhile; end waits for that copy and writes it to rgbw
+ * (width*height*4 floats).  One readback in flight per film. */
+int gr_film_readback_begin(gr_film *film);
+int gr_film_readback_end(gr_film *film, float *rgbw);
+/* ExrWriter::update + write from a host film copy (no GPU needed). */
+int gr_exr_write(const float *rgbw, uint32_t width, uint32_t height, const char *path);
 /* Restores a film from a host copy (checkpoint / resume). */
 int gr_film_upload(gr_film *film, const float *rgbw);
 /* spp iterations of Stage::trace with the given integrator; samples splat into
