@@ -88,6 +88,9 @@ SIGNATURES = {
     "gr_scene_destroy": (None, [_vp]),
     "gr_scene_add_mesh": (_i, [_vp, _f32p, _u32, _f32p, _u32, _f32p, _u32, _u32p, _u32, _f32p, _u32p]),
     "gr_scene_add_sphere": (_i, [_vp, _f32p, ctypes.c_float, _f32p, _u32p]),
+    "gr_film_readback_begin": (_i, [_vp]),
+    "gr_film_readback_end": (_i, [_vp, _f32p]),
+    "gr_exr_write": (_i, [_f32p, _u32, _u32, ctypes.c_char_p]),
     "gr_obj_load": (_i, [ctypes.c_char_p, _f32p, _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "gr_obj_sizes": (_i, [_vp, _u32p, _u32p, _u32p, _u32p]),
     "gr_obj_read": (_i, [_vp, _f32p, _f32p, _f32p, _u32p]),

@@ -252,6 +252,16 @@ class Film:
         check(lib().gr_film_readback(self._h, _p(out, _f32p)), self.ctx.handle, "film_readback")
         return out
 
+    def readback_begin(self):
+        """Starts a host copy of the film as it will be after the work queued so
+        far; later traces render meanwhile.  readback_end() returns it."""
+        check(lib().gr_film_readback_begin(self._h), self.ctx.handle, "film_readback_begin")
+
+    def readback_end(self):
+        out = np.zeros((self.height * self.width, 4), np.float32)
+        check(lib().gr_film_readback_end(self._h, _p(out, _f32p)), self.ctx.handle, "film_readback_end")
+        return out
+
     def upload(self, rgbw):
         a = np.ascontiguousarray(rgbw, np.float32)
         assert a.size == self.width * self.height * 4

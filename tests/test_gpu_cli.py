@@ -45,3 +45,22 @@ def test_driver_rejects_a_missing_required_mesh(tmp_path):
     r = subprocess.run([CLI, "--iterations", "1", "--height", "8", "--obj", str(tmp_path / "none.obj"),
                         "--output", str(tmp_path / "o.exr")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "Couldn't open OBJ file" in r.stderr
+
+
+def test_overlapped_readback_is_the_film_at_its_point_in_the_queue(gpu_ctx):
+    """readback_begin after pass k, then pass k+1 queued: readback_end returns
+    the film after pass k, not after k+1."""
+    print("case readback overlap", flush=True)
+    spec = scenes.box_scene()
+    stage = Stage(spec, ctx=gpu_ctx)
+    f = Film(gpu_ctx, 48, 40)
+    trace(stage, f, "pt", spp=2, max_depth=4, seed=3)
+    want = f.readback()
+    f.readback_begin()
+    trace(stage, f, "pt", spp=2, max_depth=4, seed=3, sample_offset=2)
+    got = f.readback_end()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert not np.array_equal(f.readback(), want)
+    with pytest.raises(Exception):
+        f.readback_end()  # nothing in flight
+
