@@ -1628,12 +1628,16 @@ struct BatchDev {
     const JobDev* jobs;
     uint32_t njobs, pool;
     unsigned long long total;   // samples of all jobs
-    uint32_t *slot_job, *slot_sample, *slot_depth;
-    float2* slot_st;
+    uint4* slot;         // per slot: job, sample within the job, depth, unused (one 16-B record)
+    float2* slot_st;     // per slot: film position (read when the path retires)
+    const unsigned long long* job_first;  // per job: index of its first sample (a compact search table)
     uint32_t* free_list;
     uint32_t* retire;
-    // [0] free slots, [1] paths started this step, [2] first free-list entry they take,
-    // [3] their first queue position, [4] paths retired this step
+    // [0] free slots (the free list is a stack: entries [0, ctr[0]) are free),
+    // [1] paths started this step, [2] first free-list entry they take,
+    // [3] their first queue position, [4] paths retired this step (their
+    // slots go onto the stack at ctr[0] + their retire index; the next plan
+    // adds them to the count)
     uint32_t* ctr;
     unsigned long long* issued;  // [0] samples started so far, [1] first sample of this step
     uint32_t* done;              // per job: kDoneSpread counters of finished samples (summed by the host)
@@ -1656,7 +1660,7 @@ __global__ void k_batch_init(RenderArgs a, BatchDev b) {
 // they go; clears the step's other queues.
 __global__ void k_batch_plan(RenderArgs a, BatchDev b, uint32_t cur) {
     if (threadIdx.x != 0) return;
-    const uint32_t freec = b.ctr[0];
+    const uint32_t freec = b.ctr[0] + b.ctr[4];
     const unsigned long long issued = b.issued[0];
     const unsigned long long rem = b.total - issued;
     const uint32_t k = rem < (unsigned long long)freec ? (uint32_t)rem : freec;
@@ -1682,7 +1686,7 @@ __global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, u
     uint32_t lo = 0, hi = b.njobs - 1;  // last job whose first sample <= sidx
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) / 2;
-        if (b.jobs[mid].first <= sidx) lo = mid;
+        if (b.job_first[mid] <= sidx) lo = mid;
         else hi = mid - 1;
     }
     const JobDev& J = b.jobs[lo];
@@ -1697,9 +1701,7 @@ __global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, u
     a.rb.T[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     a.rb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     a.rb.aux[slot] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-    b.slot_job[slot] = lo;
-    b.slot_sample[slot] = local;
-    b.slot_depth[slot] = 0u;
+    b.slot[slot] = make_uint4(lo, local, 0u, 0u);
     b.slot_st[slot] = c.st;
 }
 
@@ -1720,11 +1722,11 @@ __global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, u
         const float4 ro = a.rb.rays[cur][2 * (size_t)i];
         const vec3 o = ld3(ro), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
         scene = __float_as_uint(ro.w);
-        const JobDev& J = b.jobs[b.slot_job[slot]];
-        const uint32_t depth = b.slot_depth[slot];
-        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + b.slot_sample[slot], slot, depth,
-                               h, o, d, out);
-        if (out.alive) b.slot_depth[slot] = depth + 1u;
+        const uint4 st = b.slot[slot];
+        const JobDev& J = b.jobs[st.x];
+        const uint32_t depth = st.z;
+        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + st.y, slot, depth, h, o, d, out);
+        if (out.alive) b.slot[slot].z = depth + 1u;
         retire = !out.alive;
     }
     const uint32_t nxt = cur ^ 1u;
@@ -1755,9 +1757,10 @@ __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) 
     uint32_t slot = 0, j = kNone;
     if (ok) {
         slot = b.retire[i];
-        j = b.slot_job[slot];
+        const uint4 st = b.slot[slot];
+        j = st.x;
         const JobDev& J = b.jobs[j];
-        const uint32_t local = b.slot_sample[slot];
+        const uint32_t local = st.y;
         J.L[local] = a.rb.L[slot];
         J.st[local] = b.slot_st[slot];
     }
@@ -1772,8 +1775,7 @@ __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) 
             atomicAdd(&b.done[lj * kDoneSpread + blockIdx.x % kDoneSpread], (uint32_t)__popcll(same));
         left &= ~same;
     }
-    const uint32_t k = wave_append(ok, &b.ctr[0]);
-    if (ok) b.free_list[k] = slot;
+    if (ok) b.free_list[b.ctr[0] + i] = slot;  // no atomic: the retire index is the stack position
 }
 
 }  // namespace
@@ -1837,10 +1839,9 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     const size_t o_scenes = off; off = align(off + scenes.size() * sizeof(SceneRef));
     const size_t o_L = off; off = align(off + total * sizeof(float4));
     const size_t o_st = off; off = align(off + total * sizeof(float2));
-    const size_t o_sjob = off; off = align(off + (size_t)pool * 4);
-    const size_t o_ssmp = off; off = align(off + (size_t)pool * 4);
-    const size_t o_sdep = off; off = align(off + (size_t)pool * 4);
+    const size_t o_slot = off; off = align(off + (size_t)pool * 16);
     const size_t o_sst = off; off = align(off + (size_t)pool * 8);
+    const size_t o_first = off; off = align(off + (size_t)njobs * 8);
     const size_t o_free = off; off = align(off + (size_t)pool * 4);
     const size_t o_ret = off; off = align(off + (size_t)pool * 4);
     const size_t o_ctr = off; off = align(off + 64);
@@ -1866,6 +1867,8 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) e = hipEventCreate(&ev_job[j]);
     std::vector<float4*> job_L(njobs);
     std::vector<float2*> job_st(njobs);
+    std::vector<unsigned long long> job_first(njobs);
+    for (uint32_t j = 0; j < njobs; ++j) job_first[j] = hj[j].first;
     if (e == hipSuccess) {
         for (uint32_t j = 0; j < njobs; ++j) {
             hj[j].L = reinterpret_cast<float4*>(blob + o_L) + hj[j].first;
@@ -1880,6 +1883,8 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             e = hipMemcpyAsync(blob + o_jobs, hj.data(), njobs * sizeof(JobDev), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
             e = hipMemcpyAsync(blob + o_scenes, hs.data(), hs.size() * sizeof(SceneRef), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(blob + o_first, job_first.data(), njobs * 8, hipMemcpyHostToDevice, s);
     }
     RenderArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1889,10 +1894,9 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     b.njobs = njobs;
     b.pool = pool;
     b.total = total;
-    b.slot_job = reinterpret_cast<uint32_t*>(blob + o_sjob);
-    b.slot_sample = reinterpret_cast<uint32_t*>(blob + o_ssmp);
-    b.slot_depth = reinterpret_cast<uint32_t*>(blob + o_sdep);
+    b.slot = reinterpret_cast<uint4*>(blob + o_slot);
     b.slot_st = reinterpret_cast<float2*>(blob + o_sst);
+    b.job_first = reinterpret_cast<const unsigned long long*>(blob + o_first);
     b.free_list = reinterpret_cast<uint32_t*>(blob + o_free);
     b.retire = reinterpret_cast<uint32_t*>(blob + o_ret);
     b.ctr = reinterpret_cast<uint32_t*>(blob + o_ctr);
