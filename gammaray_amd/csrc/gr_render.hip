@@ -402,30 +402,59 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Reserves n consecutive entries per lane with one atomic per wave; returns
-// the lane's first entry.  Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t* counter) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t incl = n;
+// Reserves n consecutive entries per thread with one atomic per block;
+// returns the thread's first entry.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t* counter) {
+    __shared__ uint32_t s_part[17];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = (blockDim.x + 63u) >> 6;
+    uint32_t incl = n;  // inclusive prefix sum over the wave
 #pragma unroll
     for (uint32_t d = 1; d < 64; d <<= 1) {
         const uint32_t v = __shfl_up(incl, d);
         if (lane >= d) incl += v;
     }
-    uint32_t base = 0;
-    if (lane == 63u && incl) base = atomicAdd(counter, incl);
-    base = __shfl(base, 63);
-    return base + incl - n;
+    if (lane == 63u) s_part[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t c = s_part[w];
+            s_part[w] = run;
+            run += c;
+        }
+        s_part[16] = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t first = s_part[16] + s_part[wave] + incl - n;
+    __syncthreads();  // s_part is reused by the next call
+    return first;
 }
 
-// Appends `alive` lanes to a queue with one atomic per wave; returns the slot.
-__device__ __forceinline__ uint32_t wave_append(bool alive, uint32_t* counter) {
+// Appends the block's `alive` threads to a queue with one atomic per block
+// (up to 16 waves): atomics on one address serialise in L2, so a queue fed
+// by every block is cheaper with a quarter of the atomics of per-wave appends.
+// Returns each alive thread's slot.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_append(bool alive, uint32_t* counter) {
+    __shared__ uint32_t s_part[17];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = (blockDim.x + 63u) >> 6;
     const uint64_t m = __ballot(alive);
-    uint32_t base = 0;
-    if ((threadIdx.x & 63u) == 0 && m) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
-    return base + (uint32_t)__popcll(m & lanes_below());
+    if (lane == 0) s_part[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t c = s_part[w];
+            s_part[w] = run;
+            run += c;
+        }
+        s_part[16] = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_part[16] + s_part[wave] + (uint32_t)__popcll(m & lanes_below());
+    __syncthreads();  // s_part is reused by the next call
+    return slot;
 }
+
 
 // State of a guided trace (GR_INTEGRATOR_GUIDED).
 constexpr uint32_t kGuideNoQuery = 0xfffffffeu;  // eligible vertex, guide not sampled yet
@@ -718,14 +747,14 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
         shade_path<NEE, GUIDED>(a, a.sv, a.sh, a.seed, a.max_depth, a.path_base + p, p, depth, h, o, d, out);
     }
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t slot = wave_append(out.alive, &a.rb.counts[nxt]);
+    const uint32_t slot = block_append(out.alive, &a.rb.counts[nxt]);
     if (out.alive) {
         a.rb.queue[nxt][slot] = p;
         a.rb.rays[nxt][2 * (size_t)slot] = make_float4(out.new_o.x, out.new_o.y, out.new_o.z, 0.0f);
         a.rb.rays[nxt][2 * (size_t)slot + 1] = make_float4(out.new_d.x, out.new_d.y, out.new_d.z, 0.0f);
     }
     if (NEE) {
-        const uint32_t ss = wave_append(out.shadow, &a.rb.counts[2]);
+        const uint32_t ss = block_append(out.shadow, &a.rb.counts[2]);
         if (out.shadow) {
             a.rb.seg[2 * (size_t)ss] = make_float4(out.hp.x, out.hp.y, out.hp.z, 0.0f);
             a.rb.seg[2 * (size_t)ss + 1] = make_float4(out.seg_to.x, out.seg_to.y, out.seg_to.z, 0.0f);
@@ -785,7 +814,7 @@ __global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t dept
         }
     }
     const bool query = eligible && a.gd.active;
-    const uint32_t qi = wave_append(query, &a.rb.counts[3]);
+    const uint32_t qi = block_append(query, &a.rb.counts[3]);
     if (i < n) a.gd.gslot[p] = query ? qi : (eligible ? kGuideNoQuery : kNone);
     if (query) {
         float4* r = reinterpret_cast<float4*>(a.gd.raw + (size_t)qi * 8);
@@ -826,7 +855,7 @@ __global__ __launch_bounds__(256) void k_guide_finalize(RenderArgs a, uint32_t n
             ok = (v == v) && v < 3.0e38f;
         }
     }
-    const uint32_t k = wave_append(ok, a.gd.rec_count);
+    const uint32_t k = block_append(ok, a.gd.rec_count);
     if (ok && k < a.gd.rec_cap) {
         float* o = a.gd.recs + (size_t)k * kRecordFloats;
         o[0] = r0.x, o[1] = r0.y, o[2] = r0.z, o[3] = r0.w;
@@ -1081,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
         else b.nl[p] = k + 1;
     }
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t slot = wave_append(alive, &a.rb.counts[nxt]);
+    const uint32_t slot = block_append(alive, &a.rb.counts[nxt]);
     if (alive) {
         a.rb.queue[nxt][slot] = p;
         a.rb.rays[nxt][2 * (size_t)slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
@@ -1121,7 +1150,7 @@ __global__ __launch_bounds__(256) void k_light_start(RenderArgs a, BdptArgs b) {
         }
         for (uint32_t w = 0; w < b.mwords; ++w) b.mask[(size_t)p * b.mwords + w] = 0u;
     }
-    const uint32_t slot = wave_append(alive, &a.rb.counts[0]);
+    const uint32_t slot = block_append(alive, &a.rb.counts[0]);
     if (alive) {
         a.rb.queue[0][slot] = p;
         a.rb.rays[0][2 * (size_t)slot] = make_float4(org.x, org.y, org.z, 0.0f);
@@ -1301,24 +1330,23 @@ __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, 
         nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
         for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) n += min(nl, kBdptMaxDepth - c) + 1;
     }
-    uint32_t at = wave_reserve(n, &a.rb.counts[4]);
+    uint32_t at = block_reserve(n, &a.rb.counts[4]);
     for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) {
         const uint32_t lm = min(nl, kBdptMaxDepth - c);
         for (uint32_t l = 0; l <= lm; ++l) b.list[at++] = ((c - 1) * (b.lmax + 1) + l) * chunk + q;
     }
 }
 
-// One lane per listed strategy (grid-stride over the list, whole waves per
+// One lane per listed strategy (grid-stride over the list, whole blocks per
 // step): evaluates connect(c, l) without its visibility test, stores
 // contribution * weight in the strategy's slot, and queues the shadow segment
 // of every strategy that needs one (wave-compacted).
 __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t chunk = p1 - p0;
     const uint32_t n = a.rb.counts[4];
-    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t j = base + lane;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {  // block-uniform steps
+        const uint32_t j = base + threadIdx.x;
         bool emit = false;
         uint32_t p = 0, s = 0;
         Connection cn;
@@ -1333,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
             emit = l > 0 && cn.needs_vis;
             b.slots[t] = make_float4(cw.x, cw.y, cw.z, emit ? 1.0f : 0.0f);
         }
-        const uint32_t slot = wave_append(emit, &a.rb.counts[2]);
+        const uint32_t slot = block_append(emit, &a.rb.counts[2]);
         if (emit) {
             a.rb.seg[2 * (size_t)slot] = make_float4(cn.from.x, cn.from.y, cn.from.z, 0.0f);
             a.rb.seg[2 * (size_t)slot + 1] = make_float4(cn.to.x, cn.to.y, cn.to.z, 0.0f);
@@ -1730,16 +1758,16 @@ __global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, u
         retire = !out.alive;
     }
     const uint32_t nxt = cur ^ 1u;
-    const uint32_t q = wave_append(out.alive, &a.rb.counts[nxt]);
+    const uint32_t q = block_append(out.alive, &a.rb.counts[nxt]);
     if (out.alive) {
         a.rb.queue[nxt][q] = slot;
         a.rb.rays[nxt][2 * (size_t)q] = make_float4(out.new_o.x, out.new_o.y, out.new_o.z, __uint_as_float(scene));
         a.rb.rays[nxt][2 * (size_t)q + 1] = make_float4(out.new_d.x, out.new_d.y, out.new_d.z, 0.0f);
     }
-    const uint32_t r = wave_append(retire, &b.ctr[4]);
+    const uint32_t r = block_append(retire, &b.ctr[4]);
     if (retire) b.retire[r] = slot;
     if (NEE) {
-        const uint32_t ss = wave_append(out.shadow, &a.rb.counts[2]);
+        const uint32_t ss = block_append(out.shadow, &a.rb.counts[2]);
         if (out.shadow) {
             a.rb.seg[2 * (size_t)ss] = make_float4(out.hp.x, out.hp.y, out.hp.z, __uint_as_float(scene));
             a.rb.seg[2 * (size_t)ss + 1] = make_float4(out.seg_to.x, out.seg_to.y, out.seg_to.z, 0.0f);
