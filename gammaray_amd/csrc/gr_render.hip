@@ -10,7 +10,7 @@
 //   per bounce:         k_trace (closest hit) over the live-path queue,
 //                       k_shade_* (material sampling, emission, Russian
 //                       roulette, next-event shadow rays) compacting survivors
-//                       into the next queue with one atomic per wave,
+//                       into the next queue with one atomic per block,
 //                       [NEE] k_trace (visibility) + k_shadow_accum
 //   k_film_gather       each pixel gathers its 9x9 neighbourhood of samples in
 //                       the order the reference's serial splat visits them, so
