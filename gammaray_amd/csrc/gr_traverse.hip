@@ -29,6 +29,7 @@ enum TraceMode { kClosest = 0, kOcclusionExact = 1, kOcclusionAnyHit = 2 };
 struct RayFrame {
     vec3 o, d, inv;
     bool nx, ny, nz;
+    uint32_t sign;  // nx | ny << 1 | nz << 2: dir_is_neg by axis
 };
 
 __device__ __forceinline__ RayFrame make_frame(vec3 o, vec3 d) {
@@ -40,30 +41,32 @@ __device__ __forceinline__ RayFrame make_frame(vec3 o, vec3 d) {
     r.nx = d.x < 0.0f;
     r.ny = d.y < 0.0f;
     r.nz = d.z < 0.0f;
+    r.sign = (r.nx ? 1u : 0u) | (r.ny ? 2u : 0u) | (r.nz ? 4u : 0u);
     return r;
 }
 
 // The part of BBox::intersect that does not depend on the current closest
 // distance; *tmin receives the entry distance the final `t_min < max_dist`
-// compares.
+// compares.  The reference's early returns become one combined condition
+// over the same comparisons of the same values (no divergent branches).
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
                                      const RayFrame& r, float pad, float* tmin) {
     float t_min = ((r.nx ? hx : lx) - r.o.x) * r.inv.x;
     float t_max = ((r.nx ? lx : hx) - r.o.x) * r.inv.x;
-    float ty_min = ((r.ny ? hy : ly) - r.o.y) * r.inv.y;
+    const float ty_min = ((r.ny ? hy : ly) - r.o.y) * r.inv.y;
     float ty_max = ((r.ny ? ly : hy) - r.o.y) * r.inv.y;
+    const float tz_min = ((r.nz ? hz : lz) - r.o.z) * r.inv.z;
+    float tz_max = ((r.nz ? lz : hz) - r.o.z) * r.inv.z;
     t_max *= pad;
     ty_max *= pad;
-    if (t_min > ty_max || ty_min > t_max) return false;
-    if (ty_min > t_min) t_min = ty_min;
-    if (ty_max < t_max) t_max = ty_max;
-    float tz_min = ((r.nz ? hz : lz) - r.o.z) * r.inv.z;
-    float tz_max = ((r.nz ? lz : hz) - r.o.z) * r.inv.z;
     tz_max *= pad;
-    if (t_min > tz_max || tz_min > t_max) return false;
-    if (tz_max < t_max) t_max = tz_max;
+    const bool ok_y = !(t_min > ty_max || ty_min > t_max);
+    t_min = ty_min > t_min ? ty_min : t_min;
+    t_max = ty_max < t_max ? ty_max : t_max;
+    const bool ok_z = !(t_min > tz_max || tz_min > t_max);
+    t_max = tz_max < t_max ? tz_max : t_max;
     *tmin = t_min;
-    return t_max > 0.0f;
+    return ok_y & ok_z & (t_max > 0.0f);
 }
 
 // Mesh::intersect_world distance (src/geom/mesh.rs:278-304) from the stored
@@ -189,6 +192,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     RayFrame r;
     r.o = r.d = r.inv = mk(0.0f, 0.0f, 0.0f);
     r.nx = r.ny = r.nz = false;
+    r.sign = 0u;
     // the lane's scene (the launch's scene unless MULTI)
     const GpuNode* nodes = sv.nodes;
     const GpuComp* comps = sv.comps;
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
             const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
             const uint32_t axis = nd.q3.z;
-            const bool flip = axis == 0 ? r.nx : (axis == 1 ? r.ny : r.nz);
+            const bool flip = ((r.sign >> axis) & 1u) != 0u;
             const uint32_t near_ref = flip ? nd.q3.y : nd.q3.x;
             const uint32_t far_ref = flip ? nd.q3.x : nd.q3.y;
             const bool near_ok = flip ? s1 : s0;
