@@ -40,9 +40,21 @@ def algorithmic_bytes(n, counts, out_bytes):
     return n * (RAY_BYTES + out_bytes) + NODE_BYTES * c[0] + TRI_BYTES * c[1] + SPHERE_BYTES * c[2]
 
 
-def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=3):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=5, single_sample=1 << 20):
     """The oracle's Bvh::intersect / Bvh::visibility restatement on `sample`
-    rays and segments of the same batches, median of `repeats` passes."""
+    rays and segments of the same batches, median of `repeats` passes after a
+    warm-up, on `threads` host threads; plus one pass on one thread over the
+    first `single_sample` rays and segments."""
     from gammaray_amd import scenes
     from oracle.oracle import OracleScene
     t0 = time.time()
@@ -61,7 +73,14 @@ def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=3):
         tc.append(t1 - t0)
         ts.append(t2 - t1)
     c, sh = float(np.median(tc)), float(np.median(ts))
+    m = min(single_sample, sample)
+    t0 = time.perf_counter()
+    o.intersect(r6[:m], threads=1)
+    o.visibility(s6[:m], threads=1)
+    t1 = 2 * m / (time.perf_counter() - t0) / 1e9
     return dict(value=2 * sample / (c + sh) / 1e9, unit="Grays/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), single_thread_value=round(t1, 6),
+                single_thread_sample="%d rays + %d segments on 1 thread" % (m, m),
                 sample="%d closest-hit rays + %d shadow segments of the timed batches "
                        "(oracle Bvh::intersect / Bvh::visibility restatement, %d threads; median of %d passes: "
                        "closest %.2f s, shadow %.2f s; %.1f s of CPU work in all)" % (
@@ -108,12 +127,18 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
             from oracle import oracle as O
             o = O.OracleScene(spec)
             oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT}[integ]
-            t0 = time.perf_counter()
-            O.render(o, spec, W, H, oi, spp=cpu_spp, max_depth=depth, light_depth=ldepth, seed=7,
-                     threads=threads)
-            dt = time.perf_counter() - t0
+            O.render(o, spec, W, H, oi, spp=1, max_depth=depth, light_depth=ldepth, seed=7, threads=threads)
+            times = []
+            for it in range(5):  # median of 5 iterations after a warm-up
+                t0 = time.perf_counter()
+                O.render(o, spec, W, H, oi, spp=cpu_spp, max_depth=depth, light_depth=ldepth, seed=7,
+                         sample_offset=1 + it * cpu_spp, threads=threads)
+                times.append(time.perf_counter() - t0)
+            dt = float(np.median(times))
             rec["cpu_baseline"] = {"mpaths_s": round(W * H * cpu_spp / dt / 1e6, 4), "cores": threads,
-                                   "kind": "port", "sample": "%d iteration(s) of the same frame" % cpu_spp}
+                                   "kind": "port", "cpu_model": cpu_model(),
+                                   "sample": "median of 5 timed passes of %d iteration(s) of the same frame "
+                                             "after a warm-up" % cpu_spp}
             rec["gpu_over_cpu"] = round(rec["mpaths_s"] / rec["cpu_baseline"]["mpaths_s"], 1)
         log("%s: %.1f Mpaths/s (%.2f ms)%s" % (name, rec["mpaths_s"], ms,
                                               ", cpu %.3f Mpaths/s" % rec["cpu_baseline"]["mpaths_s"] if cpu else ""))
