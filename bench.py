@@ -148,7 +148,7 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
     return out
 
 
-def bench_batch(ctx, threads, pool=1 << 22, steps=16):
+def bench_batch(ctx, threads, pool=1 << 23, steps=16):
     """64 jobs (side log-uniform 128..1024, spp in {4,16,64,256}) over 8
     displaced-mesh scenes (10k..2M triangles), PT+NEE depth 8, through one
     path pool with the bounce step as a hipGraph; aggregate Mpaths/s and job

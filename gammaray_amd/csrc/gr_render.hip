@@ -1810,7 +1810,7 @@ __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) 
 
 hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_batch_params& prm, double* job_ms) {
     const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
-    const uint32_t pool = prm.path_pool ? prm.path_pool : (1u << 22);
+    const uint32_t pool = prm.path_pool ? prm.path_pool : (1u << 23);
     const uint32_t poll = prm.steps_per_poll ? prm.steps_per_poll : 16u;
     // distinct scenes, job table
     std::vector<gr_scene*> scenes;

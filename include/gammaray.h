@@ -258,7 +258,7 @@ typedef struct {
 } gr_job;
 typedef struct {
     uint32_t integrator;       /* GR_INTEGRATOR_PT or GR_INTEGRATOR_PT_NEE, for every job */
-    uint32_t path_pool;        /* paths in flight; 0 = 4,194,304 */
+    uint32_t path_pool;        /* paths in flight; 0 = 8,388,608 */
     uint32_t steps_per_poll;   /* bounce steps between completion checks; 0 = 16 */
     uint32_t use_graph;        /* 1: replay the captured bounce step as a hipGraph; 0: launch eagerly */
     gr_tuning tuning;          /* the shared traversal launches; lds_stack 0 = the first job's scene's setting */
