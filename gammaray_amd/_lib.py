@@ -66,6 +66,7 @@ class GuideParams(ctypes.Structure):
 
 GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED = 0, 1, 2, 3
 GR_INTEGRATOR_DISPLAY = 4
+GR_INTEGRATOR_BDPT_MIS = 5
 
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.
 SIGNATURES = {

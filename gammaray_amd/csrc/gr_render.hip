@@ -995,7 +995,7 @@ struct BVert {
     float4 d;  // tangent.xyz, throughput.y
     float4 e;  // binormal.xyz, throughput.z
     float4 f;  // geometric normal.xyz, emission.x
-    float4 g;  // emission.y, emission.z
+    float4 g;  // emission.y, emission.z, MIS area densities: from the walk's predecessor, from its successor
 };
 
 struct BdptArgs {
@@ -1006,6 +1006,7 @@ struct BdptArgs {
     uint32_t* mask;      // visibility bits per path, mwords words each
     uint2* entry;        // shadow queue: (path, strategy)
     uint32_t* list;      // the chunk's valid strategies, as s * chunk + (path - p0)
+    uint32_t mis;        // power-heuristic MIS weights instead of the reference's 1/(c+l)
     float4* slots;       // per (strategy, path): contribution * weight, needs-visibility flag
     uint32_t cmax, lmax, mwords;
 };
@@ -1014,14 +1015,14 @@ constexpr uint32_t kBdptMaxDepth = 16;    // BDPT_MAX_DEPTH, integrators.rs:103
 constexpr uint32_t kLightFailed = 0x80000000u;
 
 __device__ __forceinline__ void store_vert(BVert* v, vec3 wi, uint32_t prim, vec3 pt, bool conn, const Frame& f,
-                                           vec3 T, vec3 le) {
+                                           vec3 T, vec3 le, float a_fwd = 0.0f) {
     v->a = make_float4(wi.x, wi.y, wi.z, __uint_as_float(prim));
     v->b = make_float4(pt.x, pt.y, pt.z, __uint_as_float(conn ? 1u : 0u));
     v->c = make_float4(f.n.x, f.n.y, f.n.z, T.x);
     v->d = make_float4(f.t.x, f.t.y, f.t.z, T.y);
     v->e = make_float4(f.b.x, f.b.y, f.b.z, T.z);
     v->f = make_float4(f.gn.x, f.gn.y, f.gn.z, le.x);
-    v->g = make_float4(le.y, le.z, 0.0f, 0.0f);
+    v->g = make_float4(le.y, le.z, a_fwd, 0.0f);
 }
 
 struct VertView {
@@ -1063,6 +1064,15 @@ __device__ __forceinline__ float convert_density(vec3 prev, vec3 next, const Fra
     return fabsf(dot(nf.n, dir)) / (dist * dist);
 }
 
+// A solid-angle density at `from` toward `to` as an area density at `to`
+// (convert_density with the receiving vertex's shading normal n).
+__device__ __forceinline__ float to_area(vec3 from, vec3 to, vec3 n) {
+    const float dist = length(to - from);
+    if (dist == 0.0f) return 0.0f;
+    const vec3 dir = (to - from) / dist;
+    return fabsf(dot(n, dir)) / (dist * dist);
+}
+
 // One step of random_walk (integrators.rs:138-229) for every queued subpath.
 template <bool CAMERA>
 __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t k, uint32_t cur) {
@@ -1070,13 +1080,16 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
     const uint32_t n = a.rb.counts[cur];
     bool alive = false;
     uint32_t p = 0;
+    float next_pdf = 0.0f;
     vec3 new_o = mk(0.0f, 0.0f, 0.0f), new_d = new_o;
     if (i < n) {
         p = a.rb.queue[cur][i];
         const uint4 hv = a.rb.hits[i];
         const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
         const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
-        vec3 T = ld3(a.rb.T[p]);
+        const float4 T4 = a.rb.T[p];
+        vec3 T = ld3(T4);
+        const float prev_pdf = T4.w;  // solid-angle density of d at the previous vertex
         BVert* v = (CAMERA ? b.cam : b.lig) + (size_t)k * a.npaths + p;
         if (h.prim != kNone) {
             const vec3 prev_T = T;
@@ -1087,11 +1100,22 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
             const uint4 r0 = draws(a.seed, a.path_base + p, CAMERA ? kStreamCamera : kStreamLight, k, 0);
             const dev::MatSample smp = dev::sample_world(m, wi, f, CAMERA, r0.x, u01(r0.y), u01(r0.z));
             const bool conn = dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0;
+            float a_fwd = 0.0f;
+            if (b.mis && k > 0) {
+                // area densities for the MIS weights: this vertex from its predecessor,
+                // the predecessor from this vertex (given the direction sampled here)
+                BVert* pv = v - a.npaths;
+                const float4 pb = pv->b, pc = pv->c;
+                const vec3 pp = mk(pb.x, pb.y, pb.z), pn = mk(pc.x, pc.y, pc.z);
+                a_fwd = prev_pdf * to_area(pp, hp, f.n);
+                pv->g.w = dev::pdf_world(m, smp.outgoing, wi, f) * to_area(hp, pp, pn);
+            }
             T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
             T = T * correct_shading_normal(wi, smp.outgoing, f, CAMERA);
             new_o = hp + smp.outgoing * kRayPush;
             new_d = smp.outgoing;
-            store_vert(v, wi, h.prim, hp, conn, f, prev_T, smp.emission);
+            store_vert(v, wi, h.prim, hp, conn, f, prev_T, smp.emission, a_fwd);
+            next_pdf = smp.pdf;
             if (k + 1 >= 4 || dev::nearly_zero3(T)) {
                 const float live = 0.25f + dev::clamp_unit(dev::luminance(T)) * (0.75f - 0.25f);
                 if (u01(r0.w) < live) T = T / live;
@@ -1105,7 +1129,7 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
             T = mk(0.0f, 0.0f, 0.0f);
             store_vert(v, -d, kNone, mk(0.0f, 0.0f, 0.0f), false, z, T, T);
         }
-        a.rb.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
+        a.rb.T[p] = make_float4(T.x, T.y, T.z, next_pdf);
         if (CAMERA) b.nc[p] = k + 1;
         else b.nl[p] = k + 1;
     }
@@ -1142,9 +1166,9 @@ __global__ __launch_bounds__(256) void k_light_start(RenderArgs a, BdptArgs b) {
                               (fabsf(dot(gn, dir)) / (lp.point_pdf * dir_pdf));
             const vec3 one = mk(1.0f, 1.0f, 1.0f);
             store_vert(b.lig + p, mk(0.0f, 0.0f, 0.0f), lp.prim, lp.p, true, lp.f, one / lp.point_pdf,
-                       mk(0.0f, 0.0f, 0.0f));
+                       mk(0.0f, 0.0f, 0.0f), lp.point_pdf);
             b.nl[p] = 1;
-            a.rb.T[p] = make_float4(init.x, init.y, init.z, 0.0f);
+            a.rb.T[p] = make_float4(init.x, init.y, init.z, dir_pdf);
             org = lp.p + dir * kRayPush;
             alive = !dev::is_zero(init) && 1 < b.lmax;
         }
@@ -1191,6 +1215,92 @@ __device__ Connection connect(const RenderArgs& a, const BdptArgs& b, uint32_t p
     return r;
 }
 
+__device__ __forceinline__ float remap0(float p) { return p != 0.0f ? p : 1.0f; }
+
+// Density of a light's cosine emission (Prim::sample_ray_world,
+// src/geom/prim.rs:37-51) toward dir from the light point of frame f.
+__device__ __forceinline__ float emit_dir_pdf(const Frame& f, vec3 dir) {
+    const float c = dot(f.gn, dir);
+    return c > 0.0f ? c * kInvPi : 0.0f;
+}
+
+__device__ __forceinline__ bool vert_conn(const BVert* v) { return __float_as_uint(v->b.w) != 0u; }
+
+// Power-heuristic MIS weight of strategy (c, l) of path p among the
+// strategies that can make the same path z_1..z_k (k = c + l; z_i = camera
+// vertex i for i <= c, z_{c+m} = light vertex l - m + 1): j camera and k - j
+// light vertices, j = 1..k, within the depth caps, joining two connectible
+// vertices (j < k) or ending on the emitter (j = k), with z_k on a light.
+// With p_j = prod_{i<=j} pF(z_i) prod_{i>j} pL(z_i) (area densities: pF along
+// the camera walk, pL along the light walk), the weight is p_c^2 / sum_j p_j^2,
+// summed through the ratios p_{j-1}/p_j = pL(z_j)/pF(z_j); zero densities
+// (specular steps, the camera) count as 1.  The walks store each vertex's own
+// densities; the ones that change with the connection -- at the two joined
+// vertices and their neighbours -- are recomputed here.
+__device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l) {
+    const size_t np = a.npaths;
+    const BVert* cam = b.cam + p;  // camera vertex i at cam[(i - 1) * np]
+    const BVert* lig = b.lig + p;  // light vertex m at lig[(m - 1) * np]
+    const uint32_t k = c + l;
+    const VertView xc = load_vert(cam + (size_t)(c - 1) * np);
+    vec3 xp = mk(0.0f, 0.0f, 0.0f), xpn = xp;  // camera vertex c - 1
+    if (c >= 2) {
+        const float4 pb = cam[(size_t)(c - 2) * np].b, pc = cam[(size_t)(c - 2) * np].c;
+        xp = mk(pb.x, pb.y, pb.z);
+        xpn = mk(pc.x, pc.y, pc.z);
+    }
+    float pl_c, pl_c1 = 0.0f, pf_c1 = 0.0f, pf_c2 = 0.0f;
+    bool conn_c, conn_c1 = false;
+    if (l == 0) {
+        if (xc.prim == kNone || !a.sh.mats[xc.prim].has_light) return 0.0f;
+        pl_c = light_area_pdf(a.sh, xc.prim);
+        conn_c = true;  // as a light point it can be connected to
+        if (c >= 2) pl_c1 = emit_dir_pdf(xc.f, normalized(xp - xc.pt)) * to_area(xc.pt, xp, xpn);
+    } else {
+        const VertView yl = load_vert(lig + (size_t)(l - 1) * np);
+        const GpuMaterial& mc = a.sh.mats[xc.prim];
+        const vec3 d_cy = normalized(yl.pt - xc.pt), d_yc = -d_cy;
+        pf_c1 = dev::pdf_world(mc, xc.wi, d_cy, xc.f) * to_area(xc.pt, yl.pt, yl.f.n);
+        if (l == 1) {
+            pl_c = emit_dir_pdf(yl.f, d_yc) * to_area(yl.pt, xc.pt, xc.f.n);
+        } else {
+            const GpuMaterial& my = a.sh.mats[yl.prim];
+            pl_c = dev::pdf_world(my, yl.wi, d_yc, yl.f) * to_area(yl.pt, xc.pt, xc.f.n);
+            const float4 qb = lig[(size_t)(l - 2) * np].b, qc = lig[(size_t)(l - 2) * np].c;
+            pf_c2 = dev::pdf_world(my, d_yc, yl.wi, yl.f) * to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
+        }
+        if (c >= 2) pl_c1 = dev::pdf_world(mc, d_cy, xc.wi, xc.f) * to_area(xc.pt, xp, xpn);
+        conn_c = xc.conn;
+        conn_c1 = yl.conn;
+    }
+    auto valid = [&](uint32_t j, bool cj, bool cj1) {
+        return j <= b.cmax && k - j <= b.lmax && (j == k || (cj && cj1));
+    };
+    float sum = valid(c, conn_c, conn_c1) ? 1.0f : 0.0f;
+    float r = 1.0f;
+    bool hi = conn_c;  // conn of z_{j+1}
+    for (uint32_t j = c - 1; j >= 1; --j) {
+        const BVert* vi = cam + (size_t)j * np;  // z_{j+1}
+        const float pf = vi->g.z;
+        const float pl = j + 1 == c ? pl_c : (j + 2 == c ? pl_c1 : vi->g.w);
+        r = r * (remap0(pl) / remap0(pf));
+        const bool cj = vert_conn(cam + (size_t)(j - 1) * np);
+        if (valid(j, cj, hi)) sum += r * r;
+        hi = cj;
+    }
+    r = 1.0f;
+    for (uint32_t m = 1; m <= l; ++m) {
+        const BVert* y = lig + (size_t)(l - m) * np;  // z_{c+m}
+        const float4 g = y->g;
+        const float pf = m == 1 ? pf_c1 : (m == 2 ? pf_c2 : g.w);
+        r = r * (remap0(pf) / remap0(g.z));
+        const bool cj = vert_conn(y);
+        const bool cj1 = m < l && vert_conn(lig + (size_t)(l - m - 1) * np);
+        if (valid(c + m, cj, cj1)) sum += r * r;
+    }
+    return sum > 0.0f ? 1.0f / sum : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_vis_scatter(RenderArgs a, BdptArgs b) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.rb.counts[2] || a.rb.vis[j] == 0) return;
@@ -1215,10 +1325,8 @@ __global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b, u
             const uint32_t s = (c - 1) * (b.lmax + 1) + l;
             const float4 v = b.slots[(size_t)s * chunk + q];
             vec3 cw = mk(v.x, v.y, v.z);
-            if (v.w != 0.0f && !(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) {
-                const float w = 1.0f / (float)(c + l);
-                cw = mk(0.0f, 0.0f, 0.0f) * w;
-            }
+            if (v.w != 0.0f && !(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32))))
+                cw = mk(0.0f, 0.0f, 0.0f);  // an occluded connection contributes zero * weight
             L = L + cw;
         }
     }
@@ -1356,7 +1464,8 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
             p = p0 + (t - s * chunk);
             const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
             cn = connect(a, b, p, c, l);
-            const float w = 1.0f / (float)(c + l);
+            // weight, integrators.rs:345-354 (1/(c+l)), or the power heuristic
+            const float w = !b.mis ? 1.0f / (float)(c + l) : dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l);
             const vec3 cw = cn.contrib * w;
             emit = l > 0 && cn.needs_vis;
             b.slots[t] = make_float4(cw.x, cw.y, cw.z, emit ? 1.0f : 0.0f);
@@ -1371,7 +1480,7 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
 }
 
 // BdptIntegrator::integrate for every path of one batch, then the film splat.
-hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb,
+hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb, bool mis,
                       float4* sample_out, hipStream_t s) {
     const size_t npaths = a.npaths;
     const size_t nstrat = (size_t)cmax * (lmax + 1);
@@ -1385,6 +1494,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, chunk * nstrat, &b, &seg, &vis);
     if (e == hipSuccess && sc->tuning.sort_bits) e = reserve_sort(sc->sort, std::max(npaths, shadow));
     if (e != hipSuccess) return e;
+    b.mis = mis ? 1u : 0u;
     a.rb.seg = seg;
     a.rb.vis = vis;
     uint32_t* cnt = film->rb.counts;
@@ -1524,7 +1634,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             a.gd.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
         }
     }
-    if (prm.integrator == GR_INTEGRATOR_BDPT) {
+    if (prm.integrator == GR_INTEGRATOR_BDPT || prm.integrator == GR_INTEGRATOR_BDPT_MIS) {
         const uint32_t cmax = (prm.max_depth == 0 || prm.max_depth > kBdptMaxDepth) ? kBdptMaxDepth : prm.max_depth;
         const uint32_t lmax =
             (prm.light_depth == 0 || prm.light_depth + 1 > kBdptMaxDepth) ? kBdptMaxDepth : prm.light_depth + 1;
@@ -1539,7 +1649,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             a.path_base = (prm.sample_offset + done) * (uint64_t)npx;
             float4* out = prm.d_sample_radiance ? reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx
                                                 : nullptr;
-            e = bdpt_batch(sc, a, film, cmax, lmax, nb, out, s);
+            e = bdpt_batch(sc, a, film, cmax, lmax, nb, prm.integrator == GR_INTEGRATOR_BDPT_MIS, out, s);
         }
         return e;
     }

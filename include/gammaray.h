@@ -198,6 +198,9 @@ typedef struct {
                                    colour of the first hit's material, black on a miss */
 #define GR_INTEGRATOR_GUIDED 3  /* PT_NEE whose BSDF sampling is mixed (one-sample MIS) with a
                                    learned guide trained online (no reference counterpart) */
+#define GR_INTEGRATOR_BDPT_MIS 5 /* BdptIntegrator's subpaths and connections (integrators.rs:102-418) with
+                                    power-heuristic MIS weights over the strategies in place of the
+                                    uniform 1/(c+l) of BdptIntegrator::weight (integrators.rs:345-354) */
 
 typedef struct gr_guide gr_guide;
 

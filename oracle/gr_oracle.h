@@ -84,6 +84,7 @@ int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t
 #define ORC_PT_NEE 1  /* path tracing with next-event estimation + MIS */
 #define ORC_BDPT 2    /* BdptIntegrator with camera/light vertex caps */
 #define ORC_DISPLAY 4 /* DisplayColorIntegrator, integrators.rs:18-34 */
+#define ORC_BDPT_MIS 5 /* BDPT over the same strategies with power-heuristic MIS weights (no reference counterpart) */
 /* Camera::new's xform = translation * rotation (row-major 4x4). */
 void orc_camera_matrix(const float quat[4], const float translate[3], float out[16]);
 /* Camera::compute_ray for n lens points st (2 floats each) -> rays (6 each). */

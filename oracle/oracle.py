@@ -186,7 +186,7 @@ class OracleScene:
 
 
 # ---- rendering -------------------------------------------------------------
-PT, PT_NEE, BDPT, DISPLAY = 0, 1, 2, 4
+PT, PT_NEE, BDPT, DISPLAY, BDPT_MIS = 0, 1, 2, 4, 5
 
 
 def camera_matrix(cam):

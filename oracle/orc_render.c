@@ -369,6 +369,10 @@ typedef struct {
     int lobe_kind, connectible;
     uint32_t prim;
     float pdf_forward, pdf_reverse;
+    /* MIS (ORC_BDPT_MIS): area-measure densities of reaching this vertex from
+     * its predecessor in the walk that made it (a_fwd) and from its successor
+     * in the opposite direction (a_rev, set when the successor is sampled). */
+    float a_fwd, a_rev;
 } bvert; /* BdptVertex, integrators.rs:107-130 */
 
 static float correct_shading_normal(ovec wi, ovec wo, const orc_frame *f, int c2l)
@@ -387,9 +391,18 @@ static float convert_density(ovec prev, ovec next, const orc_frame *nf)
     return fabsf(v_dot(nf->normal, dir)) / (dist * dist);
 }
 
-/* BdptIntegrator::random_walk, integrators.rs:138-229. */
+/* Solid-angle density at `from` toward `to` as an area density at `to`:
+ * |n_to . dir| / dist^2 with the receiving vertex's shading normal. */
+static float to_area(ovec from, ovec to, const orc_frame *tf)
+{
+    return convert_density(from, to, tf);
+}
+
+/* BdptIntegrator::random_walk, integrators.rs:138-229.  prev_pdf: the
+ * solid-angle density with which the first ray's direction was sampled (0 on
+ * the camera side, where it is never needed). */
 static void random_walk(const orc_scene *s, ovec o, ovec d, ovec T, int c2l, bvert *st, int *len,
-                        int max_len, uint64_t seed, uint64_t path, uint32_t stream)
+                        int max_len, uint64_t seed, uint64_t path, uint32_t stream, float prev_pdf)
 {
     while (!v_is_exactly_zero(T) && *len < max_len) {
         orc_hit h;
@@ -410,11 +423,19 @@ static void random_walk(const orc_scene *s, ovec o, ovec d, ovec T, int c2l, bve
             T = v_scale(T, correct_shading_normal(wi, smp.outgoing, &f, c2l));
             o = v_add(p, v_scale(smp.outgoing, ORC_RAY_PUSH_DIST));
             d = smp.outgoing;
-            if (*len > 0) st[*len - 1].pdf_reverse = pdf_rev * convert_density(st[*len - 1].point, p, &f);
+            float a_fwd = 0.0f;
+            if (*len > 0) {
+                bvert *pv = &st[*len - 1];
+                pv->pdf_reverse = pdf_rev * convert_density(pv->point, p, &f);
+                pv->a_rev = pdf_rev * to_area(p, pv->point, &pv->props);
+                a_fwd = prev_pdf * to_area(pv->point, p, &f);
+            }
             bvert *v = &st[(*len)++];
             v->incoming = wi; v->point = p; v->props = f; v->throughput = prev_T;
             v->emission = smp.emission; v->lobe_kind = smp.kind; v->connectible = conn;
             v->prim = h.prim; v->pdf_forward = smp.pdf; v->pdf_reverse = 1.0f;
+            v->a_fwd = a_fwd; v->a_rev = 0.0f;
+            prev_pdf = smp.pdf;
             if (*len >= 4 || v_is_nearly_zero(T)) {
                 float rv = orc_draw_f32(seed, path, stream, b, ORC_SLOT_RR);
                 float live = clamped_lerp(0.25f, 0.75f, v_luminance(T));
@@ -431,6 +452,7 @@ static void random_walk(const orc_scene *s, ovec o, ovec d, ovec T, int c2l, bve
             v->connectible = 0;
             v->prim = ORC_NONE;
             v->pdf_forward = 1.0f; v->pdf_reverse = 1.0f;
+            v->a_fwd = 0.0f; v->a_rev = 0.0f;
         }
     }
 }
@@ -459,15 +481,96 @@ static ovec bdpt_connect(const orc_scene *s, int c, int l, const bvert *cam, con
     return v_zero();
 }
 
+/* Density of the light's cosine emission (Prim::sample_ray_world,
+ * src/geom/prim.rs:37-51) toward dir from the light point of frame f. */
+static float emit_dir_pdf(const orc_frame *f, ovec dir)
+{
+    float c = v_dot(f->geom_normal, dir);
+    return c > 0.0f ? c * ORC_FRAC_1_PI : 0.0f;
+}
+
+static float remap0(float p) { return p != 0.0f ? p : 1.0f; }
+
+/* Power-heuristic MIS weight of strategy (c, l) among the strategies that
+ * can make the same path z_1..z_k (k = c + l; z_i = camera vertex i for
+ * i <= c, z_{c+m} = light vertex l-m+1): j camera vertices and k-j light
+ * vertices, j = 1..k, within the depth caps, connecting two connectible
+ * vertices (j < k) or hitting the emitter (j = k), with the light point z_k
+ * on a light.  p_j = prod_{i<=j} pF(z_i) prod_{i>j} pL(z_i); the weight is
+ * p_c^2 / sum_j p_j^2 from the ratios p_{j-1}/p_j = pL(z_j)/pF(z_j). */
+static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam, const bvert *lig, int cmax,
+                             int lmax)
+{
+    const int k = c + l;
+    float pf[2 * BDPT_MAX_DEPTH + 2], pl[2 * BDPT_MAX_DEPTH + 2];
+    int conn[2 * BDPT_MAX_DEPTH + 2];
+    for (int i = 1; i <= c; ++i) {
+        pf[i] = cam[i - 1].a_fwd;
+        pl[i] = cam[i - 1].a_rev;
+        conn[i] = cam[i - 1].connectible;
+    }
+    for (int m = 1; m <= l; ++m) {
+        const bvert *y = &lig[l - m];  /* light vertex l-m+1 */
+        pf[c + m] = y->a_rev;
+        pl[c + m] = y->a_fwd;
+        conn[c + m] = y->connectible;
+    }
+    const bvert *xc = &cam[c - 1];
+    int emissive;
+    if (l == 0) {
+        emissive = xc->prim != ORC_NONE && s->prims[xc->prim].mat.has_light;
+        if (!emissive) return 0.0f;
+        pl[k] = light_area_pdf(s, xc->prim);
+        conn[k] = 1;  /* as a light point it can be connected to */
+        if (c >= 2) {
+            const bvert *xp = &cam[c - 2];
+            pl[c - 1] = emit_dir_pdf(&xc->props, v_normalized(v_sub(xp->point, xc->point))) *
+                        to_area(xc->point, xp->point, &xp->props);
+        }
+    } else {
+        emissive = 1;
+        const bvert *yl = &lig[l - 1];
+        const orc_material *mc = &s->prims[xc->prim].mat;
+        const ovec d_cy = v_normalized(v_sub(yl->point, xc->point)), d_yc = v_neg(d_cy);
+        pf[c + 1] = orc_pdf_world(mc, xc->incoming, d_cy, &xc->props) * to_area(xc->point, yl->point, &yl->props);
+        if (l == 1) {
+            pl[c] = emit_dir_pdf(&yl->props, d_yc) * to_area(yl->point, xc->point, &xc->props);
+        } else {
+            const orc_material *my = &s->prims[yl->prim].mat;
+            pl[c] = orc_pdf_world(my, yl->incoming, d_yc, &yl->props) * to_area(yl->point, xc->point, &xc->props);
+            const bvert *yp = &lig[l - 2];
+            pf[c + 2] = orc_pdf_world(my, d_yc, yl->incoming, &yl->props) * to_area(yl->point, yp->point, &yp->props);
+        }
+        if (c >= 2) {
+            const bvert *xp = &cam[c - 2];
+            pl[c - 1] = orc_pdf_world(mc, d_cy, xc->incoming, &xc->props) * to_area(xc->point, xp->point, &xp->props);
+        }
+    }
+    float sum = 0.0f, r = 1.0f;
+#define ORC_MIS_VALID(j) ((j) <= cmax && k - (j) <= lmax && emissive && ((j) == k || (conn[j] && conn[(j) + 1])))
+    if (ORC_MIS_VALID(c)) sum += 1.0f;
+    for (int j = c - 1; j >= 1; --j) {
+        r = r * (remap0(pl[j + 1]) / remap0(pf[j + 1]));
+        if (ORC_MIS_VALID(j)) sum += r * r;
+    }
+    r = 1.0f;
+    for (int j = c + 1; j <= k; ++j) {
+        r = r * (remap0(pf[j]) / remap0(pl[j]));
+        if (ORC_MIS_VALID(j)) sum += r * r;
+    }
+#undef ORC_MIS_VALID
+    return sum > 0.0f ? 1.0f / sum : 0.0f;
+}
+
 static ovec bdpt_integrate(const orc_scene *s, ovec o, ovec d, uint64_t seed, uint64_t path,
-                           uint32_t max_depth, uint32_t light_depth)
+                           uint32_t max_depth, uint32_t light_depth, int mis)
 {
     bvert cam[BDPT_MAX_DEPTH], lig[BDPT_MAX_DEPTH];
     int nc = 0, nl = 0;
     int cmax = (max_depth == 0 || max_depth > BDPT_MAX_DEPTH) ? BDPT_MAX_DEPTH : (int)max_depth;
     int lmax = (light_depth == 0 || light_depth + 1 > BDPT_MAX_DEPTH) ? BDPT_MAX_DEPTH : (int)light_depth + 1;
     ovec L = v_zero();
-    random_walk(s, o, d, v_one(), 1, cam, &nc, cmax, seed, path, ORC_STREAM_CAMERA_PATH);
+    random_walk(s, o, d, v_one(), 1, cam, &nc, cmax, seed, path, ORC_STREAM_CAMERA_PATH, 0.0f);
     if (nc == 0 || s->num_lights == 0) return L;
     light_sample ls = sample_light(s, seed, path, ORC_STREAM_LIGHT_PATH, 0, 1);
     if (ls.point_pdf == 0.0f || ls.dir_pdf == 0.0f) return L;
@@ -480,13 +583,16 @@ static ovec bdpt_integrate(const orc_scene *s, ovec o, ovec d, uint64_t seed, ui
     v0->throughput = v_div(v_one(), ls.point_pdf);
     v0->emission = v_zero(); v0->lobe_kind = LOBE_NONE; v0->connectible = 1; v0->prim = ls.prim;
     v0->pdf_forward = ls.point_pdf * ls.dir_pdf; v0->pdf_reverse = 1.0f;
+    v0->a_fwd = ls.point_pdf; v0->a_rev = 0.0f;
     random_walk(s, v_add(ls.point, v_scale(ls.dir, ORC_RAY_PUSH_DIST)), ls.dir, init, 0, lig, &nl, lmax,
-                seed, path, ORC_STREAM_LIGHT_PATH);
+                seed, path, ORC_STREAM_LIGHT_PATH, ls.dir_pdf);
     for (int c = 1; c < nc + 1; ++c) {
         for (int l = 0; l < nl + 1; ++l) {
             if (c + l > BDPT_MAX_DEPTH) continue;
             ovec con = bdpt_connect(s, c, l, cam, lig);
-            float w = 1.0f / (float)(c + l); /* weight, integrators.rs:345-354 */
+            if (mis && v_is_exactly_zero(con)) continue;
+            float w = mis ? bdpt_mis_weight(s, c, l, cam, lig, cmax, lmax)
+                          : 1.0f / (float)(c + l); /* weight, integrators.rs:345-354 */
             L = v_add(L, v_scale(con, w));
         }
     }
@@ -521,7 +627,8 @@ int orc_render(const orc_scene *s, const float cam[16], float wx, float wy, uint
             if (integrator == 0) L = pt_integrate(s, o, d, seed, path, max_depth);
             else if (integrator == 1) L = nee_integrate(s, o, d, seed, path, max_depth);
             else if (integrator == ORC_DISPLAY) L = display_integrate(s, o, d);
-            else L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth);
+            else if (integrator == ORC_BDPT_MIS) L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth, 1);
+            else L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth, 0);
             col[3 * i] = L.x; col[3 * i + 1] = L.y; col[3 * i + 2] = L.z;
         }
         if (colors) memcpy(&colors[3 * npx * it], col, sizeof(float) * 3 * npx);

@@ -122,7 +122,8 @@ def test_material_functions_match_oracle(gpu_ctx):
         assert np.mean(g[:, 17] == r[:, 17]) > 0.999
 
 
-@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT),
+                                          ("bdpt_mis", O.BDPT_MIS)])
 def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
     """Same seed -> the same paths: per-sample radiance agrees."""
     print("case paired", integrator, flush=True)
@@ -139,7 +140,8 @@ def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
     assert np.isfinite(g).all()
 
 
-@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT),
+                                          ("bdpt_mis", O.BDPT_MIS)])
 def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
     """Per-pixel means of GPU and oracle (different seeds) within 3 sigma."""
     print("case 3sigma", integrator, flush=True)
@@ -215,3 +217,28 @@ def test_rmse_against_cpu_4096spp_reference_agrees(gpu_ctx, integrator, oi):
     e_gpu = np.sqrt(np.mean((_image(film.readback()) - R) ** 2))
     print("rmse cpu %.6f gpu %.6f" % (e_cpu, e_gpu), flush=True)
     assert abs(e_gpu - e_cpu) / e_cpu < 0.02, (e_gpu, e_cpu)
+
+
+def test_bdpt_mis_through_a_mirror_matches_path_tracing(gpu_ctx):
+    """Paths through a mirror (not connectible) keep only some strategies; the
+    MIS weights renormalise over those, so GPU BDPT with MIS and GPU PT agree
+    in the mean while the uniform 1/(c+l) weights fall short."""
+    print("case bdpt_mis mirror", flush=True)
+    spec = scenes.box_scene()
+    for p in spec.prims:
+        if p.material[0] == 0:
+            p.material = scenes.diffuse()
+    spec.prims[6].material = scenes.mirror()  # the inner block
+    W, H = 32, 32
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    pt = _sample_radiance(stage, film, "pt", 1024, seed=5)
+    mis = _sample_radiance(stage, film, "bdpt_mis", 256, seed=6)
+    uni = _sample_radiance(stage, film, "bdpt", 256, seed=7)
+    def z(a, b):
+        la, lb = a.sum(2).ravel(), b.sum(2).ravel()
+        return (la.mean() - lb.mean()) / np.sqrt(la.var() / la.size + lb.var() / lb.size)
+    z_mis, z_uni = z(mis, pt), z(uni, pt)
+    print("z mis %.2f uniform %.2f" % (z_mis, z_uni), flush=True)
+    assert abs(z_mis) < 4.0, z_mis
+    assert z_uni < -5.0, z_uni

@@ -324,3 +324,54 @@ def test_display_integrator_is_black_on_a_miss_and_follows_the_hit():
     hit = np.all(c == np.float32([0.9, 0.1, 0.3]), axis=1)
     miss = np.all(c == 0.0, axis=1)
     assert np.all(hit | miss) and 0.02 < hit.mean() < 0.9
+
+
+def _z_total(a, b):
+    la, lb = a.sum(2).ravel(), b.sum(2).ravel()
+    return abs(la.mean() - lb.mean()) / math.sqrt(la.var() / la.size + lb.var() / lb.size), la.mean(), lb.mean()
+
+
+@pytest.mark.parametrize("scene", ["lambert", "disney"])
+def test_bdpt_mis_is_unbiased_against_unbounded_pt(scene):
+    """Power-heuristic weights over the strategies that can make each path sum
+    to one, so BDPT with MIS estimates the unbounded PT image (up to the
+    16-vertex cap) -- on Lambertian and on pdf-consistent Disney materials."""
+    s = _lambert_box() if scene == "lambert" else scenes.box_scene(consistent=True)
+    o = O.OracleScene(s)
+    W = H = 16
+    _, a = O.render(o, s, W, H, O.PT, spp=512, seed=1, colors=True)
+    _, b = O.render(o, s, W, H, O.BDPT_MIS, spp=128, seed=2, colors=True)
+    z, ma, mb = _z_total(a, b)
+    assert z < 4.0, (ma, mb, z)
+
+
+def _mirror_box():
+    s = _lambert_box()
+    s.prims[6].material = scenes.mirror()  # the inner block
+    return s
+
+
+def test_bdpt_mis_is_unbiased_through_a_mirror_where_uniform_weights_are_not():
+    """A mirror vertex cannot be connected to, so paths through it lose
+    strategies: the reference's 1/(c+l) still divides by all of them (biased
+    low), the MIS weights renormalise over those that exist."""
+    s = _mirror_box()
+    o = O.OracleScene(s)
+    W = H = 16
+    _, pt = O.render(o, s, W, H, O.PT, spp=1024, seed=1, colors=True)
+    _, mis = O.render(o, s, W, H, O.BDPT_MIS, spp=256, seed=2, colors=True)
+    _, uni = O.render(o, s, W, H, O.BDPT, spp=512, seed=3, colors=True)
+    z_mis, m_pt, m_mis = _z_total(pt, mis)
+    z_uni, _, m_uni = _z_total(pt, uni)
+    assert z_mis < 4.0, (m_pt, m_mis, z_mis)
+    assert z_uni > 5.0 and m_uni < m_pt, (m_pt, m_uni, z_uni)
+
+
+def test_bdpt_mis_has_lower_variance_than_uniform_weights():
+    s = _lambert_box()
+    o = O.OracleScene(s)
+    W = H = 16
+    _, mis = O.render(o, s, W, H, O.BDPT_MIS, spp=128, seed=4, colors=True)
+    _, uni = O.render(o, s, W, H, O.BDPT, spp=128, seed=4, colors=True)
+    vm, vu = mis.sum(2).var(0).sum(), uni.sum(2).var(0).sum()
+    assert vm < 0.8 * vu, (vm, vu)
