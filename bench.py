@@ -92,6 +92,7 @@ PATH_WORKLOADS = [
     ("box256_pt", "box_scene", 256, 256, 16, 6, "pt", 0, 4),
     ("showcase1080_pt_nee", "showcase_scene", 1920, 1080, 64, 8, "pt_nee", 0, 1),
     ("showcase1080_bdpt", "showcase_scene", 1920, 1080, 64, 8, "bdpt", 5, 1),
+    ("showcase1080_bdpt_mis", "showcase_scene", 1920, 1080, 64, 8, "bdpt_mis", 5, 1),
 ]
 
 
@@ -126,7 +127,7 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
         if cpu:
             from oracle import oracle as O
             o = O.OracleScene(spec)
-            oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT}[integ]
+            oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT, "bdpt_mis": O.BDPT_MIS}[integ]
             O.render(o, spec, W, H, oi, spp=1, max_depth=depth, light_depth=ldepth, seed=7, threads=threads)
             times = []
             for it in range(5):  # median of 5 iterations after a warm-up
@@ -301,6 +302,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
+    ap.add_argument("--paths", default="",
+                    help="comma-separated subset of the path workloads (%s)" % ", ".join(w[0] for w in PATH_WORKLOADS))
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the guided comparison")
@@ -313,6 +316,10 @@ def main():
                     help="for counter runs: one %d MiB device copy (calibration), then one launch "
                          "of each traversal kernel; no timing, no JSON" % (COPY_PROBE_BYTES >> 20))
     args = ap.parse_args()
+    only = [w for w in args.paths.split(",") if w]
+    unknown = set(only) - {w[0] for w in PATH_WORKLOADS}
+    if unknown:
+        sys.exit("unknown path workload(s): %s" % ", ".join(sorted(unknown)))
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the film/"
               "ray batches would shard by tiles; see the design notes). Run with --gpus 1.",
@@ -450,7 +457,7 @@ def main():
     del rays, segs, hits, vis, rays8, segs8
     paths = None
     if not args.no_paths:
-        paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads)
+        paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads, only=only or None)
     batch = None if args.no_batch else bench_batch(ctx, threads)
     guided = None if args.no_guided else bench_guided(ctx, ref_spp=args.ref_spp)
 
