@@ -93,6 +93,7 @@ PATH_WORKLOADS = [
     ("showcase1080_pt_nee", "showcase_scene", 1920, 1080, 64, 8, "pt_nee", 0, 1),
     ("showcase1080_bdpt", "showcase_scene", 1920, 1080, 64, 8, "bdpt", 5, 1),
     ("showcase1080_bdpt_mis", "showcase_scene", 1920, 1080, 64, 8, "bdpt_mis", 5, 1),
+    ("showcase1080_bdpt_lt", "showcase_scene", 1920, 1080, 64, 8, "bdpt_lt", 5, 1),
 ]
 
 
@@ -127,7 +128,7 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
         if cpu:
             from oracle import oracle as O
             o = O.OracleScene(spec)
-            oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT, "bdpt_mis": O.BDPT_MIS}[integ]
+            oi = {"pt": O.PT, "pt_nee": O.PT_NEE, "bdpt": O.BDPT, "bdpt_mis": O.BDPT_MIS, "bdpt_lt": O.BDPT_LT}[integ]
             O.render(o, spec, W, H, oi, spp=1, max_depth=depth, light_depth=ldepth, seed=7, threads=threads)
             times = []
             for it in range(5):  # median of 5 iterations after a warm-up

@@ -67,6 +67,7 @@ class GuideParams(ctypes.Structure):
 GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED = 0, 1, 2, 3
 GR_INTEGRATOR_DISPLAY = 4
 GR_INTEGRATOR_BDPT_MIS = 5
+GR_INTEGRATOR_BDPT_LT = 6
 
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.
 SIGNATURES = {

@@ -213,7 +213,8 @@ def hits_to_numpy(hits):
 # ---------------------------------------------------------------------------
 INTEGRATORS = {"pt": _lib.GR_INTEGRATOR_PT, "pt_nee": _lib.GR_INTEGRATOR_PT_NEE,
                "bdpt": _lib.GR_INTEGRATOR_BDPT, "guided": _lib.GR_INTEGRATOR_GUIDED,
-               "display": _lib.GR_INTEGRATOR_DISPLAY, "bdpt_mis": _lib.GR_INTEGRATOR_BDPT_MIS}
+               "display": _lib.GR_INTEGRATOR_DISPLAY, "bdpt_mis": _lib.GR_INTEGRATOR_BDPT_MIS,
+               "bdpt_lt": _lib.GR_INTEGRATOR_BDPT_LT}
 
 
 def make_camera(spec):

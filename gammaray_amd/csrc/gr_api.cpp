@@ -615,7 +615,7 @@ int gr_trace(gr_scene* s, const gr_camera* cam, const gr_trace_params* prm, gr_f
     if (!s->committed) return fail(ctx, GR_ERR_STATE, "scene not committed%s");
     if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
     if (film->ctx != ctx) return fail(ctx, GR_ERR_INVALID, "film belongs to another context%s");
-    if (prm->integrator > GR_INTEGRATOR_BDPT_MIS) return fail(ctx, GR_ERR_INVALID, "unknown integrator%s");
+    if (prm->integrator > GR_INTEGRATOR_BDPT_LT) return fail(ctx, GR_ERR_INVALID, "unknown integrator%s");
     if (prm->integrator == GR_INTEGRATOR_GUIDED && (!prm->guide || prm->guide->ctx != ctx))
         return fail(ctx, GR_ERR_INVALID, "guided tracing needs a guide of the same context%s");
     if (prm->integrator != GR_INTEGRATOR_PT && prm->integrator != GR_INTEGRATOR_DISPLAY && s->shade.num_lights == 0)

@@ -160,7 +160,7 @@ private:
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: gammaray [--iterations N] [--height H] [--integrator bdpt|bdpt_mis|pt|pt_nee|display]\n"
+                 "usage: gammaray [--iterations N] [--height H] [--integrator bdpt|bdpt_mis|bdpt_lt|pt|pt_nee|display]\n"
                  "                [--obj PATH] [--output PATH] [--seed S] [--max-depth D] [--light-depth D]\n"
                  "                [--device N]\n");
 }
@@ -202,6 +202,7 @@ int main(int argc, char** argv) {
     else if (integ == "pt_nee") integrator = GR_INTEGRATOR_PT_NEE;
     else if (integ == "display") integrator = GR_INTEGRATOR_DISPLAY;
     else if (integ == "bdpt_mis") integrator = GR_INTEGRATOR_BDPT_MIS;
+    else if (integ == "bdpt_lt") integrator = GR_INTEGRATOR_BDPT_LT;
     else {
         usage();
         return 2;

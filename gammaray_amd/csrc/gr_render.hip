@@ -514,6 +514,7 @@ __device__ __forceinline__ uint32_t vrec_slot(const GuideArgs& g, uint32_t p, ui
 struct CameraSample {
     float2 st;
     vec3 o, d;
+    float cos_t;  // cosine between the camera-space direction and the view axis (-z)
 };
 
 // Sample point (film.rs:55-76) and camera ray (camera.rs:87-97) of one path.
@@ -536,7 +537,18 @@ __device__ __forceinline__ CameraSample camera_sample(const float* m, float wx, 
     c.o = mk(x / w, y / w, zz / w);
     c.d = mk(dir.x * m[0] + dir.y * m[4] + dir.z * m[8], dir.x * m[1] + dir.y * m[5] + dir.z * m[9],
              dir.x * m[2] + dir.y * m[6] + dir.z * m[10]);
+    c.cos_t = -dir.z;
     return c;
+}
+
+// Solid-angle density of one camera ray through a window point whose
+// camera-space direction makes cos_t with the view axis: sample points cover
+// the film with density 1 per pixel area and iteration (a pixel's point is
+// uniform over the 4x4 pixels around it), i.e. 1 / (W H) per path over the
+// window of area (2 wx)(2 wy) at distance 1, whose solid-angle element is
+// cos^3 theta dA.  BDPT's light-tracing strategy and its MIS weights use it.
+__device__ __forceinline__ float camera_dir_pdf(float wx, float wy, float cos_t) {
+    return 1.0f / (((4.0f * wx) * wy) * ((cos_t * cos_t) * cos_t));
 }
 
 __global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
@@ -550,7 +562,7 @@ __global__ __launch_bounds__(256) void k_camera(RenderArgs a) {
     a.rb.rays[0][2 * (size_t)p] = make_float4(c.o.x, c.o.y, c.o.z, 0.0f);
     a.rb.rays[0][2 * (size_t)p + 1] = make_float4(c.d.x, c.d.y, c.d.z, 0.0f);
     a.rb.queue[0][p] = p;
-    a.rb.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.rb.T[p] = make_float4(1.0f, 1.0f, 1.0f, camera_dir_pdf(a.wx, a.wy, c.cos_t));  // w: BDPT's ray density
     a.rb.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     a.rb.aux[p] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);  // previous bsdf pdf, previous bounce specular
     if (p == 0) {
@@ -897,7 +909,8 @@ constexpr int kTile = 16, kApron = 4, kSpan = kTile + 2 * kApron;
 // covers) are computed once, when the tile stages it in LDS, with the same
 // arithmetic the splat uses; each pixel then only compares its coordinates
 // with the staged bounds and weights the samples that cover it.
-__global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters) {
+__global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters,
+                                                              const float4* splat, const float* ew) {
     __shared__ float s_cd[kSpan * kSpan], s_rd[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
         s_b[kSpan * kSpan];
     __shared__ int4 s_box[kSpan * kSpan];  // min_col, max_col, min_row, max_row
@@ -950,6 +963,16 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
             }
         }
         __syncthreads();
+    }
+    if (inside && splat) {
+        // BDPT's light-traced radiance of the batch, weighted by the film weight
+        // the batch's camera samples give the pixel in expectation (per iteration:
+        // its column's expectation times its row's)
+        const float e = ew[x] * ew[W + y];
+        const float4 sp = splat[(size_t)y * W + x];
+        acc.x = acc.x + e * sp.x;
+        acc.y = acc.y + e * sp.y;
+        acc.z = acc.z + e * sp.z;
     }
     if (inside) film[(size_t)y * W + x] = acc;
 }
@@ -1006,9 +1029,12 @@ struct BdptArgs {
     uint32_t* mask;      // visibility bits per path, mwords words each
     uint2* entry;        // shadow queue: (path, strategy)
     uint32_t* list;      // the chunk's valid strategies, as s * chunk + (path - p0)
-    uint32_t mis;        // power-heuristic MIS weights instead of the reference's 1/(c+l)
+    uint32_t mis;        // 0: the reference's 1/(c+l); 1: power-heuristic MIS; 2: MIS + light tracing
     float4* slots;       // per (strategy, path): contribution * weight, needs-visibility flag
+    float4* splat;       // light tracing: per pixel, the batch's splatted radiance (xyz)
+    float* ew;           // light tracing: expected film weight per column (width) then per row (height)
     uint32_t cmax, lmax, mwords;
+    uint32_t lt_base;    // strategy index of light tracing's l = 1: cmax (lmax + 1)
 };
 
 constexpr uint32_t kBdptMaxDepth = 16;    // BDPT_MAX_DEPTH, integrators.rs:103
@@ -1109,6 +1135,8 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
                 const vec3 pp = mk(pb.x, pb.y, pb.z), pn = mk(pc.x, pc.y, pc.z);
                 a_fwd = prev_pdf * to_area(pp, hp, f.n);
                 pv->g.w = dev::pdf_world(m, smp.outgoing, wi, f) * to_area(hp, pp, pn);
+            } else if (b.mis) {
+                a_fwd = prev_pdf * to_area(o, hp, f.n);  // the camera's first vertex, from the camera
             }
             T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
             T = T * correct_shading_normal(wi, smp.outgoing, f, CAMERA);
@@ -1226,6 +1254,55 @@ __device__ __forceinline__ float emit_dir_pdf(const Frame& f, vec3 dir) {
 
 __device__ __forceinline__ bool vert_conn(const BVert* v) { return __float_as_uint(v->b.w) != 0u; }
 
+// The pinhole camera as the light-tracing strategy sees it: its position
+// (the camera transform of the origin), the rows of the transform's 3x3 part
+// (camera-space axes in world space), the window and the film size.
+struct LtCamera {
+    vec3 e, ax, ay, az;
+    float wx, wy;
+    uint32_t width, height;
+};
+
+__device__ __forceinline__ LtCamera lt_camera(const RenderArgs& a) {
+    const float* m = a.cam;
+    LtCamera c;
+    c.e = mk(m[12] / m[15], m[13] / m[15], m[14] / m[15]);
+    c.ax = mk(m[0], m[1], m[2]);
+    c.ay = mk(m[4], m[5], m[6]);
+    c.az = mk(m[8], m[9], m[10]);
+    c.wx = a.wx;
+    c.wy = a.wy;
+    c.width = a.width;
+    c.height = a.height;
+    return c;
+}
+
+// Projection of world point y seen from the camera: its film position
+// (col_cont - 0.5, row_cont - 0.5, as the film computes it for a sample), the
+// unit direction from y to the camera and cos theta.  False when y is behind
+// the camera or its splat (the pixels within the filter width) misses the film.
+__device__ __forceinline__ bool lt_project(const LtCamera& c, vec3 y, float& cd, float& rd, vec3& to_cam,
+                                           float& cos_t) {
+    const vec3 d = y - c.e;
+    const float len = length(d);
+    if (!(len > 0.0f)) return false;
+    const vec3 u = d / len;
+    const float dx = dot(u, c.ax), dy = dot(u, c.ay), dz = dot(u, c.az);
+    if (!(dz < 0.0f)) return false;
+    const float s = (dx / -dz) / c.wx, t = (dy / -dz) / c.wy;
+    const float wf = (float)c.width, hf = (float)c.height;
+    const float col_cont = 0.0f + (0.5f * (s + 1.0f)) * (wf - 0.0f);
+    const float row_cont = 0.0f + (0.5f * (t + 1.0f)) * (hf - 0.0f);
+    cd = col_cont - 0.5f;
+    rd = row_cont - 0.5f;
+    if (!(floorf(cd + 2.0f) >= 0.0f && ceilf(cd - 2.0f) <= wf - 1.0f && floorf(rd + 2.0f) >= 0.0f &&
+          ceilf(rd - 2.0f) <= hf - 1.0f))
+        return false;
+    to_cam = -u;
+    cos_t = -dz / sqrtf(dx * dx + dy * dy + dz * dz);
+    return true;
+}
+
 // Power-heuristic MIS weight of strategy (c, l) of path p among the
 // strategies that can make the same path z_1..z_k (k = c + l; z_i = camera
 // vertex i for i <= c, z_{c+m} = light vertex l - m + 1): j camera and k - j
@@ -1234,57 +1311,76 @@ __device__ __forceinline__ bool vert_conn(const BVert* v) { return __float_as_ui
 // With p_j = prod_{i<=j} pF(z_i) prod_{i>j} pL(z_i) (area densities: pF along
 // the camera walk, pL along the light walk), the weight is p_c^2 / sum_j p_j^2,
 // summed through the ratios p_{j-1}/p_j = pL(z_j)/pF(z_j); zero densities
-// (specular steps, the camera) count as 1.  The walks store each vertex's own
-// densities; the ones that change with the connection -- at the two joined
-// vertices and their neighbours -- are recomputed here.
+// (delta lobes) count as 1.  The walks store each vertex's own densities
+// (the camera's first vertex: the camera's ray density); the ones that change
+// with the connection -- at the two joined vertices and their neighbours --
+// are recomputed here.  With light tracing (b.mis == 2) j = 0 is a strategy
+// too (all k vertices from the light walk, z_1 joined to the camera, which
+// counts as connectible), and c = 0 is allowed: z_1 = y_l seen by the camera.
 __device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l) {
     const size_t np = a.npaths;
     const BVert* cam = b.cam + p;  // camera vertex i at cam[(i - 1) * np]
     const BVert* lig = b.lig + p;  // light vertex m at lig[(m - 1) * np]
     const uint32_t k = c + l;
-    const VertView xc = load_vert(cam + (size_t)(c - 1) * np);
-    vec3 xp = mk(0.0f, 0.0f, 0.0f), xpn = xp;  // camera vertex c - 1
-    if (c >= 2) {
-        const float4 pb = cam[(size_t)(c - 2) * np].b, pc = cam[(size_t)(c - 2) * np].c;
-        xp = mk(pb.x, pb.y, pb.z);
-        xpn = mk(pc.x, pc.y, pc.z);
-    }
-    float pl_c, pl_c1 = 0.0f, pf_c1 = 0.0f, pf_c2 = 0.0f;
-    bool conn_c, conn_c1 = false;
-    if (l == 0) {
-        if (xc.prim == kNone || !a.sh.mats[xc.prim].has_light) return 0.0f;
-        pl_c = light_area_pdf(a.sh, xc.prim);
-        conn_c = true;  // as a light point it can be connected to
-        if (c >= 2) pl_c1 = emit_dir_pdf(xc.f, normalized(xp - xc.pt)) * to_area(xc.pt, xp, xpn);
-    } else {
+    float pl_c = 0.0f, pl_c1 = 0.0f, pf_c1 = 0.0f, pf_c2 = 0.0f;
+    bool conn_c = true, conn_c1 = false;  // conn of z_c (the camera when c = 0) and z_{c+1}
+    if (c == 0) {
         const VertView yl = load_vert(lig + (size_t)(l - 1) * np);
-        const GpuMaterial& mc = a.sh.mats[xc.prim];
-        const vec3 d_cy = normalized(yl.pt - xc.pt), d_yc = -d_cy;
-        pf_c1 = dev::pdf_world(mc, xc.wi, d_cy, xc.f) * to_area(xc.pt, yl.pt, yl.f.n);
-        if (l == 1) {
-            pl_c = emit_dir_pdf(yl.f, d_yc) * to_area(yl.pt, xc.pt, xc.f.n);
-        } else {
+        const LtCamera lc = lt_camera(a);
+        float cd, rd, cos_t;
+        vec3 d_yc;
+        if (!lt_project(lc, yl.pt, cd, rd, d_yc, cos_t)) return 0.0f;
+        pf_c1 = camera_dir_pdf(lc.wx, lc.wy, cos_t) * to_area(lc.e, yl.pt, yl.f.n);
+        if (l >= 2) {
             const GpuMaterial& my = a.sh.mats[yl.prim];
-            pl_c = dev::pdf_world(my, yl.wi, d_yc, yl.f) * to_area(yl.pt, xc.pt, xc.f.n);
             const float4 qb = lig[(size_t)(l - 2) * np].b, qc = lig[(size_t)(l - 2) * np].c;
             pf_c2 = dev::pdf_world(my, d_yc, yl.wi, yl.f) * to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
         }
-        if (c >= 2) pl_c1 = dev::pdf_world(mc, d_cy, xc.wi, xc.f) * to_area(xc.pt, xp, xpn);
-        conn_c = xc.conn;
         conn_c1 = yl.conn;
+    } else {
+        const VertView xc = load_vert(cam + (size_t)(c - 1) * np);
+        vec3 xp = mk(0.0f, 0.0f, 0.0f), xpn = xp;  // camera vertex c - 1
+        if (c >= 2) {
+            const float4 pb = cam[(size_t)(c - 2) * np].b, pc = cam[(size_t)(c - 2) * np].c;
+            xp = mk(pb.x, pb.y, pb.z);
+            xpn = mk(pc.x, pc.y, pc.z);
+        }
+        if (l == 0) {
+            if (xc.prim == kNone || !a.sh.mats[xc.prim].has_light) return 0.0f;
+            pl_c = light_area_pdf(a.sh, xc.prim);
+            conn_c = true;  // as a light point it can be connected to
+            if (c >= 2) pl_c1 = emit_dir_pdf(xc.f, normalized(xp - xc.pt)) * to_area(xc.pt, xp, xpn);
+        } else {
+            const VertView yl = load_vert(lig + (size_t)(l - 1) * np);
+            const GpuMaterial& mc = a.sh.mats[xc.prim];
+            const vec3 d_cy = normalized(yl.pt - xc.pt), d_yc = -d_cy;
+            pf_c1 = dev::pdf_world(mc, xc.wi, d_cy, xc.f) * to_area(xc.pt, yl.pt, yl.f.n);
+            if (l == 1) {
+                pl_c = emit_dir_pdf(yl.f, d_yc) * to_area(yl.pt, xc.pt, xc.f.n);
+            } else {
+                const GpuMaterial& my = a.sh.mats[yl.prim];
+                pl_c = dev::pdf_world(my, yl.wi, d_yc, yl.f) * to_area(yl.pt, xc.pt, xc.f.n);
+                const float4 qb = lig[(size_t)(l - 2) * np].b, qc = lig[(size_t)(l - 2) * np].c;
+                pf_c2 = dev::pdf_world(my, d_yc, yl.wi, yl.f) * to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
+            }
+            if (c >= 2) pl_c1 = dev::pdf_world(mc, d_cy, xc.wi, xc.f) * to_area(xc.pt, xp, xpn);
+            conn_c = xc.conn;
+            conn_c1 = yl.conn;
+        }
     }
-    auto valid = [&](uint32_t j, bool cj, bool cj1) {
-        return j <= b.cmax && k - j <= b.lmax && (j == k || (cj && cj1));
+    auto valid = [&](int j, bool cj, bool cj1) {
+        return (uint32_t)j <= b.cmax && k - (uint32_t)j <= b.lmax && ((uint32_t)j == k || (cj && cj1));
     };
-    float sum = valid(c, conn_c, conn_c1) ? 1.0f : 0.0f;
+    float sum = valid((int)c, conn_c, conn_c1) ? 1.0f : 0.0f;
     float r = 1.0f;
     bool hi = conn_c;  // conn of z_{j+1}
-    for (uint32_t j = c - 1; j >= 1; --j) {
+    const int jmin = b.mis == 2 ? 0 : 1;
+    for (int j = (int)c - 1; j >= jmin; --j) {
         const BVert* vi = cam + (size_t)j * np;  // z_{j+1}
         const float pf = vi->g.z;
-        const float pl = j + 1 == c ? pl_c : (j + 2 == c ? pl_c1 : vi->g.w);
+        const float pl = j + 1 == (int)c ? pl_c : (j + 2 == (int)c ? pl_c1 : vi->g.w);
         r = r * (remap0(pl) / remap0(pf));
-        const bool cj = vert_conn(cam + (size_t)(j - 1) * np);
+        const bool cj = j == 0 || vert_conn(cam + (size_t)(j - 1) * np);
         if (valid(j, cj, hi)) sum += r * r;
         hi = cj;
     }
@@ -1296,9 +1392,109 @@ __device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, 
         r = r * (remap0(pf) / remap0(g.z));
         const bool cj = vert_conn(y);
         const bool cj1 = m < l && vert_conn(lig + (size_t)(l - m - 1) * np);
-        if (valid(c + m, cj, cj1)) sum += r * r;
+        if (valid((int)(c + m), cj, cj1)) sum += r * r;
     }
     return sum > 0.0f ? 1.0f / sum : 0.0f;
+}
+
+// The light-tracing strategy (c = 0) for light vertex l of path p: the
+// radiance the light subpath carries to y and y scatters toward the camera,
+// T f |n.w| / d^2, times the camera's density of that direction (the
+// importance of a pinhole whose film is sampled uniformly); the visibility
+// segment runs from the camera to y.
+__device__ Connection light_trace(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t l) {
+    Connection r;
+    r.contrib = mk(0.0f, 0.0f, 0.0f);
+    r.needs_vis = false;
+    const VertView y = load_vert(b.lig + (size_t)(l - 1) * a.npaths + p);
+    if (y.prim == kNone || !y.conn) return r;
+    const LtCamera lc = lt_camera(a);
+    float cd, rd, cos_t;
+    vec3 to_cam;
+    if (!lt_project(lc, y.pt, cd, rd, to_cam, cos_t)) return r;
+    const GpuMaterial& m = a.sh.mats[y.prim];
+    const vec3 f = l == 1 ? dev::light_world(m, to_cam, y.f) : dev::f_world(m, y.wi, to_cam, y.f, false);
+    const float g = convert_density(lc.e, y.pt, y.f);
+    r.contrib = mul(y.T, f) * (g * camera_dir_pdf(lc.wx, lc.wy, cos_t));
+    if (dev::nearly_zero3(r.contrib)) {
+        r.contrib = mk(0.0f, 0.0f, 0.0f);
+        return r;
+    }
+    r.needs_vis = true;
+    r.from = lc.e;
+    r.to = y.pt;
+    return r;
+}
+
+// Splats the visible light-tracing contributions of paths [p0, p1) onto the
+// batch's splat buffer with the film's Mitchell weights over the pixels
+// within the filter width (the footprint a camera sample at the same film
+// position has).  Float atomics: the per-pixel sums do not keep the paths'
+// order.
+__global__ __launch_bounds__(256) void k_lt_splat(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+    const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1) return;
+    const uint32_t nl = b.nl[p];
+    if (nl & kLightFailed) return;
+    const uint32_t chunk = p1 - p0, q = p - p0;
+    const LtCamera lc = lt_camera(a);
+    const int W = (int)a.width, H = (int)a.height;
+    for (uint32_t l = 1; l <= nl; ++l) {
+        const uint32_t s = b.lt_base + l - 1;
+        const float4 v = b.slots[(size_t)s * chunk + q];
+        if (v.w == 0.0f || !(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) continue;
+        const float4 pb = b.lig[(size_t)(l - 1) * a.npaths + p].b;
+        float cd, rd, cos_t;
+        vec3 to_cam;
+        if (!lt_project(lc, mk(pb.x, pb.y, pb.z), cd, rd, to_cam, cos_t)) continue;
+        const int x0 = clampi(f2i(ceilf(cd - 2.0f)), 0, W - 1), x1 = clampi(f2i(floorf(cd + 2.0f)), 0, W - 1);
+        const int y0 = clampi(f2i(ceilf(rd - 2.0f)), 0, H - 1), y1 = clampi(f2i(floorf(rd + 2.0f)), 0, H - 1);
+        for (int y = y0; y <= y1; ++y) {
+            const float wy = mitchell1(((float)y - rd) / 2.0f);
+            for (int x = x0; x <= x1; ++x) {
+                const float wt = mitchell1(((float)x - cd) / 2.0f) * wy;
+                float* px = reinterpret_cast<float*>(b.splat + (size_t)y * W + x);
+                unsafeAtomicAdd(px + 0, v.x * wt);
+                unsafeAtomicAdd(px + 1, v.y * wt);
+                unsafeAtomicAdd(px + 2, v.z * wt);
+            }
+        }
+    }
+}
+
+// Antiderivative of the 1-D Mitchell kernel k(t) = mitchell_filter1(t / 2)
+// (B = C = 1/3, support |t| <= 2, integral 1), odd, in double.
+__device__ double mitchell_integral(double t) {
+    const double a = fabs(t);
+    double r;
+    if (a <= 1.0) {
+        r = (1.75 * a * a * a * a - 4.0 * a * a * a + (16.0 / 3.0) * a) / 6.0;
+    } else {
+        const double u = a < 2.0 ? a : 2.0;
+        const double fu = -(7.0 / 12.0) * u * u * u * u + 4.0 * u * u * u - 10.0 * u * u + (32.0 / 3.0) * u;
+        const double f1 = -(7.0 / 12.0) * 1.0 * 1.0 * 1.0 * 1.0 + 4.0 * 1.0 * 1.0 * 1.0 - 10.0 * 1.0 * 1.0 +
+                          (32.0 / 3.0) * 1.0;
+        r = (1.75 - 4.0 + 16.0 / 3.0) / 6.0 + (fu - f1) / 6.0;
+    }
+    return t < 0.0 ? -r : r;
+}
+
+// Expected film weight one iteration's samples give each pixel column
+// (threads [0, width)) and row (threads [width, width + height)): the sample
+// of pixel q lies uniformly within q +- 2, so E = (1/4) sum_q
+// int_{-2}^{2} k(x - q - j) dj -- 1 inside the film, less within two pixels of
+// its edges.  A pixel's expectation is its column's times its row's.
+__global__ void k_expected_weights(uint32_t width, uint32_t height, float* ew) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= width + height) return;
+    const int64_t n = i < width ? width : height;
+    const int64_t x = i < width ? i : i - width;
+    double e = 0.0;
+    for (int64_t q = x - 4; q <= x + 4; ++q) {
+        if (q < 0 || q >= n) continue;
+        e += mitchell_integral((double)(x - q) + 2.0) - mitchell_integral((double)(x - q) - 2.0);
+    }
+    ew[i] = (float)(0.25 * e);
 }
 
 __global__ __launch_bounds__(256) void k_vis_scatter(RenderArgs a, BdptArgs b) {
@@ -1381,10 +1577,11 @@ hipError_t reserve(RenderBuffers& rb, size_t paths) {
 
 // BDPT buffers: vertices for cmax camera and lmax light vertices per path,
 // counts, visibility masks, and a shadow queue of `shadow` entries.
-hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_t lmax, size_t shadow,
-                        size_t nslots, BdptArgs* b, float4** seg, uint32_t** vis) {
+hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_t lmax, uint32_t nstrat,
+                        size_t shadow, size_t nslots, size_t npx, size_t nfilm_lines, BdptArgs* b, float4** seg,
+                        uint32_t** vis) {
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const uint32_t mwords = (cmax * (lmax + 1) + 31) / 32;
+    const uint32_t mwords = (nstrat + 31) / 32;
     size_t off = 0;
     const size_t o_cam = off; off = align(off + (size_t)cmax * npaths * sizeof(BVert));
     const size_t o_lig = off; off = align(off + (size_t)lmax * npaths * sizeof(BVert));
@@ -1396,6 +1593,8 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     const size_t o_vis = off; off = align(off + shadow * 4);
     const size_t o_slots = off; off = align(off + nslots * sizeof(float4));
     const size_t o_list = off; off = align(off + nslots * sizeof(uint32_t));
+    const size_t o_splat = off; off = align(off + npx * sizeof(float4));
+    const size_t o_ew = off; off = align(off + nfilm_lines * sizeof(float));
     if (rb.bdpt_capacity < off) {
         (void)hipFree(rb.bdpt_blob);
         rb.bdpt_blob = nullptr;
@@ -1413,6 +1612,9 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     b->entry = reinterpret_cast<uint2*>(base + o_entry);
     b->slots = reinterpret_cast<float4*>(base + o_slots);
     b->list = reinterpret_cast<uint32_t*>(base + o_list);
+    b->splat = reinterpret_cast<float4*>(base + o_splat);
+    b->ew = reinterpret_cast<float*>(base + o_ew);
+    b->lt_base = cmax * (lmax + 1);
     b->cmax = cmax;
     b->lmax = lmax;
     b->mwords = mwords;
@@ -1432,17 +1634,20 @@ namespace {
 __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t chunk = p1 - p0;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nc = 0, nl = 0, n = 0;
+    uint32_t nc = 0, nl = 0, n = 0, nlt = 0;
     if (q < chunk) {
         nl = b.nl[p0 + q];
         nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
         for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) n += min(nl, kBdptMaxDepth - c) + 1;
+        if (b.mis == 2 && !(nl & kLightFailed)) nlt = nl;  // light tracing: (0, l), l = 1..nl
+        n += nlt;
     }
     uint32_t at = block_reserve(n, &a.rb.counts[4]);
     for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) {
         const uint32_t lm = min(nl, kBdptMaxDepth - c);
         for (uint32_t l = 0; l <= lm; ++l) b.list[at++] = ((c - 1) * (b.lmax + 1) + l) * chunk + q;
     }
+    for (uint32_t l = 1; l <= nlt; ++l) b.list[at++] = (b.lt_base + l - 1) * chunk + q;
 }
 
 // One lane per listed strategy (grid-stride over the list, whole blocks per
@@ -1462,12 +1667,21 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
             const uint32_t t = b.list[j];
             s = t / chunk;
             p = p0 + (t - s * chunk);
-            const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
-            cn = connect(a, b, p, c, l);
-            // weight, integrators.rs:345-354 (1/(c+l)), or the power heuristic
-            const float w = !b.mis ? 1.0f / (float)(c + l) : dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l);
-            const vec3 cw = cn.contrib * w;
-            emit = l > 0 && cn.needs_vis;
+            vec3 cw;
+            if (s >= b.lt_base) {  // light tracing: light vertex l seen by the camera
+                const uint32_t l = s - b.lt_base + 1;
+                cn = light_trace(a, b, p, l);
+                cw = dev::is_zero(cn.contrib) ? cn.contrib : cn.contrib * mis_weight(a, b, p, 0, l);
+                emit = cn.needs_vis;
+            } else {
+                const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
+                cn = connect(a, b, p, c, l);
+                // weight, integrators.rs:345-354 (1/(c+l)), or the power heuristic
+                const float w =
+                    !b.mis ? 1.0f / (float)(c + l) : dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l);
+                cw = cn.contrib * w;
+                emit = l > 0 && cn.needs_vis;
+            }
             b.slots[t] = make_float4(cw.x, cw.y, cw.z, emit ? 1.0f : 0.0f);
         }
         const uint32_t slot = block_append(emit, &a.rb.counts[2]);
@@ -1480,21 +1694,31 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
 }
 
 // BdptIntegrator::integrate for every path of one batch, then the film splat.
-hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb, bool mis,
-                      float4* sample_out, hipStream_t s) {
+hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, uint32_t lmax, uint32_t nb,
+                      uint32_t mis, float4* sample_out, hipStream_t s) {
     const size_t npaths = a.npaths;
-    const size_t nstrat = (size_t)cmax * (lmax + 1);
+    // strategies (c, l), c >= 1, at s = (c - 1)(lmax + 1) + l; light tracing's (0, l) after them
+    const size_t nstrat = (size_t)cmax * (lmax + 1) + (mis == 2 ? lmax : 0);
     // strategies per connection chunk: at most 2^27 (2 GiB of slots, 512 MiB of
-    // strategy list; the shadow queue holds chunk * cmax * lmax segments)
+    // strategy list)
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(npaths, ((size_t)1 << 27) / nstrat));
-    const size_t shadow = chunk * cmax * lmax;
+    // segments: (c, l) with l >= 1, plus light tracing's (0, l)
+    const size_t shadow = chunk * (cmax + (mis == 2 ? 1 : 0)) * lmax;
     BdptArgs b;
     float4* seg = nullptr;
     uint32_t* vis = nullptr;
-    hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, shadow, chunk * nstrat, &b, &seg, &vis);
+    const size_t npx = (size_t)a.width * a.height;
+    hipError_t e = reserve_bdpt(film->rb, npaths, cmax, lmax, (uint32_t)nstrat, shadow, chunk * nstrat, npx,
+                                (size_t)a.width + a.height, &b, &seg, &vis);
     if (e == hipSuccess && sc->tuning.sort_bits) e = reserve_sort(sc->sort, std::max(npaths, shadow));
     if (e != hipSuccess) return e;
-    b.mis = mis ? 1u : 0u;
+    b.mis = mis;
+    if (mis == 2) {
+        e = hipMemsetAsync(b.splat, 0, npx * sizeof(float4), s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_expected_weights, blocks_for(a.width + a.height), dim3(256), 0, s, a.width, a.height,
+                           b.ew);
+    }
     a.rb.seg = seg;
     a.rb.vis = vis;
     uint32_t* cnt = film->rb.counts;
@@ -1530,11 +1754,14 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
                             vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(k_connect_sum, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
+        if (mis == 2)
+            hipLaunchKernelGGL(k_lt_splat, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
     }
     if (e != hipSuccess) return e;
     if (sample_out) hipLaunchKernelGGL(k_export_samples, blocks_for(npaths), dim3(256), 0, s, a, sample_out);
     const dim3 gather_grid((a.width + kTile - 1) / kTile, (a.height + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb);
+    hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb,
+                       b.mis == 2 ? b.splat : nullptr, b.mis == 2 ? b.ew : nullptr);
     return hipGetLastError();
 }
 }  // namespace
@@ -1634,7 +1861,8 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             a.gd.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
         }
     }
-    if (prm.integrator == GR_INTEGRATOR_BDPT || prm.integrator == GR_INTEGRATOR_BDPT_MIS) {
+    if (prm.integrator == GR_INTEGRATOR_BDPT || prm.integrator == GR_INTEGRATOR_BDPT_MIS ||
+        prm.integrator == GR_INTEGRATOR_BDPT_LT) {
         const uint32_t cmax = (prm.max_depth == 0 || prm.max_depth > kBdptMaxDepth) ? kBdptMaxDepth : prm.max_depth;
         const uint32_t lmax =
             (prm.light_depth == 0 || prm.light_depth + 1 > kBdptMaxDepth) ? kBdptMaxDepth : prm.light_depth + 1;
@@ -1649,7 +1877,8 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             a.path_base = (prm.sample_offset + done) * (uint64_t)npx;
             float4* out = prm.d_sample_radiance ? reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx
                                                 : nullptr;
-            e = bdpt_batch(sc, a, film, cmax, lmax, nb, prm.integrator == GR_INTEGRATOR_BDPT_MIS, out, s);
+            const uint32_t mis = prm.integrator == GR_INTEGRATOR_BDPT_LT ? 2u : prm.integrator == GR_INTEGRATOR_BDPT_MIS ? 1u : 0u;
+            e = bdpt_batch(sc, a, film, cmax, lmax, nb, mis, out, s);
         }
         return e;
     }
@@ -1724,7 +1953,8 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         if (prm.d_sample_radiance)
             hipLaunchKernelGGL(k_export_samples, blocks_for(a.npaths), dim3(256), 0, s, a,
                                reinterpret_cast<float4*>(prm.d_sample_radiance) + (size_t)done * npx);
-        hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb);
+        hipLaunchKernelGGL(k_film_gather, gather_grid, dim3(kTile * kTile), 0, s, a, film->d_pixels, nb, nullptr,
+                           nullptr);
         e = hipGetLastError();
     }
     return e;
@@ -2112,7 +2342,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             g.rb.st = job_st[j];
             const dim3 grid((g.width + kTile - 1) / kTile, (g.height + kTile - 1) / kTile);
             hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, s, g, jobs[j].film->d_pixels,
-                               jobs[j].spp);
+                               jobs[j].spp, nullptr, nullptr);
             e = hipGetLastError();
             if (e == hipSuccess) e = hipEventRecord(ev_job[j], s);
             gathered[j] = 1;

@@ -201,6 +201,10 @@ typedef struct {
 #define GR_INTEGRATOR_BDPT_MIS 5 /* BdptIntegrator's subpaths and connections (integrators.rs:102-418) with
                                     power-heuristic MIS weights over the strategies in place of the
                                     uniform 1/(c+l) of BdptIntegrator::weight (integrators.rs:345-354) */
+#define GR_INTEGRATOR_BDPT_LT 6  /* GR_INTEGRATOR_BDPT_MIS plus the light-tracing strategy: every light
+                                    vertex the camera sees is joined to it and splatted onto the film with
+                                    the film's filter; the power-heuristic weights include that strategy.
+                                    d_sample_radiance then holds the camera strategies' part only */
 
 typedef struct gr_guide gr_guide;
 

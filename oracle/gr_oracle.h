@@ -85,6 +85,7 @@ int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t
 #define ORC_BDPT 2    /* BdptIntegrator with camera/light vertex caps */
 #define ORC_DISPLAY 4 /* DisplayColorIntegrator, integrators.rs:18-34 */
 #define ORC_BDPT_MIS 5 /* BDPT over the same strategies with power-heuristic MIS weights (no reference counterpart) */
+#define ORC_BDPT_LT 6  /* ORC_BDPT_MIS plus the light-tracing strategy (light vertices splatted onto the film) */
 /* Camera::new's xform = translation * rotation (row-major 4x4). */
 void orc_camera_matrix(const float quat[4], const float translate[3], float out[16]);
 /* Camera::compute_ray for n lens points st (2 floats each) -> rays (6 each). */
@@ -96,6 +97,8 @@ void orc_sample_points(uint64_t seed, uint64_t sample_offset, uint32_t width, ui
  * (r, g, b, weight) per pixel. */
 void orc_report_samples(float *film, uint32_t width, uint32_t height, const float *st, const float *colors,
                         int64_t n);
+/* Expected per-iteration film weight of each of n pixel columns (or rows). */
+void orc_expected_weights(uint32_t n, float *out);
 /* spp iterations of Stage::trace: per iteration, sample points, integrate
  * every pixel's sample (path id = iteration * W * H + pixel), splat in sample
  * order.  colors (optional) receives spp * W * H * 3 sample radiances. */

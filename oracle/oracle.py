@@ -60,6 +60,7 @@ def lib():
         L.orc_mitchell1.argtypes = [ctypes.c_float]
         L.orc_mitchell1.restype = ctypes.c_float
         L.orc_sample_points.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _f32p]
+        L.orc_expected_weights.argtypes = [ctypes.c_uint32, _f32p]
         L.orc_report_samples.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                          ctypes.c_int64]
         L.orc_render.argtypes = [_vp, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
@@ -186,7 +187,7 @@ class OracleScene:
 
 
 # ---- rendering -------------------------------------------------------------
-PT, PT_NEE, BDPT, DISPLAY, BDPT_MIS = 0, 1, 2, 4, 5
+PT, PT_NEE, BDPT, DISPLAY, BDPT_MIS, BDPT_LT = 0, 1, 2, 4, 5, 6
 
 
 def camera_matrix(cam):
@@ -222,6 +223,13 @@ def sample_points(seed, iteration, width, height):
     st = np.zeros((height * width, 2), np.float32)
     lib().orc_sample_points(seed, iteration, width, height, _p(st, _f32p))
     return st
+
+
+def expected_weights(n):
+    """Expected per-iteration film weight of each of n pixel columns (or rows)."""
+    out = np.zeros(n, np.float32)
+    lib().orc_expected_weights(n, _p(out, _f32p))
+    return out
 
 
 def report_samples(film, width, height, st, colors):

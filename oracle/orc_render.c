@@ -399,8 +399,8 @@ static float to_area(ovec from, ovec to, const orc_frame *tf)
 }
 
 /* BdptIntegrator::random_walk, integrators.rs:138-229.  prev_pdf: the
- * solid-angle density with which the first ray's direction was sampled (0 on
- * the camera side, where it is never needed). */
+ * solid-angle density with which the first ray's direction was sampled (the
+ * camera's density of its ray, or the light's emission density). */
 static void random_walk(const orc_scene *s, ovec o, ovec d, ovec T, int c2l, bvert *st, int *len,
                         int max_len, uint64_t seed, uint64_t path, uint32_t stream, float prev_pdf)
 {
@@ -421,15 +421,17 @@ static void random_walk(const orc_scene *s, ovec o, ovec d, ovec T, int c2l, bve
             int conn = orc_count_lobes(m, LOBE_DIFFUSE | LOBE_GLOSSY) != 0;
             T = v_mul(T, v_scale(smp.radiance, fabsf(v_dot(f.normal, smp.outgoing)) / smp.pdf));
             T = v_scale(T, correct_shading_normal(wi, smp.outgoing, &f, c2l));
-            o = v_add(p, v_scale(smp.outgoing, ORC_RAY_PUSH_DIST));
-            d = smp.outgoing;
-            float a_fwd = 0.0f;
+            float a_fwd;
             if (*len > 0) {
                 bvert *pv = &st[*len - 1];
                 pv->pdf_reverse = pdf_rev * convert_density(pv->point, p, &f);
                 pv->a_rev = pdf_rev * to_area(p, pv->point, &pv->props);
                 a_fwd = prev_pdf * to_area(pv->point, p, &f);
+            } else {
+                a_fwd = prev_pdf * to_area(o, p, &f);  /* the camera's first vertex, from the camera */
             }
+            o = v_add(p, v_scale(smp.outgoing, ORC_RAY_PUSH_DIST));
+            d = smp.outgoing;
             bvert *v = &st[(*len)++];
             v->incoming = wi; v->point = p; v->props = f; v->throughput = prev_T;
             v->emission = smp.emission; v->lobe_kind = smp.kind; v->connectible = conn;
@@ -491,19 +493,72 @@ static float emit_dir_pdf(const orc_frame *f, ovec dir)
 
 static float remap0(float p) { return p != 0.0f ? p : 1.0f; }
 
+/* The pinhole camera as the light-tracing strategy sees it: its position
+ * (Xform::transform of the origin), the rows of the transform's 3x3 part
+ * (camera-space axes in world space) and the window half-extents. */
+typedef struct {
+    ovec e, ax, ay, az;
+    float wx, wy;
+    uint32_t width, height;
+} lt_camera;
+
+/* Solid-angle density of one camera ray through a window point whose
+ * camera-space direction makes cos_theta with the view axis: sample points
+ * cover the film with density 1 per pixel area and iteration (a pixel's
+ * point is uniform over the 4x4 pixels around it), i.e. 1 / (W H) per path
+ * over the window of area (2 wx)(2 wy) at distance 1, whose solid angle
+ * element is cos^3 theta dA. */
+static float camera_dir_pdf(float wx, float wy, float cos_theta)
+{
+    return 1.0f / (((4.0f * wx) * wy) * ((cos_theta * cos_theta) * cos_theta));
+}
+
+/* Projection of a world point seen from the camera: the film position
+ * (col_cont - 0.5, row_cont - 0.5) as report_samples computes it for a
+ * sample, the unit direction from the point to the camera, the distance and
+ * cos theta.  0 when the point is behind the camera or its splat (the pixels
+ * within the filter width) misses the film. */
+static int lt_project(const lt_camera *c, ovec y, float *cd, float *rd, ovec *to_cam, float *dist, float *cos_t)
+{
+    const ovec d = v_sub(y, c->e);
+    const float len = v_mag(d);
+    if (!(len > 0.0f)) return 0;
+    const ovec u = v_div(d, len);
+    const float dx = v_dot(u, c->ax), dy = v_dot(u, c->ay), dz = v_dot(u, c->az);
+    if (!(dz < 0.0f)) return 0;
+    const float s = (dx / -dz) / c->wx, t = (dy / -dz) / c->wy;
+    const float wf = (float)c->width, hf = (float)c->height;
+    const float col_cont = lerpf(0.0f, wf, 0.5f * (s + 1.0f));
+    const float row_cont = lerpf(0.0f, hf, 0.5f * (t + 1.0f));
+    *cd = col_cont - 0.5f;
+    *rd = row_cont - 0.5f;
+    if (!(floorf(*cd + FILTER_WIDTH) >= 0.0f && ceilf(*cd - FILTER_WIDTH) <= wf - 1.0f &&
+          floorf(*rd + FILTER_WIDTH) >= 0.0f && ceilf(*rd - FILTER_WIDTH) <= hf - 1.0f))
+        return 0;
+    *to_cam = v_neg(u);
+    *dist = len;
+    *cos_t = -dz / sqrtf(dx * dx + dy * dy + dz * dz);
+    return 1;
+}
+
 /* Power-heuristic MIS weight of strategy (c, l) among the strategies that
  * can make the same path z_1..z_k (k = c + l; z_i = camera vertex i for
  * i <= c, z_{c+m} = light vertex l-m+1): j camera vertices and k-j light
  * vertices, j = 1..k, within the depth caps, connecting two connectible
  * vertices (j < k) or hitting the emitter (j = k), with the light point z_k
  * on a light.  p_j = prod_{i<=j} pF(z_i) prod_{i>j} pL(z_i); the weight is
- * p_c^2 / sum_j p_j^2 from the ratios p_{j-1}/p_j = pL(z_j)/pF(z_j). */
+ * p_c^2 / sum_j p_j^2 from the ratios p_{j-1}/p_j = pL(z_j)/pF(z_j).
+ * With a light-tracing camera (ltc != NULL) j = 0 is a strategy too (all k
+ * vertices from the light walk, z_1 joined to the camera, which counts as
+ * connectible; pF(z_1) is the camera's density), and c = 0 is allowed:
+ * z_1 = y_l seen directly by the camera. */
 static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam, const bvert *lig, int cmax,
-                             int lmax)
+                             int lmax, const lt_camera *ltc)
 {
     const int k = c + l;
     float pf[2 * BDPT_MAX_DEPTH + 2], pl[2 * BDPT_MAX_DEPTH + 2];
     int conn[2 * BDPT_MAX_DEPTH + 2];
+    conn[0] = 1;
     for (int i = 1; i <= c; ++i) {
         pf[i] = cam[i - 1].a_fwd;
         pl[i] = cam[i - 1].a_rev;
@@ -515,9 +570,21 @@ static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam,
         pl[c + m] = y->a_fwd;
         conn[c + m] = y->connectible;
     }
-    const bvert *xc = &cam[c - 1];
     int emissive;
-    if (l == 0) {
+    if (c == 0) {  /* light tracing: y_l joined to the camera */
+        emissive = 1;
+        const bvert *yl = &lig[l - 1];
+        float cd, rd, dist, cos_t;
+        ovec d_yc;
+        if (!lt_project(ltc, yl->point, &cd, &rd, &d_yc, &dist, &cos_t)) return 0.0f;
+        pf[1] = camera_dir_pdf(ltc->wx, ltc->wy, cos_t) * to_area(ltc->e, yl->point, &yl->props);
+        if (l >= 2) {
+            const orc_material *my = &s->prims[yl->prim].mat;
+            const bvert *yp = &lig[l - 2];
+            pf[2] = orc_pdf_world(my, d_yc, yl->incoming, &yl->props) * to_area(yl->point, yp->point, &yp->props);
+        }
+    } else if (l == 0) {
+        const bvert *xc = &cam[c - 1];
         emissive = xc->prim != ORC_NONE && s->prims[xc->prim].mat.has_light;
         if (!emissive) return 0.0f;
         pl[k] = light_area_pdf(s, xc->prim);
@@ -529,6 +596,7 @@ static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam,
         }
     } else {
         emissive = 1;
+        const bvert *xc = &cam[c - 1];
         const bvert *yl = &lig[l - 1];
         const orc_material *mc = &s->prims[xc->prim].mat;
         const ovec d_cy = v_normalized(v_sub(yl->point, xc->point)), d_yc = v_neg(d_cy);
@@ -547,9 +615,10 @@ static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam,
         }
     }
     float sum = 0.0f, r = 1.0f;
+    const int jmin = ltc ? 0 : 1;
 #define ORC_MIS_VALID(j) ((j) <= cmax && k - (j) <= lmax && emissive && ((j) == k || (conn[j] && conn[(j) + 1])))
     if (ORC_MIS_VALID(c)) sum += 1.0f;
-    for (int j = c - 1; j >= 1; --j) {
+    for (int j = c - 1; j >= jmin; --j) {
         r = r * (remap0(pl[j + 1]) / remap0(pf[j + 1]));
         if (ORC_MIS_VALID(j)) sum += r * r;
     }
@@ -562,15 +631,51 @@ static float bdpt_mis_weight(const orc_scene *s, int c, int l, const bvert *cam,
     return sum > 0.0f ? 1.0f / sum : 0.0f;
 }
 
-static ovec bdpt_integrate(const orc_scene *s, ovec o, ovec d, uint64_t seed, uint64_t path,
-                           uint32_t max_depth, uint32_t light_depth, int mis)
+/* One light-tracing contribution: a light vertex's radiance toward the
+ * camera, to be splatted at film position (cd, rd). */
+typedef struct {
+    float cd, rd;
+    ovec color;
+} lt_record;
+
+/* The light-tracing strategy (c = 0) for light vertex y: radiance carried by
+ * the light subpath to y and scattered toward the camera, T f |n.w| / d^2,
+ * times the camera's density of that direction (the importance of a pinhole
+ * whose film is sampled uniformly), occluded or not, times its MIS weight. */
+static int light_trace(const orc_scene *s, int l, const bvert *cam, const bvert *lig, int cmax, int lmax,
+                       const lt_camera *ltc, lt_record *out)
+{
+    const bvert *y = &lig[l - 1];
+    if (y->prim == ORC_NONE || !y->connectible) return 0;
+    float cd, rd, dist, cos_t;
+    ovec to_cam;
+    if (!lt_project(ltc, y->point, &cd, &rd, &to_cam, &dist, &cos_t)) return 0;
+    const orc_material *m = &s->prims[y->prim].mat;
+    ovec f = l == 1 ? orc_light_world(m, to_cam, &y->props) : orc_f_world(m, y->incoming, to_cam, &y->props, 0);
+    float g = convert_density(ltc->e, y->point, &y->props);
+    ovec contrib = v_scale(v_mul(y->throughput, f), g * camera_dir_pdf(ltc->wx, ltc->wy, cos_t));
+    if (v_is_nearly_zero(contrib)) return 0;
+    if (orc_visibility_one(s, ltc->e, y->point, NULL) != 1) return 0;
+    float w = bdpt_mis_weight(s, 0, l, cam, lig, cmax, lmax, ltc);
+    out->cd = cd;
+    out->rd = rd;
+    out->color = v_scale(contrib, w);
+    return 1;
+}
+
+/* lt: the light-tracing camera (NULL: camera strategies only); lt_out gets
+ * up to lmax light-tracing records, *lt_n their count. */
+static ovec bdpt_integrate(const orc_scene *s, ovec o, ovec d, float cam_pdf, uint64_t seed, uint64_t path,
+                           uint32_t max_depth, uint32_t light_depth, int mis, const lt_camera *lt,
+                           lt_record *lt_out, int *lt_n)
 {
     bvert cam[BDPT_MAX_DEPTH], lig[BDPT_MAX_DEPTH];
     int nc = 0, nl = 0;
     int cmax = (max_depth == 0 || max_depth > BDPT_MAX_DEPTH) ? BDPT_MAX_DEPTH : (int)max_depth;
     int lmax = (light_depth == 0 || light_depth + 1 > BDPT_MAX_DEPTH) ? BDPT_MAX_DEPTH : (int)light_depth + 1;
     ovec L = v_zero();
-    random_walk(s, o, d, v_one(), 1, cam, &nc, cmax, seed, path, ORC_STREAM_CAMERA_PATH, 0.0f);
+    if (lt_n) *lt_n = 0;
+    random_walk(s, o, d, v_one(), 1, cam, &nc, cmax, seed, path, ORC_STREAM_CAMERA_PATH, cam_pdf);
     if (nc == 0 || s->num_lights == 0) return L;
     light_sample ls = sample_light(s, seed, path, ORC_STREAM_LIGHT_PATH, 0, 1);
     if (ls.point_pdf == 0.0f || ls.dir_pdf == 0.0f) return L;
@@ -591,12 +696,72 @@ static ovec bdpt_integrate(const orc_scene *s, ovec o, ovec d, uint64_t seed, ui
             if (c + l > BDPT_MAX_DEPTH) continue;
             ovec con = bdpt_connect(s, c, l, cam, lig);
             if (mis && v_is_exactly_zero(con)) continue;
-            float w = mis ? bdpt_mis_weight(s, c, l, cam, lig, cmax, lmax)
+            float w = mis ? bdpt_mis_weight(s, c, l, cam, lig, cmax, lmax, lt)
                           : 1.0f / (float)(c + l); /* weight, integrators.rs:345-354 */
             L = v_add(L, v_scale(con, w));
         }
     }
+    if (lt) {
+        for (int l = 1; l <= nl; ++l)
+            if (light_trace(s, l, cam, lig, cmax, lmax, lt, &lt_out[*lt_n])) ++*lt_n;
+    }
     return L;
+}
+
+/* Adds a light-tracing record to the per-pixel splat sums with the film's
+ * Mitchell weights over the pixels within the filter width (the footprint
+ * report_samples gives a sample at the same position).  Each pixel's sum
+ * estimates the filter-weighted radiance over the film per iteration. */
+static void lt_splat(float *splat, uint32_t w, uint32_t h, const lt_record *r)
+{
+    int64_t last_col = (int64_t)w - 1, last_row = (int64_t)h - 1;
+    int64_t min_col = clamp_i(f2i(ceilf(r->cd - FILTER_WIDTH)), 0, last_col);
+    int64_t max_col = clamp_i(f2i(floorf(r->cd + FILTER_WIDTH)), 0, last_col);
+    int64_t min_row = clamp_i(f2i(ceilf(r->rd - FILTER_WIDTH)), 0, last_row);
+    int64_t max_row = clamp_i(f2i(floorf(r->rd + FILTER_WIDTH)), 0, last_row);
+    for (int64_t y = min_row; y < max_row + 1; ++y) {
+        for (int64_t x = min_col; x < max_col + 1; ++x) {
+            float *px = &splat[3 * (y * (int64_t)w + x)];
+            float wt = mitchell2((float)x - r->cd, (float)y - r->rd);
+            px[0] = px[0] + r->color.x * wt;
+            px[1] = px[1] + r->color.y * wt;
+            px[2] = px[2] + r->color.z * wt;
+        }
+    }
+}
+
+/* Antiderivative of the 1-D Mitchell kernel k(t) = mitchell_filter1(t / 2)
+ * (B = C = 1/3, support |t| <= 2, integral 1), odd, in double. */
+static double mitchell_integral(double t)
+{
+    const double a = fabs(t);
+    double r;
+    if (a <= 1.0) {
+        r = (1.75 * a * a * a * a - 4.0 * a * a * a + (16.0 / 3.0) * a) / 6.0;
+    } else {
+        const double b = a < 2.0 ? a : 2.0;
+#define ORC_MITCHELL_F(u) (-(7.0 / 12.0) * (u) * (u) * (u) * (u) + 4.0 * (u) * (u) * (u) - 10.0 * (u) * (u) + (32.0 / 3.0) * (u))
+        r = (1.75 - 4.0 + 16.0 / 3.0) / 6.0 + (ORC_MITCHELL_F(b) - ORC_MITCHELL_F(1.0)) / 6.0;
+#undef ORC_MITCHELL_F
+    }
+    return t < 0.0 ? -r : r;
+}
+
+/* Expected film weight one iteration's samples give pixel column x of n
+ * (rows alike): the sample of pixel q lies uniformly within q +- 2, so
+ * E = (1/4) sum_q int_{-2}^{2} k(x - q - j) dj -- 1 inside the film, less
+ * within two pixels of its edges.  A pixel's expectation is the product of
+ * its column's and its row's. */
+void orc_expected_weights(uint32_t n, float *out)
+{
+    for (int64_t x = 0; x < (int64_t)n; ++x) {
+        double e = 0.0;
+        for (int64_t q = x - 4; q <= x + 4; ++q) {
+            if (q < 0 || q >= (int64_t)n) continue;
+            e += mitchell_integral((double)(x - q) + 2.0) - mitchell_integral((double)(x - q) - 2.0);
+        }
+        out[x] = (float)(0.25 * e);
+    }
 }
 
 /* ---- render driver (Stage::trace, src/render/stage.rs:23-36) -------------- */
@@ -612,6 +777,32 @@ int orc_render(const orc_scene *s, const float cam[16], float wx, float wy, uint
     float *col = (float *)malloc(sizeof(float) * 3 * npx);
     if (!st || !col) { free(st); free(col); return -1; }
     if (nthreads <= 0) nthreads = 1;
+    /* light tracing (ORC_BDPT_LT): per path up to lmax splat records, summed
+     * per pixel in path order after each iteration */
+    const int lt_on = integrator == ORC_BDPT_LT;
+    const int lmax = (light_depth == 0 || light_depth + 1 > BDPT_MAX_DEPTH) ? BDPT_MAX_DEPTH : (int)light_depth + 1;
+    lt_camera ltc;
+    lt_record *ltr = NULL;
+    int *ltn = NULL;
+    float *splat = NULL, *ew_col = NULL, *ew_row = NULL;
+    if (lt_on) {
+        ltc.e = v_new(cam[12] / cam[15], cam[13] / cam[15], cam[14] / cam[15]);
+        ltc.ax = v_new(cam[0], cam[1], cam[2]);
+        ltc.ay = v_new(cam[4], cam[5], cam[6]);
+        ltc.az = v_new(cam[8], cam[9], cam[10]);
+        ltc.wx = wx; ltc.wy = wy; ltc.width = width; ltc.height = height;
+        ltr = (lt_record *)malloc(sizeof(lt_record) * npx * (size_t)lmax);
+        ltn = (int *)malloc(sizeof(int) * npx);
+        splat = (float *)malloc(sizeof(float) * 3 * npx);
+        ew_col = (float *)malloc(sizeof(float) * width);
+        ew_row = (float *)malloc(sizeof(float) * height);
+        if (!ltr || !ltn || !splat || !ew_col || !ew_row) {
+            free(st); free(col); free(ltr); free(ltn); free(splat); free(ew_col); free(ew_row);
+            return -1;
+        }
+        orc_expected_weights(width, ew_col);
+        orc_expected_weights(height, ew_row);
+    }
     for (uint32_t it = 0; it < spp; ++it) {
         uint64_t iter = sample_offset + it;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
@@ -623,19 +814,42 @@ int orc_render(const orc_scene *s, const float cam[16], float wx, float wy, uint
             st[2 * i] = ss; st[2 * i + 1] = tt;
             ovec o, d;
             camera_ray(cam, wx, wy, ss, tt, &o, &d);
+            const ovec cdir = v_normalized(v_new(wx * ss, wy * tt, -1.0f));
+            const float cam_pdf = camera_dir_pdf(wx, wy, -cdir.z);
             ovec L;
             if (integrator == 0) L = pt_integrate(s, o, d, seed, path, max_depth);
             else if (integrator == 1) L = nee_integrate(s, o, d, seed, path, max_depth);
             else if (integrator == ORC_DISPLAY) L = display_integrate(s, o, d);
-            else if (integrator == ORC_BDPT_MIS) L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth, 1);
-            else L = bdpt_integrate(s, o, d, seed, path, max_depth, light_depth, 0);
+            else if (integrator == ORC_BDPT_MIS || integrator == ORC_BDPT_LT)
+                L = bdpt_integrate(s, o, d, cam_pdf, seed, path, max_depth, light_depth, 1, lt_on ? &ltc : NULL,
+                                   lt_on ? &ltr[(size_t)i * lmax] : NULL, lt_on ? &ltn[i] : NULL);
+            else L = bdpt_integrate(s, o, d, cam_pdf, seed, path, max_depth, light_depth, 0, NULL, NULL, NULL);
             col[3 * i] = L.x; col[3 * i + 1] = L.y; col[3 * i + 2] = L.z;
         }
         if (colors) memcpy(&colors[3 * npx * it], col, sizeof(float) * 3 * npx);
-        if (film) report_samples(film, width, height, st, col, npx);
+        if (lt_on) {
+            memset(splat, 0, sizeof(float) * 3 * npx);
+            for (uint64_t i = 0; i < npx; ++i)
+                for (int r = 0; r < ltn[i]; ++r) lt_splat(splat, width, height, &ltr[i * lmax + r]);
+        }
+        if (film) {
+            report_samples(film, width, height, st, col, npx);
+            if (lt_on) {
+                /* this iteration's light-traced estimate of the pixel's filtered
+                 * radiance, weighted by the film weight the iteration's camera
+                 * samples give the pixel in expectation */
+                for (uint64_t p = 0; p < npx; ++p) {
+                    const float ew = ew_col[p % width] * ew_row[p / width];
+                    film[4 * p] = film[4 * p] + ew * splat[3 * p];
+                    film[4 * p + 1] = film[4 * p + 1] + ew * splat[3 * p + 1];
+                    film[4 * p + 2] = film[4 * p + 2] + ew * splat[3 * p + 2];
+                }
+            }
+        }
     }
     free(st);
     free(col);
+    free(ltr); free(ltn); free(splat); free(ew_col); free(ew_row);
     return 0;
 }
 

@@ -123,7 +123,7 @@ def test_material_functions_match_oracle(gpu_ctx):
 
 
 @pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT),
-                                          ("bdpt_mis", O.BDPT_MIS)])
+                                          ("bdpt_mis", O.BDPT_MIS), ("bdpt_lt", O.BDPT_LT)])
 def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
     """Same seed -> the same paths: per-sample radiance agrees."""
     print("case paired", integrator, flush=True)
@@ -141,7 +141,7 @@ def test_paired_samples_match_oracle(gpu_ctx, integrator, oi):
 
 
 @pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT),
-                                          ("bdpt_mis", O.BDPT_MIS)])
+                                          ("bdpt_mis", O.BDPT_MIS), ("bdpt_lt", O.BDPT_LT)])
 def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
     """Per-pixel means of GPU and oracle (different seeds) within 3 sigma."""
     print("case 3sigma", integrator, flush=True)
@@ -242,3 +242,59 @@ def test_bdpt_mis_through_a_mirror_matches_path_tracing(gpu_ctx):
     print("z mis %.2f uniform %.2f" % (z_mis, z_uni), flush=True)
     assert abs(z_mis) < 4.0, z_mis
     assert z_uni < -5.0, z_uni
+
+
+def _lambert_mirror_box():
+    spec = scenes.box_scene()
+    for p in spec.prims:
+        if p.material[0] == 0:
+            p.material = scenes.diffuse()
+    spec.prims[6].material = scenes.mirror()  # the inner block
+    return spec
+
+
+def _interior(film_rgbw, W, H):
+    f = np.asarray(film_rgbw, np.float64).reshape(H, W, 4)
+    return (f[..., :3] / f[..., 3:4])[2:-2, 2:-2]
+
+
+def test_light_tracing_film_matches_oracle(gpu_ctx):
+    """One iteration per batch, same seed: the GPU film with light tracing
+    (camera strategies through the film gather, light vertices splatted with
+    float atomics) equals the oracle's up to the splats' summation order."""
+    print("case bdpt_lt film vs oracle", flush=True)
+    spec = _lambert_mirror_box()
+    W, H, spp = 40, 32, 4
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    trace(stage, film, "bdpt_lt", spp=spp, max_depth=6, light_depth=5, seed=13, batch=1)
+    g = film.readback().astype(np.float64)
+    o = O.OracleScene(spec)
+    r, _ = O.render(o, spec, W, H, O.BDPT_LT, spp=spp, max_depth=6, light_depth=5, seed=13)
+    r = np.asarray(r, np.float64).reshape(-1, 4)
+    assert np.array_equal(g[:, 3], r[:, 3])  # the camera samples' weights: bit-exact
+    close = np.all(np.isclose(g[:, :3], r[:, :3], rtol=1e-3, atol=1e-5), axis=1)
+    assert close.mean() > 0.99, close.mean()
+    np.testing.assert_allclose(g[:, :3].sum(0), r[:, :3].sum(0), rtol=1e-3)
+
+
+def test_light_tracing_through_a_mirror_matches_path_tracing(gpu_ctx):
+    """The GPU film with light tracing (8 iterations per batch) estimates the
+    same image as GPU path tracing: interior means of independent films agree."""
+    print("case bdpt_lt mirror vs pt", flush=True)
+    spec = _lambert_mirror_box()
+    W, H = 32, 32
+    stage = Stage(spec, ctx=gpu_ctx)
+    def means(integrator, spp, seeds):
+        out = []
+        for seed in seeds:
+            film = Film(gpu_ctx, W, H)
+            trace(stage, film, integrator, spp=spp, seed=seed)
+            out.append(_interior(film.readback(), W, H).sum(2).mean())
+            film.close()
+        return np.array(out)
+    a = means("pt", 512, range(1, 9))
+    b = means("bdpt_lt", 128, range(21, 29))
+    z = abs(a.mean() - b.mean()) / np.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
+    print("pt %.5f bdpt_lt %.5f z %.2f" % (a.mean(), b.mean(), z), flush=True)
+    assert z < 4.0, z

@@ -375,3 +375,65 @@ def test_bdpt_mis_has_lower_variance_than_uniform_weights():
     _, uni = O.render(o, s, W, H, O.BDPT, spp=128, seed=4, colors=True)
     vm, vu = mis.sum(2).var(0).sum(), uni.sum(2).var(0).sum()
     assert vm < 0.8 * vu, (vm, vu)
+
+
+def _film_interior_means(s, oi, spp, seeds, W=16, H=16):
+    """Per seed: the film image's mean over pixels at least two from the edge
+    (summed channels)."""
+    o = O.OracleScene(s)
+    out = []
+    for seed in seeds:
+        f, _ = O.render(o, s, W, H, oi, spp=spp, seed=seed)
+        f = np.asarray(f, np.float64).reshape(H, W, 4)
+        out.append((f[..., :3] / f[..., 3:4])[2:-2, 2:-2].sum(2).mean())
+    return np.array(out)
+
+
+@pytest.mark.parametrize("scene", ["lambert", "disney", "mirror"])
+def test_bdpt_light_tracing_is_unbiased_against_unbounded_pt(scene):
+    """Light tracing splats light vertices onto the film; with the MIS weights
+    extended to it (j = 0) the film still estimates the unbounded PT image --
+    on Lambertian, pdf-consistent Disney and mirror-block boxes (the mirror's
+    caustics are where light tracing carries most weight)."""
+    s = {"lambert": _lambert_box, "disney": lambda: scenes.box_scene(consistent=True), "mirror": _mirror_box}[scene]()
+    a = _film_interior_means(s, O.PT, 256, range(1, 9))
+    b = _film_interior_means(s, O.BDPT_LT, 64, range(11, 19))
+    z = abs(a.mean() - b.mean()) / math.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
+    assert z < 4.0, (a.mean(), b.mean(), z)
+
+
+def test_light_tracing_lowers_variance_with_caustics():
+    """Through the mirror block light reaches the walls along paths camera
+    subpaths rarely find; light tracing lowers the film's per-pixel variance
+    there below the camera strategies' MIS."""
+    s = _mirror_box()
+    o = O.OracleScene(s)
+    W = H = 24
+    def var(oi):
+        ims = []
+        for seed in range(6):
+            f, _ = O.render(o, s, W, H, oi, spp=64, seed=40 + seed)
+            f = np.asarray(f, np.float64).reshape(H, W, 4)
+            ims.append((f[..., :3] / f[..., 3:4])[2:-2, 2:-2].sum(2))
+        return np.array(ims).var(0, ddof=1).mean()
+    v_mis, v_lt = var(O.BDPT_MIS), var(O.BDPT_LT)
+    assert v_lt < 0.9 * v_mis, (v_lt, v_mis)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 9, 37])
+def test_expected_film_weights_match_the_sample_distribution(n):
+    """The light-traced estimate is weighted by the film weight the camera
+    samples give a pixel in expectation: 1 inside the film (the Mitchell
+    kernel integrates to 1), less near the edges.  Pinned against the mean
+    weight of 4000 iterations of sample points splatted with unit colour."""
+    e = O.expected_weights(n)
+    if n >= 9:
+        np.testing.assert_allclose(e[4:-4], 1.0, atol=1e-6)
+    W, H = n, 3
+    film = np.zeros((W * H, 4), np.float32)
+    for it in range(4000):
+        st = O.sample_points(5, it, W, H)
+        O.report_samples(film, W, H, st, np.ones((W * H, 3), np.float32))
+    w = film[:, 3].reshape(H, W).astype(np.float64) / 4000
+    want = np.outer(O.expected_weights(H), e)
+    np.testing.assert_allclose(w, want, atol=0.02)
