@@ -1640,23 +1640,30 @@ __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, 
         nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
         for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) n += min(nl, kBdptMaxDepth - c) + 1;
         if (b.mis == 2 && !(nl & kLightFailed)) nlt = nl;  // light tracing: (0, l), l = 1..nl
-        n += nlt;
     }
     uint32_t at = block_reserve(n, &a.rb.counts[4]);
     for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) {
         const uint32_t lm = min(nl, kBdptMaxDepth - c);
         for (uint32_t l = 0; l <= lm; ++l) b.list[at++] = ((c - 1) * (b.lmax + 1) + l) * chunk + q;
     }
-    for (uint32_t l = 1; l <= nlt; ++l) b.list[at++] = (b.lt_base + l - 1) * chunk + q;
+    if (b.mis == 2) {
+        // light tracing's strategies in a list of their own (after the connection
+        // strategies' region), so that every wave of an evaluation takes one branch
+        uint32_t lt = block_reserve(nlt, &a.rb.counts[5]);
+        for (uint32_t l = 1; l <= nlt; ++l) b.list[(size_t)b.lt_base * chunk + lt++] = (b.lt_base + l - 1) * chunk + q;
+    }
 }
 
 // One lane per listed strategy (grid-stride over the list, whole blocks per
-// step): evaluates connect(c, l) without its visibility test, stores
+// step): evaluates connect(c, l) -- or, over light tracing's list (lt = 1),
+// the light vertex seen by the camera -- without its visibility test, stores
 // contribution * weight in the strategy's slot, and queues the shadow segment
 // of every strategy that needs one (wave-compacted).
-__global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
+__global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1,
+                                                      uint32_t lt) {
     const uint32_t chunk = p1 - p0;
-    const uint32_t n = a.rb.counts[4];
+    const uint32_t n = a.rb.counts[lt ? 5 : 4];
+    const uint32_t* list = b.list + (lt ? (size_t)b.lt_base * chunk : 0);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {  // block-uniform steps
         const uint32_t j = base + threadIdx.x;
@@ -1664,7 +1671,7 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
         uint32_t p = 0, s = 0;
         Connection cn;
         if (j < n) {
-            const uint32_t t = b.list[j];
+            const uint32_t t = list[j];
             s = t / chunk;
             p = p0 + (t - s * chunk);
             vec3 cw;
@@ -1746,10 +1753,14 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
         const size_t p1 = std::min(npaths, p0 + chunk);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 2u);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 4u);
+        if (mis == 2) hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, 5u);
         hipLaunchKernelGGL(k_connect_list, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
         // grid-stride over the list: enough waves to fill the chip, never more than the list can need
         const dim3 eval_grid(std::min<unsigned>(blocks_for((p1 - p0) * nstrat).x, 8192u));
-        hipLaunchKernelGGL(k_connect_eval, eval_grid, dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
+        hipLaunchKernelGGL(k_connect_eval, eval_grid, dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1, 0u);
+        if (mis == 2)
+            hipLaunchKernelGGL(k_connect_eval, dim3(std::min<unsigned>(blocks_for((p1 - p0) * lmax).x, 8192u)),
+                               dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1, 1u);
         e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
                             vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
