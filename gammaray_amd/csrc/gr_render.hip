@@ -1655,15 +1655,17 @@ __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, 
 }
 
 // One lane per listed strategy (grid-stride over the list, whole blocks per
-// step): evaluates connect(c, l) -- or, over light tracing's list (lt = 1),
-// the light vertex seen by the camera -- without its visibility test, stores
+// step): evaluates connect(c, l) -- or, over light tracing's list (LT), the
+// light vertex seen by the camera -- without its visibility test, stores
 // contribution * weight in the strategy's slot, and queues the shadow segment
-// of every strategy that needs one (wave-compacted).
-__global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1,
-                                                      uint32_t lt) {
+// of every strategy that needs one (wave-compacted).  One instance per list
+// keeps each instance's registers to its own branch.
+template <bool LT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_connect_eval(RenderArgs a, BdptArgs b,
+                                                                                            uint32_t p0, uint32_t p1) {
     const uint32_t chunk = p1 - p0;
-    const uint32_t n = a.rb.counts[lt ? 5 : 4];
-    const uint32_t* list = b.list + (lt ? (size_t)b.lt_base * chunk : 0);
+    const uint32_t n = a.rb.counts[LT ? 5 : 4];
+    const uint32_t* list = b.list + (LT ? (size_t)b.lt_base * chunk : 0);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {  // block-uniform steps
         const uint32_t j = base + threadIdx.x;
@@ -1675,7 +1677,7 @@ __global__ __launch_bounds__(256) void k_connect_eval(RenderArgs a, BdptArgs b, 
             s = t / chunk;
             p = p0 + (t - s * chunk);
             vec3 cw;
-            if (s >= b.lt_base) {  // light tracing: light vertex l seen by the camera
+            if (LT) {  // light tracing: light vertex l seen by the camera
                 const uint32_t l = s - b.lt_base + 1;
                 cn = light_trace(a, b, p, l);
                 cw = dev::is_zero(cn.contrib) ? cn.contrib : cn.contrib * mis_weight(a, b, p, 0, l);
@@ -1757,10 +1759,10 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
         hipLaunchKernelGGL(k_connect_list, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
         // grid-stride over the list: enough waves to fill the chip, never more than the list can need
         const dim3 eval_grid(std::min<unsigned>(blocks_for((p1 - p0) * nstrat).x, 8192u));
-        hipLaunchKernelGGL(k_connect_eval, eval_grid, dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1, 0u);
+        hipLaunchKernelGGL(k_connect_eval<false>, eval_grid, dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
         if (mis == 2)
-            hipLaunchKernelGGL(k_connect_eval, dim3(std::min<unsigned>(blocks_for((p1 - p0) * lmax).x, 8192u)),
-                               dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1, 1u);
+            hipLaunchKernelGGL(k_connect_eval<true>, dim3(std::min<unsigned>(blocks_for((p1 - p0) * lmax).x, 8192u)),
+                               dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
         e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
                             vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
