@@ -2009,47 +2009,49 @@ struct BatchDev {
     const JobDev* jobs;
     uint32_t njobs, pool;
     unsigned long long total;   // samples of all jobs
-    uint4* slot;         // per slot: job, sample within the job, depth, unused (one 16-B record)
-    float2* slot_st;     // per slot: film position (read when the path retires)
+    // Path state by queue position, one copy per queue side: shade_path's
+    // throughput, radiance and MIS state, the path's job, sample within the job
+    // and depth (x, y, z), and its film position.  Each bounce moves the
+    // survivors' state to their positions in the next queue, so every access
+    // is by queue position (coalesced) and a finished path's position is free
+    // again at once.
+    float4* T[2];
+    float4* L[2];
+    float4* aux[2];
+    uint4* meta[2];
+    float2* st[2];
     const unsigned long long* job_first;  // per job: index of its first sample (a compact search table)
-    uint32_t* free_list;
-    uint32_t* retire;
-    // [0] free slots (the free list is a stack: entries [0, ctr[0]) are free),
-    // [1] paths started this step, [2] first free-list entry they take,
-    // [3] their first queue position, [4] paths retired this step (their
-    // slots go onto the stack at ctr[0] + their retire index; the next plan
-    // adds them to the count)
+    uint32_t* retire;    // queue positions (current side) of the paths that finished this step
+    // [1] paths started this step, [3] their first queue position, [4] paths retired this step
     uint32_t* ctr;
     unsigned long long* issued;  // [0] samples started so far, [1] first sample of this step
     uint32_t* done;              // per job: kDoneSpread counters of finished samples (summed by the host)
 };
 constexpr uint32_t kDoneSpread = 64;  // spreads the counter updates of one job over 64 addresses
+constexpr uint32_t kNextSide = 0x80000000u;  // shadow-ray owner flag: the path's state is in the next queue
 
 __global__ void k_batch_init(RenderArgs a, BatchDev b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < b.pool) b.free_list[i] = i;
     if (i < b.njobs * kDoneSpread) b.done[i] = 0u;
     if (i == 0) {
-        b.ctr[0] = b.pool;
-        b.ctr[1] = b.ctr[2] = b.ctr[3] = b.ctr[4] = 0u;
+        b.ctr[1] = b.ctr[3] = b.ctr[4] = 0u;
         b.issued[0] = b.issued[1] = 0ull;
         a.rb.counts[0] = a.rb.counts[1] = a.rb.counts[2] = 0u;
     }
 }
 
-// How many new paths this step starts (free slots, samples left) and where
-// they go; clears the step's other queues.
+// How many new paths this step starts (room in the queue after the paths
+// that continue, samples left) and where they go; clears the step's other
+// queues.
 __global__ void k_batch_plan(RenderArgs a, BatchDev b, uint32_t cur) {
     if (threadIdx.x != 0) return;
-    const uint32_t freec = b.ctr[0] + b.ctr[4];
+    const uint32_t q = a.rb.counts[cur];  // continuing paths at queue positions [0, q)
     const unsigned long long issued = b.issued[0];
     const unsigned long long rem = b.total - issued;
-    const uint32_t k = rem < (unsigned long long)freec ? (uint32_t)rem : freec;
-    const uint32_t q = a.rb.counts[cur];
+    const uint32_t room = b.pool - q;
+    const uint32_t k = rem < (unsigned long long)room ? (uint32_t)rem : room;
     b.ctr[1] = k;
-    b.ctr[2] = freec - k;
     b.ctr[3] = q;
-    b.ctr[0] = freec - k;
     b.ctr[4] = 0u;
     b.issued[1] = issued;
     b.issued[0] = issued + k;
@@ -2058,11 +2060,10 @@ __global__ void k_batch_plan(RenderArgs a, BatchDev b, uint32_t cur) {
     a.rb.counts[2] = 0u;
 }
 
-// New camera paths in free slots, appended after the continuing rays.
+// New camera paths appended after the continuing ones.
 __global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.ctr[1]) return;
-    const uint32_t slot = b.free_list[b.ctr[2] + i];
     const unsigned long long sidx = b.issued[1] + i;
     uint32_t lo = 0, hi = b.njobs - 1;  // last job whose first sample <= sidx
     while (lo < hi) {
@@ -2076,74 +2077,93 @@ __global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, u
     const CameraSample c =
         camera_sample(J.cam, J.wx, J.wy, J.width, J.height, J.seed, J.path_base0 + local, px / J.width, px % J.width);
     const uint32_t q = b.ctr[3] + i;
-    a.rb.queue[cur][q] = slot;
     a.rb.rays[cur][2 * (size_t)q] = make_float4(c.o.x, c.o.y, c.o.z, __uint_as_float(J.scene));
     a.rb.rays[cur][2 * (size_t)q + 1] = make_float4(c.d.x, c.d.y, c.d.z, 0.0f);
-    a.rb.T[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    a.rb.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    a.rb.aux[slot] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-    b.slot[slot] = make_uint4(lo, local, 0u, 0u);
-    b.slot_st[slot] = c.st;
+    b.T[cur][q] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.L[cur][q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.aux[cur][q] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);  // previous bsdf pdf, previous bounce specular
+    b.meta[cur][q] = make_uint4(lo, local, 0u, 0u);
+    b.st[cur][q] = c.st;
 }
 
 // One bounce of every queued path, each with its own job's scene, seed and
-// depth; the shading itself is shade_path, as in the per-film wavefront.
+// depth; the shading itself is shade_path, as in the per-film wavefront (a's
+// T / L / aux are this side's arrays, indexed by queue position).  Survivors
+// move with their state to the next queue; finished paths are listed for
+// retirement; a shadow ray records where its path's radiance now lives.
 template <bool NEE>
 __global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.rb.counts[cur];
+    const uint32_t nxt = cur ^ 1u;
     ShadeOut out;
     out.alive = out.shadow = false;
-    uint32_t slot = 0, scene = 0;
+    uint32_t scene = 0;
+    uint4 m = make_uint4(0u, 0u, 0u, 0u);
     bool retire = false;
     if (i < n) {
-        slot = a.rb.queue[cur][i];
         const uint4 hv = a.rb.hits[i];
         const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
         const float4 ro = a.rb.rays[cur][2 * (size_t)i];
         const vec3 o = ld3(ro), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
         scene = __float_as_uint(ro.w);
-        const uint4 st = b.slot[slot];
-        const JobDev& J = b.jobs[st.x];
-        const uint32_t depth = st.z;
-        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + st.y, slot, depth, h, o, d, out);
-        if (out.alive) b.slot[slot].z = depth + 1u;
+        m = b.meta[cur][i];
+        const JobDev& J = b.jobs[m.x];
+        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + m.y, i, m.z, h, o, d, out);
         retire = !out.alive;
     }
-    const uint32_t nxt = cur ^ 1u;
     const uint32_t q = block_append(out.alive, &a.rb.counts[nxt]);
     if (out.alive) {
-        a.rb.queue[nxt][q] = slot;
         a.rb.rays[nxt][2 * (size_t)q] = make_float4(out.new_o.x, out.new_o.y, out.new_o.z, __uint_as_float(scene));
         a.rb.rays[nxt][2 * (size_t)q + 1] = make_float4(out.new_d.x, out.new_d.y, out.new_d.z, 0.0f);
+        b.T[nxt][q] = b.T[cur][i];
+        b.L[nxt][q] = b.L[cur][i];
+        b.aux[nxt][q] = b.aux[cur][i];
+        b.meta[nxt][q] = make_uint4(m.x, m.y, m.z + 1u, 0u);
+        b.st[nxt][q] = b.st[cur][i];
     }
     const uint32_t r = block_append(retire, &b.ctr[4]);
-    if (retire) b.retire[r] = slot;
+    if (retire) b.retire[r] = i;
     if (NEE) {
         const uint32_t ss = block_append(out.shadow, &a.rb.counts[2]);
         if (out.shadow) {
             a.rb.seg[2 * (size_t)ss] = make_float4(out.hp.x, out.hp.y, out.hp.z, __uint_as_float(scene));
             a.rb.seg[2 * (size_t)ss + 1] = make_float4(out.seg_to.x, out.seg_to.y, out.seg_to.z, 0.0f);
             a.rb.contrib[ss] = make_float4(out.c_nee.x, out.c_nee.y, out.c_nee.z, 0.0f);
-            a.rb.seg_path[ss] = slot;
+            a.rb.seg_path[ss] = out.alive ? (q | kNextSide) : i;
         }
     }
 }
 
+// The light samples that reached their lights, added to their paths'
+// radiance (in the next queue for survivors, in this one for finished paths).
+__global__ __launch_bounds__(256) void k_batch_shadow_accum(RenderArgs a, BatchDev b, uint32_t cur) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.rb.counts[2] || a.rb.vis[j] == 0) return;
+    const uint32_t owner = a.rb.seg_path[j];
+    float4* Ls = (owner & kNextSide) ? b.L[cur ^ 1u] : b.L[cur];
+    const uint32_t p = owner & ~kNextSide;
+    const float4 c = a.rb.contrib[j];
+    float4 L = Ls[p];
+    L.x = L.x + c.x;
+    L.y = L.y + c.y;
+    L.z = L.z + c.z;
+    Ls[p] = L;
+}
+
 // Finished paths (their last light sample already added): radiance and film
-// position to the job, slot back to the free list.
-__global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) {
+// position to the job.
+__global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < b.ctr[4];
-    uint32_t slot = 0, j = kNone;
+    uint32_t j = kNone;
     if (ok) {
-        slot = b.retire[i];
-        const uint4 st = b.slot[slot];
-        j = st.x;
+        const uint32_t p = b.retire[i];
+        const uint4 m = b.meta[cur][p];
+        j = m.x;
         const JobDev& J = b.jobs[j];
-        const uint32_t local = st.y;
-        J.L[local] = a.rb.L[slot];
-        J.st[local] = b.slot_st[slot];
+        J.L[m.y] = b.L[cur][p];
+        J.st[m.y] = b.st[cur][p];
     }
     // one counter update per distinct job in the wave (finished paths come in
     // job order, so a wave usually holds one or two jobs)
@@ -2156,7 +2176,6 @@ __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b) 
             atomicAdd(&b.done[lj * kDoneSpread + blockIdx.x % kDoneSpread], (uint32_t)__popcll(same));
         left &= ~same;
     }
-    if (ok) b.free_list[b.ctr[0] + i] = slot;  // no atomic: the retire index is the stack position
 }
 
 }  // namespace
@@ -2220,10 +2239,14 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     const size_t o_scenes = off; off = align(off + scenes.size() * sizeof(SceneRef));
     const size_t o_L = off; off = align(off + total * sizeof(float4));
     const size_t o_st = off; off = align(off + total * sizeof(float2));
-    const size_t o_slot = off; off = align(off + (size_t)pool * 16);
-    const size_t o_sst = off; off = align(off + (size_t)pool * 8);
+    // the second queue side's path state (the first side uses the film buffers' T, L, aux, st)
+    const size_t o_T1 = off; off = align(off + (size_t)pool * 16);
+    const size_t o_L1 = off; off = align(off + (size_t)pool * 16);
+    const size_t o_aux1 = off; off = align(off + (size_t)pool * 16);
+    const size_t o_st1 = off; off = align(off + (size_t)pool * 8);
+    const size_t o_meta0 = off; off = align(off + (size_t)pool * 16);
+    const size_t o_meta1 = off; off = align(off + (size_t)pool * 16);
     const size_t o_first = off; off = align(off + (size_t)njobs * 8);
-    const size_t o_free = off; off = align(off + (size_t)pool * 4);
     const size_t o_ret = off; off = align(off + (size_t)pool * 4);
     const size_t o_ctr = off; off = align(off + 64);
     const size_t o_iss = off; off = align(off + 16);
@@ -2275,10 +2298,17 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     b.njobs = njobs;
     b.pool = pool;
     b.total = total;
-    b.slot = reinterpret_cast<uint4*>(blob + o_slot);
-    b.slot_st = reinterpret_cast<float2*>(blob + o_sst);
+    b.T[0] = rb.T;
+    b.L[0] = rb.L;
+    b.aux[0] = rb.aux;
+    b.st[0] = rb.st;
+    b.T[1] = reinterpret_cast<float4*>(blob + o_T1);
+    b.L[1] = reinterpret_cast<float4*>(blob + o_L1);
+    b.aux[1] = reinterpret_cast<float4*>(blob + o_aux1);
+    b.st[1] = reinterpret_cast<float2*>(blob + o_st1);
+    b.meta[0] = reinterpret_cast<uint4*>(blob + o_meta0);
+    b.meta[1] = reinterpret_cast<uint4*>(blob + o_meta1);
     b.job_first = reinterpret_cast<const unsigned long long*>(blob + o_first);
-    b.free_list = reinterpret_cast<uint32_t*>(blob + o_free);
     b.retire = reinterpret_cast<uint32_t*>(blob + o_ret);
     b.ctr = reinterpret_cast<uint32_t*>(blob + o_ctr);
     b.issued = reinterpret_cast<unsigned long long*>(blob + o_iss);
@@ -2287,25 +2317,29 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     TraceSetup ts{scenes[0]->view, tu, &ss, nullptr, reinterpret_cast<const SceneRef*>(blob + o_scenes),
                   (uint32_t)scenes.size()};
     auto step = [&](uint32_t cur) -> hipError_t {
+        RenderArgs ac = a;  // shade_path's state arrays: this queue side's
+        ac.rb.T = b.T[cur];
+        ac.rb.L = b.L[cur];
+        ac.rb.aux = b.aux[cur];
         hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, s, a, b, cur);
         hipLaunchKernelGGL(k_batch_regen, blocks_for(pool), dim3(256), 0, s, a, b, cur);
         hipError_t r = launch_intersect(ts, reinterpret_cast<const gr_ray*>(rb.rays[cur]), pool, rb.counts + cur,
                                         reinterpret_cast<gr_hit*>(rb.hits), nullptr, s);
         if (r != hipSuccess) return r;
         if (nee) {
-            hipLaunchKernelGGL((k_batch_shade<true>), blocks_for(pool), dim3(256), 0, s, a, b, cur);
+            hipLaunchKernelGGL((k_batch_shade<true>), blocks_for(pool), dim3(256), 0, s, ac, b, cur);
             r = launch_occluded(ts, reinterpret_cast<const gr_segment*>(rb.seg), pool, rb.counts + 2, rb.vis,
                                 GR_OCCLUSION_EXACT, nullptr, s);
             if (r != hipSuccess) return r;
-            hipLaunchKernelGGL((k_shadow_accum<false>), blocks_for(pool), dim3(256), 0, s, a, 0u);
+            hipLaunchKernelGGL(k_batch_shadow_accum, blocks_for(pool), dim3(256), 0, s, a, b, cur);
         } else {
-            hipLaunchKernelGGL((k_batch_shade<false>), blocks_for(pool), dim3(256), 0, s, a, b, cur);
+            hipLaunchKernelGGL((k_batch_shade<false>), blocks_for(pool), dim3(256), 0, s, ac, b, cur);
         }
-        hipLaunchKernelGGL(k_batch_retire, blocks_for(pool), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(k_batch_retire, blocks_for(pool), dim3(256), 0, s, a, b, cur);
         return hipGetLastError();
     };
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(pool, njobs * kDoneSpread)), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(1u, njobs * kDoneSpread)), dim3(256), 0, s, a, b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev_start, s);

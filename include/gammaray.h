@@ -249,10 +249,10 @@ int gr_film_write_exr(gr_film *film, const char *path);
  * job, src/bin/gammaray.rs:85-112) -------------------------------------------
  * Many jobs -- each a scene, camera, film, spp and depth cap -- run through one
  * shared pool of in-flight paths.  One bounce step serves every job at once:
- * free path slots take the next samples from a global counter over all jobs,
- * one closest-hit launch traverses every job's scene (each ray names its
- * scene), shading and shadow rays follow, finished paths hand their radiance
- * to their job and free their slot.  The step is captured once as a hipGraph
+ * the room the continuing paths leave in the queue takes the next samples
+ * from a global counter over all jobs, one closest-hit launch traverses every
+ * job's scene (each ray names its scene), shading and shadow rays follow,
+ * finished paths hand their radiance to their job.  The step is captured once as a hipGraph
  * and replayed.  A job's film gets the Mitchell splat of its samples, in the
  * reference's order, as soon as its last sample finishes; it equals what
  * gr_trace renders for the same camera, seed, spp, depth and sample_offset. */
