@@ -27,6 +27,19 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 COPY_PROBE_BYTES = 1 << 29  # 512 MiB, read once and written once
+# HBM bytes per launch of the traversal kernels on the default workload
+# (icosphere level 8, 16,777,216 rays / segments, rays in input order, 8 LDS
+# stack entries, refill at 32 idle lanes), from two `rocprofv3 --pmc` runs of
+# `bench.py --traffic-probe`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the
+# factors being what the counters read for the probe's 512 MiB device copy.
+# Counters cannot be read inside this process, so the figure is reported only
+# when the timed launches run that exact configuration.
+MEASURED_TRAFFIC = {
+    "closest": (8347886.5 * 2.0 + 607525.7) * 1024,
+    "shadow": (10841658.0 * 2.0 + 383625.3) * 1024,
+}
+MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=32, waves_per_cu=0,
+                                xcd_ranges=0)
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
 NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
 
@@ -440,8 +453,13 @@ def main():
     ms_dom = k_closest if dominant == "closest" else k_shadow
     bytes_dom = bytes_closest if dominant == "closest" else bytes_shadow
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
+    setting = dict(level=args.level, rays=n, **{k: tuning.get(k) for k in
+                                                ("sort_bits", "lds_stack", "refill", "waves_per_cu", "xcd_ranges")})
+    traffic_dom = None
+    if setting == MEASURED_TRAFFIC_SETTING and args.occlusion == "exact":
+        traffic_dom = int(MEASURED_TRAFFIC[dominant])
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_dom,
                     kernel="k_trace<%s>" % ("closest" if dominant == "closest" else "occlusion"),
                     algorithmic_bytes_per_launch=int(bytes_dom),
                     kernel_ms=round(ms_dom, 4),
