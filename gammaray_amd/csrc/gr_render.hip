@@ -909,8 +909,14 @@ constexpr int kTile = 16, kApron = 4, kSpan = kTile + 2 * kApron;
 // covers) are computed once, when the tile stages it in LDS, with the same
 // arithmetic the splat uses; each pixel then only compares its coordinates
 // with the staged bounds and weights the samples that cover it.
+// Light-traced splats are summed as 64-bit fixed point (units of 2^-32) so
+// that the per-pixel sums do not depend on the order in which the atomics
+// land: a batch's image is the same on every run.  Range: |sum| < 2^31 per
+// pixel and batch.
+constexpr double kSplatScale = 4294967296.0;
+
 __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters,
-                                                              const float4* splat, const float* ew) {
+                                                              const long long* splat, const float* ew) {
     __shared__ float s_cd[kSpan * kSpan], s_rd[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
         s_b[kSpan * kSpan];
     __shared__ int4 s_box[kSpan * kSpan];  // min_col, max_col, min_row, max_row
@@ -969,7 +975,10 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
         // the batch's camera samples give the pixel in expectation (per iteration:
         // its column's expectation times its row's)
         const float e = ew[x] * ew[W + y];
-        const float4 sp = splat[(size_t)y * W + x];
+        const long long* q = splat + 4 * ((size_t)y * W + x);
+        const float4 sp = make_float4((float)((double)q[0] * (1.0 / kSplatScale)),
+                                      (float)((double)q[1] * (1.0 / kSplatScale)),
+                                      (float)((double)q[2] * (1.0 / kSplatScale)), 0.0f);
         acc.x = acc.x + e * sp.x;
         acc.y = acc.y + e * sp.y;
         acc.z = acc.z + e * sp.z;
@@ -1031,7 +1040,7 @@ struct BdptArgs {
     uint32_t* list;      // the chunk's valid strategies, as s * chunk + (path - p0)
     uint32_t mis;        // 0: the reference's 1/(c+l); 1: power-heuristic MIS; 2: MIS + light tracing
     float4* slots;       // per (strategy, path): contribution * weight, needs-visibility flag
-    float4* splat;       // light tracing: per pixel, the batch's splatted radiance (xyz)
+    long long* splat;    // light tracing: per pixel, the batch's splatted radiance (x, y, z, unused) in kSplatScale units
     float* ew;           // light tracing: expected film weight per column (width) then per row (height)
     uint32_t cmax, lmax, mwords;
     uint32_t lt_base;    // strategy index of light tracing's l = 1: cmax (lmax + 1)
@@ -1426,11 +1435,18 @@ __device__ Connection light_trace(const RenderArgs& a, const BdptArgs& b, uint32
     return r;
 }
 
+// Fixed-point value of a splatted radiance component (0 for a non-finite one).
+__device__ __forceinline__ unsigned long long splat_fixed(float v) {
+    if (!isfinite(v)) return 0ull;
+    const double d = rint((double)v * kSplatScale);
+    const double lim = 9.2e18;
+    return (unsigned long long)(long long)(d > lim ? lim : (d < -lim ? -lim : d));
+}
+
 // Splats the visible light-tracing contributions of paths [p0, p1) onto the
 // batch's splat buffer with the film's Mitchell weights over the pixels
 // within the filter width (the footprint a camera sample at the same film
-// position has).  Float atomics: the per-pixel sums do not keep the paths'
-// order.
+// position has), as fixed-point integer atomics (order-independent sums).
 __global__ __launch_bounds__(256) void k_lt_splat(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= p1) return;
@@ -1453,10 +1469,10 @@ __global__ __launch_bounds__(256) void k_lt_splat(RenderArgs a, BdptArgs b, uint
             const float wy = mitchell1(((float)y - rd) / 2.0f);
             for (int x = x0; x <= x1; ++x) {
                 const float wt = mitchell1(((float)x - cd) / 2.0f) * wy;
-                float* px = reinterpret_cast<float*>(b.splat + (size_t)y * W + x);
-                unsafeAtomicAdd(px + 0, v.x * wt);
-                unsafeAtomicAdd(px + 1, v.y * wt);
-                unsafeAtomicAdd(px + 2, v.z * wt);
+                unsigned long long* px = reinterpret_cast<unsigned long long*>(b.splat + 4 * ((size_t)y * W + x));
+                atomicAdd(px + 0, splat_fixed(v.x * wt));
+                atomicAdd(px + 1, splat_fixed(v.y * wt));
+                atomicAdd(px + 2, splat_fixed(v.z * wt));
             }
         }
     }
@@ -1593,7 +1609,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     const size_t o_vis = off; off = align(off + shadow * 4);
     const size_t o_slots = off; off = align(off + nslots * sizeof(float4));
     const size_t o_list = off; off = align(off + nslots * sizeof(uint32_t));
-    const size_t o_splat = off; off = align(off + npx * sizeof(float4));
+    const size_t o_splat = off; off = align(off + npx * 4 * sizeof(long long));
     const size_t o_ew = off; off = align(off + nfilm_lines * sizeof(float));
     if (rb.bdpt_capacity < off) {
         (void)hipFree(rb.bdpt_blob);
@@ -1612,7 +1628,7 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
     b->entry = reinterpret_cast<uint2*>(base + o_entry);
     b->slots = reinterpret_cast<float4*>(base + o_slots);
     b->list = reinterpret_cast<uint32_t*>(base + o_list);
-    b->splat = reinterpret_cast<float4*>(base + o_splat);
+    b->splat = reinterpret_cast<long long*>(base + o_splat);
     b->ew = reinterpret_cast<float*>(base + o_ew);
     b->lt_base = cmax * (lmax + 1);
     b->cmax = cmax;
@@ -1723,7 +1739,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     if (e != hipSuccess) return e;
     b.mis = mis;
     if (mis == 2) {
-        e = hipMemsetAsync(b.splat, 0, npx * sizeof(float4), s);
+        e = hipMemsetAsync(b.splat, 0, npx * 4 * sizeof(long long), s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_expected_weights, blocks_for(a.width + a.height), dim3(256), 0, s, a.width, a.height,
                            b.ew);

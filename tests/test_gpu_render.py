@@ -260,8 +260,9 @@ def _interior(film_rgbw, W, H):
 
 def test_light_tracing_film_matches_oracle(gpu_ctx):
     """One iteration per batch, same seed: the GPU film with light tracing
-    (camera strategies through the film gather, light vertices splatted with
-    float atomics) equals the oracle's up to the splats' summation order."""
+    (camera strategies through the film gather, light vertices splatted as
+    fixed-point sums) equals the oracle's (float sums in path order) up to
+    the splats' rounding."""
     print("case bdpt_lt film vs oracle", flush=True)
     spec = _lambert_mirror_box()
     W, H, spp = 40, 32, 4
@@ -298,3 +299,24 @@ def test_light_tracing_through_a_mirror_matches_path_tracing(gpu_ctx):
     z = abs(a.mean() - b.mean()) / np.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
     print("pt %.5f bdpt_lt %.5f z %.2f" % (a.mean(), b.mean(), z), flush=True)
     assert z < 4.0, z
+
+
+def test_light_tracing_films_are_deterministic_and_resume_bit_exact(gpu_ctx):
+    """Light-traced splats are summed in fixed point, so the film does not
+    depend on the order of the atomics: the same render twice is bit-identical,
+    and 3 + checkpoint + 3 iterations equal 6 (one iteration per batch)."""
+    print("case bdpt_lt deterministic", flush=True)
+    spec = _lambert_mirror_box()
+    W, H = 40, 32
+    stage = Stage(spec, ctx=gpu_ctx)
+    def render(spp, offset=0, film=None):
+        f = film or Film(gpu_ctx, W, H)
+        trace(stage, f, "bdpt_lt", spp=spp, max_depth=6, light_depth=5, seed=17, sample_offset=offset, batch=1)
+        return f
+    a = render(6).readback()
+    b = render(6).readback()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    c = Film(gpu_ctx, W, H)
+    c.upload(render(3).readback())
+    render(3, offset=3, film=c)
+    assert np.array_equal(a.view(np.uint32), c.readback().view(np.uint32))
