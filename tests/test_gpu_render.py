@@ -320,3 +320,21 @@ def test_light_tracing_films_are_deterministic_and_resume_bit_exact(gpu_ctx):
     c.upload(render(3).readback())
     render(3, offset=3, film=c)
     assert np.array_equal(a.view(np.uint32), c.readback().view(np.uint32))
+
+
+def test_trace_rejects_unknown_integrators_and_lightless_bdpt(gpu_ctx):
+    from gammaray_amd._lib import GrError
+    print("case integrator validation", flush=True)
+    stage = Stage(scenes.box_scene(), ctx=gpu_ctx)
+    film = Film(gpu_ctx, 8, 8)
+    with pytest.raises(GrError):
+        trace(stage, film, 7, spp=1)
+    dark = scenes.SceneSpec(name="dark")
+    v, f = scenes.quad((0.0, 0.0, -5.0), (5.0, 0.0, 0.0), (0.0, 5.0, 0.0), (0.0, 0.0, 1.0))
+    dark.add_mesh(v, f, scenes.diffuse())
+    dstage = Stage(dark, ctx=gpu_ctx)
+    for integ in ("pt_nee", "bdpt", "bdpt_mis", "bdpt_lt"):
+        with pytest.raises(GrError):
+            trace(dstage, film, integ, spp=1)
+    trace(dstage, film, "pt", spp=1)  # path tracing needs no light to sample
+    assert np.all(film.readback()[..., :3] == 0.0)
