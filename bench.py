@@ -29,16 +29,16 @@ HBM_PEAK_GBS = 8000.0
 COPY_PROBE_BYTES = 1 << 29  # 512 MiB, read once and written once
 # HBM bytes per launch of the traversal kernels on the default workload
 # (icosphere level 8, 16,777,216 rays / segments, rays in input order, 8 LDS
-# stack entries, refill at 32 idle lanes), from two `rocprofv3 --pmc` runs of
+# stack entries, refill at 48 idle lanes), from two `rocprofv3 --pmc` runs of
 # `bench.py --traffic-probe`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the
 # factors being what the counters read for the probe's 512 MiB device copy.
 # Counters cannot be read inside this process, so the figure is reported only
 # when the timed launches run that exact configuration.
 MEASURED_TRAFFIC = {
-    "closest": (8347886.5 * 2.0 + 607525.7) * 1024,
-    "shadow": (10841658.0 * 2.0 + 383625.3) * 1024,
+    "closest": (8294568.1 * 2.0 + 608756.0) * 1024,
+    "shadow": (10839984.3 * 2.0 + 380996.4) * 1024,
 }
-MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=32, waves_per_cu=0,
+MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=48, waves_per_cu=0,
                                 xcd_ranges=0)
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
 NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
