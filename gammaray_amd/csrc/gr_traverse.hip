@@ -183,6 +183,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     // XCD's range first, then helps with the following ones.
     uint32_t part = blockIdx.x % parts;
     uint32_t drained = 0;
+    // the current range [r_lo, r_lo + r_len), wave-uniform; recomputed only when
+    // the wave moves on to the next range (a 64-bit division when parts > 1)
+    uint32_t r_lo = 0, r_len = n;
+    auto set_range = [&]() {
+        if (parts > 1) {
+            r_lo = __builtin_amdgcn_readfirstlane((uint32_t)(((uint64_t)n * part) / parts));
+            r_len = __builtin_amdgcn_readfirstlane((uint32_t)(((uint64_t)n * (part + 1)) / parts)) - r_lo;
+        }
+    };
+    set_range();
     bool active = false;
     bool exhausted = false;
     uint32_t ray = 0, cur = 0, best = kNone;
@@ -248,8 +258,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             const uint64_t idle = __ballot(!active);
             const int nidle = __popcll(idle);
             if ((uint32_t)nidle >= refill) {
-                const uint32_t lo = (uint32_t)(((uint64_t)n * part) / parts);
-                const uint32_t len = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - lo;
+                const uint32_t lo = r_lo, len = r_len;
                 uint32_t first = 0;
                 if (lane == 0) first = atomicAdd(next_ray + part, (uint32_t)nidle);
                 first = __shfl(first, 0);
@@ -257,6 +266,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                     // this range is drained (counters only grow): go to the next
                     if (++drained == parts) exhausted = true;
                     part = part + 1 == parts ? 0u : part + 1;
+                    set_range();
                 }
                 if (!active) {
                     const uint32_t k = first + (uint32_t)__popcll(idle & lanes_below());
