@@ -240,3 +240,22 @@ def test_stack_overflow_is_reported(gpu_ctx):
     ok = ~g_ovf & ~o_ovf
     assert np.array_equal(g["prim"][ok], r["prim"][ok])
     assert np.array_equal(g["dist"][ok].view(np.uint32), r["dist"][ok].view(np.uint32))
+
+
+def test_empty_inputs_are_no_ops(gpu_ctx):
+    """Zero rays, zero segments, zero jobs and zero iterations: success, no
+    launch, nothing written (null buffers are accepted when n = 0)."""
+    print("case empty inputs", flush=True)
+    from gammaray_amd.api import Film, render_jobs, trace
+    from gammaray_amd._lib import GrError
+    spec = scenes.box_scene()
+    stage = Stage(spec, ctx=gpu_ctx)
+    stage.intersect_ptr(0, 0, 0)
+    stage.occluded_ptr(0, 0, 0)
+    with pytest.raises(GrError):
+        stage.intersect_ptr(0, 4, 0)  # rays without buffers
+    film = Film(gpu_ctx, 16, 8)
+    trace(stage, film, "pt", spp=0)
+    assert np.all(film.readback() == 0.0)
+    assert render_jobs(gpu_ctx, [], "pt_nee") == []
+    gpu_ctx.synchronize()
