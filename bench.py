@@ -38,6 +38,11 @@ MEASURED_TRAFFIC = {
     "closest": (8294568.1 * 2.0 + 608756.0) * 1024,
     "shadow": (10839984.3 * 2.0 + 380996.4) * 1024,
 }
+# TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) of the same launches (a third counter run)
+MEASURED_L2_HIT = {
+    "closest": 137438862.0 / (137438862.0 + 142896216.0),
+    "shadow": 222080269.0 / (222080269.0 + 179851997.0),
+}
 MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=48, waves_per_cu=0,
                                 xcd_ranges=0)
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
@@ -455,11 +460,12 @@ def main():
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
     setting = dict(level=args.level, rays=n, **{k: tuning.get(k) for k in
                                                 ("sort_bits", "lds_stack", "refill", "waves_per_cu", "xcd_ranges")})
-    traffic_dom = None
+    traffic_dom = l2_hit = None
     if setting == MEASURED_TRAFFIC_SETTING and args.occlusion == "exact":
         traffic_dom = int(MEASURED_TRAFFIC[dominant])
+        l2_hit = round(MEASURED_L2_HIT[dominant], 4)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_dom,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_dom, l2_hit_rate=l2_hit,
                     kernel="k_trace<%s>" % ("closest" if dominant == "closest" else "occlusion"),
                     algorithmic_bytes_per_launch=int(bytes_dom),
                     kernel_ms=round(ms_dom, 4),
