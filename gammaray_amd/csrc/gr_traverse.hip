@@ -167,10 +167,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                                                   const uint32_t* __restrict__ n_ptr, uint32_t refill,
                                                   uint32_t parts) {
     constexpr int kSpill = kStackSize - LDS;
-    __shared__ uint32_t s_ref[LDS][kWave];
-    __shared__ float s_t[LDS][kWave];
-    uint32_t spill_ref[kSpill];
-    float spill_t[kSpill];
+    // stack entries: (node ref, entry distance bits), one 8-B word per entry
+    __shared__ uint2 s_e[LDS][kWave];
+    uint2 spill[kSpill];
     const uint32_t lane = threadIdx.x;
     // A ray count that earlier kernels on the stream produced (queue length).
     if (n_ptr) {
@@ -234,18 +233,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     auto pop = [&]() -> bool {
         while (sp > 0) {
             --sp;
-            uint32_t ref;
-            float t;
+            uint2 e;
             if (sp < base) {
-                ref = spill_ref[sp];
-                t = spill_t[sp];
+                e = spill[sp];
                 base = sp;
             } else {
-                ref = s_ref[sp & (LDS - 1)][lane];
-                t = s_t[sp & (LDS - 1)][lane];
+                e = s_e[sp & (LDS - 1)][lane];
             }
-            if (t < closest) {
-                cur = ref;
+            if (__uint_as_float(e.y) < closest) {
+                cur = e.x;
                 return true;
             }
         }
@@ -354,12 +350,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                     break;
                 }
                 if (sp - base == LDS) {
-                    spill_ref[base] = s_ref[base & (LDS - 1)][lane];
-                    spill_t[base] = s_t[base & (LDS - 1)][lane];
+                    spill[base] = s_e[base & (LDS - 1)][lane];
                     ++base;
                 }
-                s_ref[sp & (LDS - 1)][lane] = far_ref;
-                s_t[sp & (LDS - 1)][lane] = far_t;
+                s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
                 ++sp;
             }
             if (near_ok && near_t < closest) {
