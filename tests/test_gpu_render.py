@@ -196,7 +196,7 @@ def _image(film_rgbw):
     return f[:, :3] / f[:, 3:4]
 
 
-@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE)])
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
 def test_rmse_against_cpu_4096spp_reference_agrees(gpu_ctx, integrator, oi):
     """GPU and CPU 64-spp renders have the same RMSE against a 4096-spp CPU
     render, within 2 %.  Both use one seed (the same addressed random numbers),
@@ -207,11 +207,11 @@ def test_rmse_against_cpu_4096spp_reference_agrees(gpu_ctx, integrator, oi):
     spec = scenes.box_scene(consistent=True)
     W, H = 64, 64
     o = O.OracleScene(spec)
-    ref, _ = O.render(o, spec, W, H, oi, spp=4096, max_depth=6, seed=23)
-    cpu, _ = O.render(o, spec, W, H, oi, spp=64, max_depth=6, seed=21)
+    ref, _ = O.render(o, spec, W, H, oi, spp=4096, max_depth=6, light_depth=5, seed=23)
+    cpu, _ = O.render(o, spec, W, H, oi, spp=64, max_depth=6, light_depth=5, seed=21)
     stage = Stage(spec, ctx=gpu_ctx)
     film = Film(gpu_ctx, W, H)
-    trace(stage, film, integrator, spp=64, max_depth=6, seed=21)
+    trace(stage, film, integrator, spp=64, max_depth=6, light_depth=5, seed=21)
     R = _image(ref)
     e_cpu = np.sqrt(np.mean((_image(cpu) - R) ** 2))
     e_gpu = np.sqrt(np.mean((_image(film.readback()) - R) ** 2))
