@@ -338,3 +338,27 @@ def test_trace_rejects_unknown_integrators_and_lightless_bdpt(gpu_ctx):
             trace(dstage, film, integ, spp=1)
     trace(dstage, film, "pt", spp=1)  # path tracing needs no light to sample
     assert np.all(film.readback()[..., :3] == 0.0)
+
+
+@pytest.mark.parametrize("integrator,oi,spp", [("pt_nee", O.PT_NEE, 2), ("bdpt", O.BDPT, 1),
+                                               ("bdpt_mis", O.BDPT_MIS, 1)])
+def test_full_size_1080p_paired_samples(gpu_ctx, integrator, oi, spp):
+    """The 1080p configuration at full size (100k-triangle mesh, 200 spheres,
+    depth 8, BDPT light depth 5): every sample's radiance against the oracle's
+    for the same seed, and the film's weights bit-exact."""
+    print("case full 1080p", integrator, flush=True)
+    spec = scenes.showcase_scene()
+    W, H = 1920, 1080
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=8, light_depth=5, seed=31)
+    o = O.OracleScene(spec)
+    f, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=8, light_depth=5, seed=31, colors=True, threads=16)
+    assert np.isfinite(g).all()
+    close = np.all(np.isclose(g, r, rtol=1e-3, atol=1e-5), axis=2)
+    assert close.mean() > 0.995, close.mean()
+    np.testing.assert_allclose(g.reshape(-1, 3).mean(0), r.reshape(-1, 3).mean(0), rtol=2e-2)
+    film2 = Film(gpu_ctx, W, H)
+    trace(stage, film2, integrator, spp=spp, max_depth=8, light_depth=5, seed=31)
+    gw = film2.readback().reshape(-1, 4)[:, 3]
+    assert np.array_equal(gw.view(np.uint32), np.asarray(f, np.float32).reshape(-1, 4)[:, 3].view(np.uint32))
