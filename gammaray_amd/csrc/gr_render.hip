@@ -1252,6 +1252,49 @@ __device__ Connection connect(const RenderArgs& a, const BdptArgs& b, uint32_t p
     return r;
 }
 
+// BSDF densities (solid angle) at the two joined vertices that the MIS
+// weight of a strategy with c >= 1 and l >= 1 needs, in both directions.
+struct JoinDensities {
+    float c_fwd, c_rev;  // at x_c: toward y_l given x_c's incoming direction, and back
+    float l_fwd, l_rev;  // at y_l (l >= 2): toward x_c given y_l's incoming direction, and back
+};
+
+// connect() with its two BSDF evaluations fused with the densities the MIS
+// weight needs at the same vertices (same values as the separate calls).
+__device__ Connection connect_mis(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l,
+                                  JoinDensities* jd) {
+    if (l == 0) return connect(a, b, p, c, l);
+    Connection r;
+    r.contrib = mk(0.0f, 0.0f, 0.0f);
+    r.needs_vis = false;
+    const VertView cv = load_vert(b.cam + (size_t)(c - 1) * a.npaths + p);
+    const VertView lv = load_vert(b.lig + (size_t)(l - 1) * a.npaths + p);
+    if (cv.prim == kNone || lv.prim == kNone) return r;
+    if (!cv.conn || !lv.conn) return r;
+    const vec3 c2l = normalized(lv.pt - cv.pt);
+    const vec3 l2c = -c2l;
+    const dev::FPdf fc = dev::f_pdf_world(a.sh.mats[cv.prim], cv.wi, c2l, cv.f, true);
+    jd->c_fwd = fc.pdf_fwd;
+    jd->c_rev = fc.pdf_rev;
+    vec3 ce;
+    if (l == 1) {
+        ce = dev::light_world(a.sh.mats[lv.prim], l2c, lv.f);
+        jd->l_fwd = jd->l_rev = 0.0f;
+    } else {
+        const dev::FPdf fl = dev::f_pdf_world(a.sh.mats[lv.prim], lv.wi, l2c, lv.f, false);
+        ce = fl.f;
+        jd->l_fwd = fl.pdf_fwd;
+        jd->l_rev = fl.pdf_rev;
+    }
+    const float g = fabsf(dot(cv.f.n, c2l));
+    const float cd = convert_density(cv.pt, lv.pt, lv.f);
+    r.contrib = mul(mul(mul(cv.T, fc.f), ce), lv.T) * (g * cd);
+    r.needs_vis = !dev::nearly_zero3(r.contrib);
+    r.from = cv.pt;
+    r.to = lv.pt;
+    return r;
+}
+
 __device__ __forceinline__ float remap0(float p) { return p != 0.0f ? p : 1.0f; }
 
 // Density of a light's cosine emission (Prim::sample_ray_world,
@@ -1326,7 +1369,8 @@ __device__ __forceinline__ bool lt_project(const LtCamera& c, vec3 y, float& cd,
 // are recomputed here.  With light tracing (b.mis == 2) j = 0 is a strategy
 // too (all k vertices from the light walk, z_1 joined to the camera, which
 // counts as connectible), and c = 0 is allowed: z_1 = y_l seen by the camera.
-__device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l) {
+__device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l,
+                            const JoinDensities* jd = nullptr) {
     const size_t np = a.npaths;
     const BVert* cam = b.cam + p;  // camera vertex i at cam[(i - 1) * np]
     const BVert* lig = b.lig + p;  // light vertex m at lig[(m - 1) * np]
@@ -1363,16 +1407,17 @@ __device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, 
             const VertView yl = load_vert(lig + (size_t)(l - 1) * np);
             const GpuMaterial& mc = a.sh.mats[xc.prim];
             const vec3 d_cy = normalized(yl.pt - xc.pt), d_yc = -d_cy;
-            pf_c1 = dev::pdf_world(mc, xc.wi, d_cy, xc.f) * to_area(xc.pt, yl.pt, yl.f.n);
+            pf_c1 = (jd ? jd->c_fwd : dev::pdf_world(mc, xc.wi, d_cy, xc.f)) * to_area(xc.pt, yl.pt, yl.f.n);
             if (l == 1) {
                 pl_c = emit_dir_pdf(yl.f, d_yc) * to_area(yl.pt, xc.pt, xc.f.n);
             } else {
                 const GpuMaterial& my = a.sh.mats[yl.prim];
-                pl_c = dev::pdf_world(my, yl.wi, d_yc, yl.f) * to_area(yl.pt, xc.pt, xc.f.n);
+                pl_c = (jd ? jd->l_fwd : dev::pdf_world(my, yl.wi, d_yc, yl.f)) * to_area(yl.pt, xc.pt, xc.f.n);
                 const float4 qb = lig[(size_t)(l - 2) * np].b, qc = lig[(size_t)(l - 2) * np].c;
-                pf_c2 = dev::pdf_world(my, d_yc, yl.wi, yl.f) * to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
+                pf_c2 = (jd ? jd->l_rev : dev::pdf_world(my, d_yc, yl.wi, yl.f)) *
+                        to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
             }
-            if (c >= 2) pl_c1 = dev::pdf_world(mc, d_cy, xc.wi, xc.f) * to_area(xc.pt, xp, xpn);
+            if (c >= 2) pl_c1 = (jd ? jd->c_rev : dev::pdf_world(mc, d_cy, xc.wi, xc.f)) * to_area(xc.pt, xp, xpn);
             conn_c = xc.conn;
             conn_c1 = yl.conn;
         }
@@ -1700,10 +1745,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 emit = cn.needs_vis;
             } else {
                 const uint32_t c = s / (b.lmax + 1) + 1, l = s % (b.lmax + 1);
-                cn = connect(a, b, p, c, l);
-                // weight, integrators.rs:345-354 (1/(c+l)), or the power heuristic
-                const float w =
-                    !b.mis ? 1.0f / (float)(c + l) : dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l);
+                float w;
+                if (!b.mis) {
+                    cn = connect(a, b, p, c, l);
+                    w = 1.0f / (float)(c + l);  // weight, integrators.rs:345-354
+                } else {
+                    JoinDensities jd;
+                    cn = connect_mis(a, b, p, c, l, &jd);
+                    w = dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l, l > 0 ? &jd : nullptr);
+                }
                 cw = cn.contrib * w;
                 emit = l > 0 && cn.needs_vis;
             }

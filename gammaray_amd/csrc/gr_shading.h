@@ -448,6 +448,89 @@ __device__ __forceinline__ float pdf_world(const GpuMaterial& m, vec3 wi, vec3 w
     return p / (float)m.nlobes;
 }
 
+// f_world(m, wi, wo, f, c2l), pdf_world(m, wi, wo, f) and pdf_world(m, wo, wi, f)
+// in one pass -- the values of the three calls, operation for operation, with
+// the terms they share evaluated once per lobe: the half vector, the
+// microfacet density D(h) and the masking terms of both directions
+// (BDPT's MIS weights need both densities at each joined vertex).
+struct FPdf {
+    vec3 f;
+    float pdf_fwd, pdf_rev;
+};
+__device__ __forceinline__ FPdf f_pdf_world(const GpuMaterial& m, vec3 wi, vec3 wo, const Frame& f, bool c2l) {
+    FPdf r;
+    r.f = mk(0.0f, 0.0f, 0.0f);
+    r.pdf_fwd = r.pdf_rev = 0.0f;
+    const vec3 il = to_local(wi, f), ol = to_local(wo, f);
+    const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
+    float pf = 0.0f, pr = 0.0f;
+    for (uint32_t k = 0; k < m.nlobes; ++k) {
+        const GpuLobe& l = m.lobes[k];
+        const uint32_t kind = lobe_kind(l);
+        const bool with_f = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
+        switch (l.type) {
+            case kLobeDisneySpecular:
+            case kLobeClearcoat: {
+                const vec3 hu = il + ol;
+                const bool sh = same_hemi(il, ol);
+                if (!sh && !with_f) break;
+                const vec3 h = normalized(hu);
+                const float d = refl_d(l, h);
+                float lam_i = 0.0f, lam_o = 0.0f;  // masking terms, GGX or GTR1
+                if (l.type == kLobeClearcoat) {
+                    lam_i = gtr1_lambda(il);
+                    lam_o = gtr1_lambda(ol);
+                } else {
+                    lam_i = ggx_lambda(l.v[0], l.v[1], il);
+                    lam_o = ggx_lambda(l.v[0], l.v[1], ol);
+                }
+                if (with_f) {
+                    const float ci = fabsf(il.z), co = fabsf(ol.z);
+                    if (!(is_zero(hu) || ci == 0.0f || co == 0.0f)) {
+                        const vec3 fr = refl_fresnel(l, dot(ol, h));
+                        const float g = 1.0f / (1.0f + lam_i + lam_o);
+                        r.f = r.f + mul(refl_colour(l), fr) * (d * g / (4.0f * co * ci));
+                    }
+                }
+                if (sh) {
+                    if (l.type == kLobeClearcoat) {
+                        pf += d / (4.0f * dot(il, h));
+                        pr += d / (4.0f * dot(ol, h));
+                    } else {
+                        const float gi = il.z == 0.0f ? 0.0f : d * (1.0f / (1.0f + lam_i)) * fabsf(dot(il, h)) / fabsf(il.z);
+                        const float go = ol.z == 0.0f ? 0.0f : d * (1.0f / (1.0f + lam_o)) * fabsf(dot(ol, h)) / fabsf(ol.z);
+                        pf += gi / (4.0f * dot(il, h));
+                        pr += go / (4.0f * dot(ol, h));
+                    }
+                }
+                break;
+            }
+            case kLobeSpecularTrans:
+                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
+                pf += lobe_pdf(l, il, ol);
+                pr += lobe_pdf(l, ol, il);
+                break;
+            case kLobePerfectMirror:
+                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
+                pf += 1.0f;
+                pr += 1.0f;
+                break;
+            default:  // Disney diffuse, perfect diffuse: cosine densities
+                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
+                if (same_hemi(il, ol)) {
+                    pf += fabsf(ol.z) * kInvPi;
+                    pr += fabsf(il.z) * kInvPi;
+                }
+                break;
+        }
+    }
+    if (m.nlobes != 0) {
+        if (il.z != 0.0f) r.pdf_fwd = pf / (float)m.nlobes;
+        if (ol.z != 0.0f) r.pdf_rev = pr / (float)m.nlobes;
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint32_t count_lobes(const GpuMaterial& m, uint32_t mask) {
     uint32_t c = 0;
     for (uint32_t k = 0; k < m.nlobes; ++k)
