@@ -2158,7 +2158,7 @@ __global__ __launch_bounds__(256) void k_batch_regen(RenderArgs a, BatchDev b, u
 // move with their state to the next queue; finished paths are listed for
 // retirement; a shadow ray records where its path's radiance now lives.
 template <bool NEE>
-__global__ __launch_bounds__(256) void k_batch_shade(RenderArgs a, BatchDev b, uint32_t cur) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_shade(RenderArgs a, BatchDev b, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.rb.counts[cur];
     const uint32_t nxt = cur ^ 1u;
