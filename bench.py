@@ -340,8 +340,8 @@ def main():
     if unknown:
         sys.exit("unknown path workload(s): %s" % ", ".join(sorted(unknown)))
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the film/"
-              "ray batches would shard by tiles; see the design notes). Run with --gpus 1.",
+        print("bench.py measures one GPU: the library has no multi-GPU path (a frame would shard "
+              "by pixel tiles or iteration ranges, ray batches by ranges). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
