@@ -344,7 +344,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             const bool far_ok = flip ? s0 : s1;
             const float near_t = flip ? t1 : t0;
             const float far_t = flip ? t0 : t1;
-            if (far_ok) {
+            // A far child whose entry distance is already >= closest would fail
+            // the deferred test at its pop (closest only decreases): not pushed.
+            if (far_ok && far_t < closest) {
                 if (sp == kStackSize) {
                     finish(false, true);
                     break;
