@@ -180,9 +180,11 @@ def bench_batch(ctx, threads, pool=1 << 23, steps=16):
     specs, jobs = scenes.batch_workload(seed=4)
     stages = [Stage(sp, ctx=ctx) for sp in specs]
     log("batch: 8 scenes %s triangles built (%.1f s)" % ([sp.num_components for sp in specs], time.time() - t0))
+    # the batch's launches trace closest-hit rays and NEE shadow segments: tune on both
     rays = torch.from_numpy(scenes.ray_batch(specs[-1], 1 << 21, seed=9)).cuda()
-    tuning = stages[-1].autotune(rays)
-    del rays
+    segs = torch.from_numpy(scenes.segment_batch(specs[-1], 1 << 21, seed=10)).cuda()
+    tuning = stages[-1].autotune(rays, segs)
+    del rays, segs
     for st in stages:
         st.set_tuning(**tuning)
     films = [Film(ctx, j["side"], j["side"]) for j in jobs]
