@@ -303,10 +303,14 @@ __global__ __launch_bounds__(kInferThreads) void k_guide_infer(const uint16_t* _
             head_weights_fast(la, lb, &wa, &wb);
             const size_t q = base + 16 * t + (lane & 15u);
             if (q < n) {
-                float* pa = mix + q * kMixFloats + 5 * g;
-                float* pb = pa + 5 * 4;
-                pa[0] = wa, pa[1] = la.kappa, pa[2] = la.mu.x, pa[3] = la.mu.y, pa[4] = la.mu.z;
-                pb[0] = wb, pb[1] = lb.kappa, pb[2] = lb.mu.x, pb[3] = lb.mu.y, pb[4] = lb.mu.z;
+                // the lane's two lobes as one 40-byte run (lobe order within a mixture
+                // is immaterial: slots 2g, 2g + 1 hold network lobes g, g + 4)
+                float2* p2 = reinterpret_cast<float2*>(mix + q * kMixFloats + 10 * g);
+                p2[0] = make_float2(wa, la.kappa);
+                p2[1] = make_float2(la.mu.x, la.mu.y);
+                p2[2] = make_float2(la.mu.z, wb);
+                p2[3] = make_float2(lb.kappa, lb.mu.x);
+                p2[4] = make_float2(lb.mu.y, lb.mu.z);
             }
         }
     }

@@ -306,7 +306,9 @@ void gr_guide_destroy(gr_guide *guide);
 int gr_guide_get_params(gr_guide *guide, float *host, uint32_t n);
 int gr_guide_set_params(gr_guide *guide, const float *host, uint32_t n);
 int gr_guide_steps(gr_guide *guide, uint64_t *adam_steps);
-/* Mixture of n vertices from their 8 raw inputs each (device memory, both 16-byte aligned). */
+/* Mixture of n vertices from their 8 raw inputs each (device memory, both 16-byte aligned):
+ * 40 floats per vertex, 8 lobes of (weight, kappa, mu.x, mu.y, mu.z); slot 2i holds the
+ * network's lobe i and slot 2i + 1 its lobe i + 4 (a mixture's lobe order is immaterial). */
 int gr_guide_infer(gr_guide *guide, const float *d_raw, uint32_t n, float *d_mix);
 /* Mean over n records (device) of the gradient of -(v / mean v) log q(direction). */
 int gr_guide_gradient(gr_guide *guide, const float *d_records, uint32_t n, float *d_grad);

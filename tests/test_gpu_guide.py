@@ -46,6 +46,15 @@ def _records(rng, n, dirs=None):
     return rec
 
 
+def _network_order(mix):
+    """The inference kernel writes a lane's two lobes as one run: mixture slot
+    2i holds the network's lobe i and slot 2i + 1 its lobe i + 4 (the order
+    of the lobes within a mixture does not matter to sampling or density).
+    Back to the network's order, to compare with the oracle lobe by lobe."""
+    m = np.asarray(mix).reshape(-1, 8, 5)
+    return m[:, [0, 2, 4, 6, 1, 3, 5, 7], :].reshape(-1, 40)
+
+
 def test_inference_matches_oracle(gpu_ctx):
     print("case guide inference", flush=True)
     rng = np.random.default_rng(1)
@@ -56,7 +65,7 @@ def test_inference_matches_oracle(gpu_ctx):
     raw = rng.uniform(0.0, 1.0, (4099, 8)).astype(np.float32)
     mix = g.infer(_cuda(raw)).cpu().numpy()
     ref = G.mixture_of(p.astype(np.float64), raw, f16=True)
-    w, k, mu = G.split_mixture(mix)
+    w, k, mu = G.split_mixture(_network_order(mix))
     wr, kr, mr = G.split_mixture(ref)
     np.testing.assert_allclose(w.sum(1), 1.0, atol=1e-5)
     np.testing.assert_allclose(w, wr, atol=5e-3)
@@ -87,7 +96,7 @@ def test_inference_ragged_counts(n, gpu_ctx):
     assert np.all(got[n:] == -7.0)
     sel = rng.choice(n, min(n, 512), replace=False)
     ref = G.mixture_of(p.astype(np.float64), raw[sel], f16=True)
-    w, k, mu = G.split_mixture(got[sel])
+    w, k, mu = G.split_mixture(_network_order(got[sel]))
     wr, kr, mr = G.split_mixture(ref)
     np.testing.assert_allclose(w, wr, atol=5e-3)
     np.testing.assert_allclose(k, kr, rtol=2e-2)
