@@ -255,7 +255,7 @@ def guide_inference_rate(ctx, guide, n=1 << 22, reps=5):
             "frac": round(tflops / FP16_DENSE_PEAK_TFLOPS, 4), "bound": "mfma", "dtype": "fp16 x fp16 -> fp32"}
 
 
-def bench_guided(ctx, ref_spp=1024, W=1920, H=1080, spp=64):
+def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
     """Learned path guide on the 1080p showcase scene: wall time for guided
     rendering (online training included) to reach the relative MSE that plain
     PT+NEE reaches with 64 spp, against a ref_spp PT+NEE render."""
@@ -327,7 +327,7 @@ def main():
                     help="comma-separated subset of the path workloads (%s)" % ", ".join(w[0] for w in PATH_WORKLOADS))
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
-    ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the guided comparison")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
     ap.add_argument("--guide-inference-only", action="store_true",
                     help="for profiles: only the guide inference timing (4M queries), printed as JSON")
     ap.add_argument("--tuning", default="auto",
