@@ -329,70 +329,144 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             continue;
         }
 
-        // ---- interior nodes until every lane sits on a leaf or is done
-        while (active && !(cur & kLeafBit)) {
-            const GpuNode nd = load_node(nodes, cur);
-            if (COUNT) c_box += 2;
-            float t0 = 0.0f, t1 = 0.0f;
-            const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
-            const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
-            const uint32_t axis = nd.q3.z;
-            const bool flip = ((r.sign >> axis) & 1u) != 0u;
-            const uint32_t near_ref = flip ? nd.q3.y : nd.q3.x;
-            const uint32_t far_ref = flip ? nd.q3.x : nd.q3.y;
-            const bool near_ok = flip ? s1 : s0;
-            const bool far_ok = flip ? s0 : s1;
-            const float near_t = flip ? t1 : t0;
-            const float far_t = flip ? t0 : t1;
-            // A far child whose entry distance is already >= closest would fail
-            // the deferred test at its pop (closest only decreases): not pushed.
-            if (far_ok && far_t < closest) {
-                if (sp == kStackSize) {
-                    finish(false, true);
-                    break;
-                }
-                if (sp - base == LDS) {
-                    spill[base] = s_e[base & (LDS - 1)][lane];
-                    ++base;
-                }
-                s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
-                ++sp;
-            }
-            if (near_ok && near_t < closest) {
-                cur = near_ref;
-            } else if (!pop()) {
-                finish(false, false);
-            }
-        }
-
-        // ---- leaves
-        if (active) {
-            uint32_t slot = cur & ~kLeafBit;
-            bool done = false;
-            for (;;) {
-                const GpuComp g = load_comp(comps, slot);
-                const uint32_t tag = __float_as_uint(g.q2.y);
-                float dist;
-                if (tag & kTagSphere) {
-                    if (COUNT) c_sph += 1;
-                    dist = hit_sphere(g, r);
+        if constexpr (MODE == kClosest) {
+            // ---- one step for every active lane: an interior node, or one
+            // primitive of its leaf.  Both fetch through the same four loads (a
+            // primitive record is 48 bytes; its fourth load repeats the first), so
+            // a wave whose lanes are at different kinds of nodes still waits on one
+            // round trip per step.
+            if (active) {
+                const bool at_leaf = (cur & kLeafBit) != 0;
+                const uint32_t slot = cur & ~kLeafBit;
+                const float4* p = at_leaf ? reinterpret_cast<const float4*>(comps + slot)
+                                          : reinterpret_cast<const float4*>(nodes + cur);
+                const float4 q0 = p[0], q1 = p[1], q2 = p[2];
+                const float4 q3 = *(at_leaf ? p : p + 3);
+                if (!at_leaf) {
+                    if (COUNT) c_box += 2;
+                    float t0 = 0.0f, t1 = 0.0f;
+                    const bool s0 = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
+                    const bool s1 = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
+                    const uint32_t axis = __float_as_uint(q3.z);
+                    const bool flip = ((r.sign >> axis) & 1u) != 0u;
+                    const uint32_t near_ref = __float_as_uint(flip ? q3.y : q3.x);
+                    const uint32_t far_ref = __float_as_uint(flip ? q3.x : q3.y);
+                    const bool near_ok = flip ? s1 : s0;
+                    const bool far_ok = flip ? s0 : s1;
+                    const float near_t = flip ? t1 : t0;
+                    const float far_t = flip ? t0 : t1;
+                    // A far child whose entry distance is already >= closest would fail
+                    // the deferred test at its pop (closest only decreases): not pushed.
+                    const bool push = far_ok && far_t < closest;
+                    if (push && sp == kStackSize) {
+                        finish(false, true);
+                    } else {
+                        if (push) {
+                            if (sp - base == LDS) {
+                                spill[base] = s_e[base & (LDS - 1)][lane];
+                                ++base;
+                            }
+                            s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
+                            ++sp;
+                        }
+                        if (near_ok && near_t < closest) {
+                            cur = near_ref;
+                        } else if (!pop()) {
+                            finish(false, false);
+                        }
+                    }
                 } else {
-                    if (COUNT) c_tri += 1;
-                    dist = hit_triangle(g, r);
+                    GpuComp g;
+                    g.q0 = q0;
+                    g.q1 = q1;
+                    g.q2 = q2;
+                    const uint32_t tag = __float_as_uint(g.q2.y);
+                    float dist;
+                    if (tag & kTagSphere) {
+                        if (COUNT) c_sph += 1;
+                        dist = hit_sphere(g, r);
+                    } else {
+                        if (COUNT) c_tri += 1;
+                        dist = hit_triangle(g, r);
+                    }
+                    bool done = false;
+                    if (dist != 0.0f && dist < closest) {
+                        closest = dist;
+                        best = slot;
+                        done = MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit);
+                    }
+                    if (done) finish(true, false);
+                    else if (!(tag & kTagLast)) cur = kLeafBit | (slot + 1);
+                    else if (!pop()) finish(false, false);
                 }
-                if (dist != 0.0f && dist < closest) {
-                    closest = dist;
-                    best = slot;
-                    if (MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit)) {
-                        done = true;
+            }
+        } else {
+            // Visibility segments keep the while-while loop (measured faster for them).
+            // ---- interior nodes until every lane sits on a leaf or is done
+            while (active && !(cur & kLeafBit)) {
+                const GpuNode nd = load_node(nodes, cur);
+                if (COUNT) c_box += 2;
+                float t0 = 0.0f, t1 = 0.0f;
+                const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
+                const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
+                const uint32_t axis = nd.q3.z;
+                const bool flip = ((r.sign >> axis) & 1u) != 0u;
+                const uint32_t near_ref = flip ? nd.q3.y : nd.q3.x;
+                const uint32_t far_ref = flip ? nd.q3.x : nd.q3.y;
+                const bool near_ok = flip ? s1 : s0;
+                const bool far_ok = flip ? s0 : s1;
+                const float near_t = flip ? t1 : t0;
+                const float far_t = flip ? t0 : t1;
+                // A far child whose entry distance is already >= closest would fail
+                // the deferred test at its pop (closest only decreases): not pushed.
+                if (far_ok && far_t < closest) {
+                    if (sp == kStackSize) {
+                        finish(false, true);
                         break;
                     }
+                    if (sp - base == LDS) {
+                        spill[base] = s_e[base & (LDS - 1)][lane];
+                        ++base;
+                    }
+                    s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
+                    ++sp;
                 }
-                if (tag & kTagLast) break;
-                ++slot;
+                if (near_ok && near_t < closest) {
+                    cur = near_ref;
+                } else if (!pop()) {
+                    finish(false, false);
+                }
             }
-            if (done) finish(true, false);
-            else if (!pop()) finish(false, false);
+
+            // ---- leaves
+            if (active) {
+                uint32_t slot = cur & ~kLeafBit;
+                bool done = false;
+                for (;;) {
+                    const GpuComp g = load_comp(comps, slot);
+                    const uint32_t tag = __float_as_uint(g.q2.y);
+                    float dist;
+                    if (tag & kTagSphere) {
+                        if (COUNT) c_sph += 1;
+                        dist = hit_sphere(g, r);
+                    } else {
+                        if (COUNT) c_tri += 1;
+                        dist = hit_triangle(g, r);
+                    }
+                    if (dist != 0.0f && dist < closest) {
+                        closest = dist;
+                        best = slot;
+                        if (MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit)) {
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (tag & kTagLast) break;
+                    ++slot;
+                }
+                if (done) finish(true, false);
+                else if (!pop()) finish(false, false);
+            }
         }
     }
 }
