@@ -29,21 +29,21 @@ HBM_PEAK_GBS = 8000.0
 COPY_PROBE_BYTES = 1 << 29  # 512 MiB, read once and written once
 # HBM bytes per launch of the traversal kernels on the default workload
 # (icosphere level 8, 16,777,216 rays / segments, rays in input order, 8 LDS
-# stack entries, refill at 48 idle lanes), from two `rocprofv3 --pmc` runs of
+# stack entries, refill at 56 idle lanes), from two `rocprofv3 --pmc` runs of
 # `bench.py --traffic-probe`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the
 # factors being what the counters read for the probe's 512 MiB device copy.
 # Counters cannot be read inside this process, so the figure is reported only
 # when the timed launches run that exact configuration.
 MEASURED_TRAFFIC = {
-    "closest": (8205011.9 * 2.0 + 607366.3) * 1024,
-    "shadow": (10840375.3 * 2.0 + 381512.3) * 1024,
+    "closest": (7898149.1 * 2.0 + 581600.0) * 1024,
+    "shadow": (10830856.9 * 2.0 + 379299.5) * 1024,
 }
 # TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) of the same launches (a third counter run)
 MEASURED_L2_HIT = {
-    "closest": 137940585.0 / (137940585.0 + 141355280.0),
-    "shadow": 218786655.0 / (218786655.0 + 179834620.0),
+    "closest": 304816695.0 / (304816695.0 + 138665037.0),
+    "shadow": 218569677.0 / (218569677.0 + 179414871.0),
 }
-MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=48, waves_per_cu=0,
+MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=56, waves_per_cu=0,
                                 xcd_ranges=0)
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
 NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
@@ -471,6 +471,9 @@ def main():
                     kernel="k_trace<%s>" % ("closest" if dominant == "closest" else "occlusion"),
                     algorithmic_bytes_per_launch=int(bytes_dom),
                     kernel_ms=round(ms_dom, 4),
+                    # both traversal kernels, whichever of them is the dominant one above
+                    closest_kernel_gbs=round(bytes_closest / (k_closest * 1e-3) / 1e9, 1),
+                    closest_kernel_ms=round(k_closest, 4),
                     shadow_kernel_gbs=round(bytes_shadow / (k_shadow * 1e-3) / 1e9, 1),
                     shadow_kernel_ms=round(k_shadow, 4),
                     sort_ms_per_step=round(sort_ms, 4),

@@ -156,7 +156,7 @@ class Stage:
         check(lib().gr_scene_get_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "get_tuning")
         return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
-    def set_tuning(self, sort_bits=0, lds_stack=8, waves_per_cu=0, refill=48, xcd_ranges=0):
+    def set_tuning(self, sort_bits=0, lds_stack=8, waves_per_cu=0, refill=56, xcd_ranges=0):
         t = _lib.Tuning(sort_bits, lds_stack, waves_per_cu, refill, xcd_ranges)
         check(lib().gr_scene_set_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "set_tuning")
         return self.get_tuning()
@@ -415,7 +415,7 @@ def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, u
     p.steps_per_poll = steps_per_poll
     p.use_graph = 1 if use_graph else 0
     if tuning:
-        t = dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=48, xcd_ranges=0)
+        t = dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=56, xcd_ranges=0)
         t.update(tuning)
         p.tuning = _lib.Tuning(**t)
     ms = (ctypes.c_double * n)()

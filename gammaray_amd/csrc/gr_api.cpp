@@ -454,7 +454,7 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 48
+    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 56
     float best_ms = 0.0f;
     auto timed = [&](const gr::TraceTuning& tu, float* ms) {
         return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
@@ -491,7 +491,7 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
         c.waves_per_cu = (uint32_t)std::max(1, occ * frac / 4);
         if ((int)c.waves_per_cu < occ) consider(c);
     }
-    for (uint32_t rf : {16u, 32u, 40u, 56u}) {
+    for (uint32_t rf : {16u, 32u, 40u, 48u, 60u}) {
         gr::TraceTuning c = best;
         c.refill = rf;
         if (rf != best.refill) consider(c);

@@ -134,7 +134,7 @@ struct TraceTuning {
     uint32_t sort_bits = 0;
     uint32_t lds_stack = 8;  // 8 waves per SIMD (deeper LDS stacks cap occupancy lower)
     uint32_t waves_per_cu = 0;
-    uint32_t refill = 48;
+    uint32_t refill = 56;
     uint32_t xcd_ranges = 0;
 };
 // Ray counters of the persistent traversal: one per XCD range.

@@ -149,7 +149,7 @@ typedef struct {
                               octant above the m-bit-per-axis Morton code of the origin */
     uint32_t lds_stack;    /* traversal stack entries per lane kept in LDS: 8 (default), 16 or 32 */
     uint32_t waves_per_cu; /* resident traversal waves per CU; 0 = occupancy limit */
-    uint32_t refill;       /* idle lanes (1..64) at which a wave fetches new rays (default 48) */
+    uint32_t refill;       /* idle lanes (1..64) at which a wave fetches new rays (default 56) */
     uint32_t xcd_ranges;   /* 1: the (sorted) batch is cut into one contiguous range per XCD,
                               each drained by the waves resident on that XCD first */
 } gr_tuning;
