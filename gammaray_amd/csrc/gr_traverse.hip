@@ -230,9 +230,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     };
 
     // Pops entries until one whose entry distance is still below `closest`.
+    // The visibility kernels' counting variant counts each popped entry's box
+    // test there (the reference's deferred test of a far child).
     auto pop = [&]() -> bool {
         while (sp > 0) {
             --sp;
+            if (COUNT && MODE != kClosest) c_box += 1;
             uint2 e;
             if (sp < base) {
                 e = spill[sp];
@@ -405,7 +408,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             // ---- interior nodes until every lane sits on a leaf or is done
             while (active && !(cur & kLeafBit)) {
                 const GpuNode nd = load_node(nodes, cur);
-                if (COUNT) c_box += 2;
+                // The reference tests the near child's box next; the far child's
+                // box is tested when it is popped (counted in pop()).
+                if (COUNT) c_box += 1;
                 float t0 = 0.0f, t1 = 0.0f;
                 const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
                 const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
@@ -419,7 +424,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                 const float far_t = flip ? t0 : t1;
                 // A far child whose entry distance is already >= closest would fail
                 // the deferred test at its pop (closest only decreases): not pushed.
-                if (far_ok && far_t < closest) {
+                // The counting variant pushes every far child, as the reference
+                // does, so that a traversal ending at its first occluder counts
+                // exactly the boxes the reference tests before it stops (those
+                // still on the stack then are never tested); a child whose slab
+                // test failed carries an infinite entry distance, failing at pop.
+                if (COUNT || (far_ok && far_t < closest)) {
                     if (sp == kStackSize) {
                         finish(false, true);
                         break;
@@ -428,7 +438,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                         spill[base] = s_e[base & (LDS - 1)][lane];
                         ++base;
                     }
-                    s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
+                    const float push_t = far_ok ? far_t : __builtin_inff();
+                    s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(push_t));
                     ++sp;
                 }
                 if (near_ok && near_t < closest) {

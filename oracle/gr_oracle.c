@@ -544,8 +544,12 @@ static float comp_intersect(const orc_scene *s, uint32_t prim, uint32_t comp, ov
 #define MAX_NODES_TO_VISIT 64
 
 /* Bvh::intersect, src/geom/bvh.rs:406-456.  Returns 0, or -1 when the 64-entry
- * VisitStack would overflow (the reference panics, bvh.rs:156-157). */
-int orc_intersect_one(const orc_scene *s, ovec o, ovec d, orc_hit *hit, uint32_t cnt[3])
+ * VisitStack would overflow (the reference panics, bvh.rs:156-157).
+ * stop_below: the traversal ends at the first accepted hit closer than this
+ * (the early exit of a visibility query that only needs one occluder closer
+ * than its target; -INFINITY never stops early, as the reference). */
+static int intersect_impl(const orc_scene *s, ovec o, ovec d, orc_hit *hit, uint32_t cnt[3],
+                          float stop_below)
 {
     float closest = FLT_MAX;
     hit->prim = ORC_NONE; hit->comp = ORC_NONE; hit->dist = 0.0f;
@@ -566,6 +570,7 @@ int orc_intersect_one(const orc_scene *s, ovec o, ovec d, orc_hit *hit, uint32_t
                     if (dist != 0.0f && dist < closest) {
                         hit->prim = pi; hit->comp = ci; hit->dist = dist;
                         closest = dist;
+                        if (dist < stop_below) goto done;
                     }
                 }
                 if (sp == 0) break;
@@ -583,8 +588,14 @@ int orc_intersect_one(const orc_scene *s, ovec o, ovec d, orc_hit *hit, uint32_t
             cur = stack[--sp];
         }
     }
+done:
     if (cnt) { cnt[0] = c_box; cnt[1] = c_tri; cnt[2] = c_sph; }
     return 0;
+}
+
+int orc_intersect_one(const orc_scene *s, ovec o, ovec d, orc_hit *hit, uint32_t cnt[3])
+{
+    return intersect_impl(s, o, d, hit, cnt, -INFINITY);
 }
 
 /* Bvh::intersect_naive, src/geom/bvh.rs:385-400. */
@@ -648,8 +659,12 @@ void orc_intersect_naive(const orc_scene *s, const float *rays, int64_t n, uint3
 }
 
 /* Bvh::visibility, src/geom/bvh.rs:460-477.  Returns 1 visible, 0 occluded,
- * -1 on stack overflow. */
-int orc_visibility_one(const orc_scene *s, ovec start, ovec target, uint32_t cnt[3])
+ * -1 on stack overflow.  early_exit = 0 runs the reference's full closest-hit
+ * traversal; 1 stops at the first accepted hit closer than target_dist - 1e-3
+ * (same answer: once the closest distance is below that bound it stays below
+ * it), so cnt then holds the work of a traversal that ends at its first
+ * occluder. */
+static int visibility_impl(const orc_scene *s, ovec start, ovec target, uint32_t cnt[3], int early_exit)
 {
     if (cnt) { cnt[0] = cnt[1] = cnt[2] = 0; }
     if (v_is_close(start, target, 1e-3f)) return 0;
@@ -657,15 +672,20 @@ int orc_visibility_one(const orc_scene *s, ovec start, ovec target, uint32_t cnt
     ovec origin = v_add(start, v_scale(dir, ORC_RAY_PUSH_DIST)); /* Ray::nudge, ray.rs:44-46 */
     float target_dist = v_mag(v_sub(target, origin));
     orc_hit h;
-    if (orc_intersect_one(s, origin, dir, &h, cnt) != 0) return -1;
+    if (intersect_impl(s, origin, dir, &h, cnt, early_exit ? target_dist - 1e-3f : -INFINITY) != 0) return -1;
     if (h.prim != ORC_NONE) {
         if (h.dist < (target_dist - 1e-3f)) return 0;
     }
     return 1;
 }
 
+int orc_visibility_one(const orc_scene *s, ovec start, ovec target, uint32_t cnt[3])
+{
+    return visibility_impl(s, start, target, cnt, 0);
+}
+
 int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t *out,
-                       uint32_t *counts, int nthreads)
+                       uint32_t *counts, int early_exit, int nthreads)
 {
     int64_t overflow = 0;
     if (nthreads <= 0) nthreads = 1;
@@ -673,7 +693,7 @@ int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t
     for (int64_t i = 0; i < n; ++i) {
         const float *g = &segs[6 * i];
         uint32_t cnt[3];
-        int v = orc_visibility_one(s, v_new(g[0], g[1], g[2]), v_new(g[3], g[4], g[5]), cnt);
+        int v = visibility_impl(s, v_new(g[0], g[1], g[2]), v_new(g[3], g[4], g[5]), cnt, early_exit);
         if (v < 0) { overflow++; v = 0; }
         out[i] = (uint8_t)v;
         if (counts) { counts[3*i] = cnt[0]; counts[3*i+1] = cnt[1]; counts[3*i+2] = cnt[2]; }

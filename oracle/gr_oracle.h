@@ -75,9 +75,12 @@ int64_t orc_intersect(const orc_scene *s, const float *rays, int64_t n, uint32_t
 void orc_intersect_naive(const orc_scene *s, const float *rays, int64_t n, uint32_t *prim,
                          uint32_t *comp, float *dist, int nthreads);
 /* Bvh::visibility (src/geom/bvh.rs:460-477).  segs: 6 floats (start, target).
- * out: 1 = visible.  counts optional as for orc_intersect. */
+ * out: 1 = visible.  counts optional as for orc_intersect.  early_exit = 0:
+ * the reference's full closest-hit traversal; 1: the traversal ends at the
+ * first accepted hit closer than the target (same answers; counts then give
+ * the work of a first-occluder traversal). */
 int64_t orc_visibility(const orc_scene *s, const float *segs, int64_t n, uint8_t *out,
-                       uint32_t *counts, int nthreads);
+                       uint32_t *counts, int early_exit, int nthreads);
 
 /* ---- rendering (orc_render.c) ---- */
 #define ORC_PT 0      /* PathTracerIntegrator + max_depth cap */

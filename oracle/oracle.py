@@ -52,7 +52,7 @@ def lib():
         L.orc_intersect.restype = ctypes.c_int64
         L.orc_intersect_naive.argtypes = [_vp, _f32p, ctypes.c_int64, _u32p, _u32p, _f32p,
                                           ctypes.c_int]
-        L.orc_visibility.argtypes = [_vp, _f32p, ctypes.c_int64, _u8p, _u32p, ctypes.c_int]
+        L.orc_visibility.argtypes = [_vp, _f32p, ctypes.c_int64, _u8p, _u32p, ctypes.c_int, ctypes.c_int]
         L.orc_visibility.restype = ctypes.c_int64
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.orc_camera_matrix.argtypes = [_f32p, _f32p, _f32p]
@@ -173,13 +173,16 @@ class OracleScene:
                                   _p(dist, _f32p), threads or default_threads())
         return dict(prim=prim, comp=comp, dist=dist)
 
-    def visibility(self, segs, counts=False, threads=None):
+    def visibility(self, segs, counts=False, threads=None, early_exit=False):
+        """Bvh::visibility per segment.  early_exit=True stops each traversal at
+        its first occluder (the same answers; `counts` then measure that
+        shorter traversal)."""
         segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 6)
         n = len(segs)
         out = np.zeros(n, np.uint8)
         cnt = np.zeros((n, 3), np.uint32) if counts else None
         ov = lib().orc_visibility(self._h, _p(segs, _f32p), n, _p(out, _u8p), _p(cnt, _u32p),
-                                  threads or default_threads())
+                                  1 if early_exit else 0, threads or default_threads())
         res = dict(visible=out, overflow=int(ov))
         if counts:
             res["counts"] = cnt

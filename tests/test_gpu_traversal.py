@@ -145,9 +145,12 @@ def test_visibility_exact_matches_oracle(name, gpu_ctx):
     segs[100:200, 4:7] = segs[100:200, 0:3] + 5e-4        # within 1e-3 per axis -> not visible
     vis, cnt = stage.visibility(_cuda(segs), mode=GR_OCCLUSION_EXACT, counts=True)
     torch.cuda.synchronize()
-    want = o.visibility(scenes.rays6(segs))
+    want = o.visibility(scenes.rays6(segs), counts=True, early_exit=True)
     assert np.array_equal(vis.cpu().numpy().astype(np.uint8), want["visible"])
     assert not want["visible"][:200].any()
+    # The kernel's work counts (box, triangle, sphere tests per segment) are
+    # those of the reference-order traversal ending at its first occluder.
+    assert np.array_equal(cnt.cpu().numpy().view(np.uint32), want["counts"])
 
 
 @pytest.mark.parametrize("name", ["box", "showcase", "ico6"])

@@ -241,6 +241,52 @@ def test_visibility_matches_brute_force():
     assert len(mism) <= 0.001 * len(segs)
 
 
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_visibility_early_exit_counts(name):
+    """The first-occluder traversal gives the reference's answers.  It does the
+    reference's whole traversal on visible segments, since they meet no
+    occluder.  On occluded ones it does a prefix of that traversal, and so no
+    more box or primitive tests."""
+    spec = SCENES[name]()
+    o = OracleScene(spec)
+    segs = scenes.rays6(scenes.segment_batch(spec, 20000, seed=6))
+    full = o.visibility(segs, counts=True)
+    early = o.visibility(segs, counts=True, early_exit=True)
+    assert np.array_equal(full["visible"], early["visible"])
+    vis = full["visible"] == 1
+    assert vis.any() and (~vis).any()
+    assert np.array_equal(full["counts"][vis], early["counts"][vis])
+    assert np.all(early["counts"] <= full["counts"])
+    # an occluder ends the traversal early for most occluded segments
+    shorter = (early["counts"][~vis].sum(1) < full["counts"][~vis].sum(1)).mean()
+    assert shorter > 0.5, shorter
+
+
+def test_visibility_early_exit_counts_by_hand():
+    """Two unit spheres on the x axis at 0 and 3 (a root and two leaves).  A
+    segment along +x from x = -5 to x = 10 is blocked by the first sphere.
+    The first-occluder traversal tests the root box, then the near child's
+    box and its sphere, and stops: 2 box tests and 1 sphere test.  The full
+    traversal also pops the far child and tests its box, which starts beyond
+    the hit already found (the deferred test of bvh.rs:426), so its sphere is
+    not tested: 3 box tests and 1 sphere test."""
+    spec = _spheres_scene([(0, 0, 0), (3, 0, 0)], radius=1.0)
+    o = OracleScene(spec)
+    seg = np.array([[-5, 0, 0, 10, 0, 0]], np.float32)
+    early = o.visibility(seg, counts=True, early_exit=True)
+    full = o.visibility(seg, counts=True)
+    assert early["visible"][0] == 0 and full["visible"][0] == 0
+    assert early["counts"][0].tolist() == [2, 0, 1]
+    assert full["counts"][0].tolist() == [3, 0, 1]
+    # a segment ending before the first sphere is visible: both traverse fully
+    # (the sphere is hit beyond the target, so nothing stops the traversal)
+    seg2 = np.array([[-5, 0, 0, -2, 0, 0]], np.float32)
+    e2 = o.visibility(seg2, counts=True, early_exit=True)
+    f2 = o.visibility(seg2, counts=True)
+    assert e2["visible"][0] == 1 and f2["visible"][0] == 1
+    assert e2["counts"][0].tolist() == f2["counts"][0].tolist()
+
+
 # --------------------------------------------------------------------------
 def _degenerate_scenes():
     out = {}
