@@ -13,6 +13,7 @@ GR_HOST_ONLY = -1
 GR_NONE = 0xFFFFFFFF
 GR_OCCLUSION_EXACT = 0
 GR_OCCLUSION_ANYHIT = 1
+GR_NOALLOC_MAX_RAYS = 1 << 20
 ERRORS = {-1: "GR_ERR_INVALID", -2: "GR_ERR_HIP", -3: "GR_ERR_NOMEM", -4: "GR_ERR_STATE",
           -5: "GR_ERR_OVERFLOW"}
 
@@ -102,6 +103,7 @@ SIGNATURES = {
     "gr_scene_info_get": (_i, [_vp, ctypes.POINTER(SceneInfo)]),
     "gr_scene_export_bvh": (_i, [_vp, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p]),
     "gr_intersect_closest": (_i, [_vp, _vp, _u32, _vp, _vp]),
+    "gr_intersect_closest_noalloc": (_i, [_vp, _vp, _u32, _vp, _vp]),
     "gr_occluded": (_i, [_vp, _vp, _u32, _vp, _i, _vp]),
     "gr_scene_take_overflow_count": (_i, [_vp, _u64p]),
     "gr_scene_set_tuning": (_i, [_vp, _vp]),

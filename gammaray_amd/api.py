@@ -128,6 +128,14 @@ class Stage:
                                          ctypes.c_void_p(counts_ptr) if counts_ptr else None),
               self.ctx.handle, "intersect_closest")
 
+    def intersect_noalloc_ptr(self, rays_ptr, n, hits_ptr, counts_ptr=None):
+        """gr_intersect_closest_noalloc: the same results with no allocation
+        (scratch in a static device array; n <= GR_NOALLOC_MAX_RAYS)."""
+        check(lib().gr_intersect_closest_noalloc(self._h, ctypes.c_void_p(rays_ptr), n,
+                                                 ctypes.c_void_p(hits_ptr),
+                                                 ctypes.c_void_p(counts_ptr) if counts_ptr else None),
+              self.ctx.handle, "intersect_closest_noalloc")
+
     def occluded_ptr(self, segs_ptr, n, vis_ptr, mode=GR_OCCLUSION_EXACT, counts_ptr=None):
         check(lib().gr_occluded(self._h, ctypes.c_void_p(segs_ptr), n, ctypes.c_void_p(vis_ptr), mode,
                                 ctypes.c_void_p(counts_ptr) if counts_ptr else None),

@@ -16,7 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgammaray_amd.so")
 ARCH = os.environ.get("GR_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gr_bvh.cpp", "gr_api.cpp", "gr_exr.cpp", "gr_obj.cpp", "gr_traverse.hip", "gr_sort.hip", "gr_guide.hip", "gr_render.hip"]
+SOURCES = ["gr_bvh.cpp", "gr_api.cpp", "gr_exr.cpp", "gr_obj.cpp", "gr_traverse.hip", "gr_sort.hip", "gr_guide.hip", "gr_render.hip",
+           "gr_noalloc.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I" + os.path.join(HERE, "..", "include")]

@@ -116,7 +116,9 @@ int gr_ctx_create(int device, gr_ctx** out) {
     // A blocking stream: ordered with work on the legacy default stream, where
     // a caller's uploads often run.
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);
+    if (e == hipSuccess) e = gr::noalloc_setup(device);
     if (e != hipSuccess) {
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
         delete c;
         return GR_ERR_HIP;
     }

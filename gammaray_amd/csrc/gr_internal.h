@@ -116,6 +116,10 @@ struct SortScratch {
 // Rounds a requested key width to the widths the sort uses (0 or 3 + 3m).
 uint32_t sort_key_bits(uint32_t bits);
 hipError_t reserve_sort(SortScratch& ss, size_t n);
+// Bytes of sort scratch for n keys, and the layout of that scratch at `base`
+// (ss.blob stays null: the memory is not the sort's to free).
+size_t sort_scratch_bytes(size_t n);
+void carve_sort(SortScratch& ss, void* base, size_t n);
 void release_sort(SortScratch& ss);
 // Sorts the first n (= *n_ptr if given, at most n_max) rays or segments of
 // `in` by (octant, origin Morton code); *perm receives the ray indices in
@@ -307,6 +311,9 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
 hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_batch_params& prm, double* job_ms);
 hipError_t material_query(gr_scene* sc, uint32_t prim, const float* in, uint32_t n, float* out);
 void release_render_buffers(RenderBuffers& rb);
+// gr_noalloc.hip: resolves the no-allocation scratch array's address on the
+// current device (once per device; called by gr_ctx_create).
+hipError_t noalloc_setup(int device);
 // gr_exr.cpp
 bool write_exr(const float* rgbw, uint32_t width, uint32_t height, const char* path);
 void encode_exr(const float* rgbw, uint32_t width, uint32_t height, std::vector<uint8_t>* out);

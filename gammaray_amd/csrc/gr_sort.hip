@@ -237,17 +237,18 @@ uint32_t sort_key_bits(uint32_t bits) {
     return 3 + 3 * ((bits - 3) / 3);
 }
 
-hipError_t reserve_sort(SortScratch& ss, size_t n) {
-    if (n <= ss.cap) return hipSuccess;
-    release_sort(ss);
+size_t sort_scratch_bytes(size_t n) {
     const size_t nblocks = (n + kTile - 1) / kTile;
     const size_t keys = ((n * 4 + 255) / 256) * 256;
     const size_t hist = ((nblocks * 256 * 4 + 255) / 256) * 256;
-    const size_t bytes = 4 * keys + hist + 256 * 4;
-    char* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) return e;
-    ss.blob = p;
+    return 4 * keys + hist + 256 * 4;
+}
+
+void carve_sort(SortScratch& ss, void* base, size_t n) {
+    const size_t nblocks = (n + kTile - 1) / kTile;
+    const size_t keys = ((n * 4 + 255) / 256) * 256;
+    const size_t hist = ((nblocks * 256 * 4 + 255) / 256) * 256;
+    char* p = static_cast<char*>(base);
     ss.keys[0] = reinterpret_cast<uint32_t*>(p);
     ss.keys[1] = reinterpret_cast<uint32_t*>(p + keys);
     ss.vals[0] = reinterpret_cast<uint32_t*>(p + 2 * keys);
@@ -255,6 +256,16 @@ hipError_t reserve_sort(SortScratch& ss, size_t n) {
     ss.hist = reinterpret_cast<uint32_t*>(p + 4 * keys);
     ss.totals = reinterpret_cast<uint32_t*>(p + 4 * keys + hist);
     ss.cap = n;
+}
+
+hipError_t reserve_sort(SortScratch& ss, size_t n) {
+    if (n <= ss.cap) return hipSuccess;
+    release_sort(ss);
+    char* p = nullptr;
+    hipError_t e = hipMalloc(&p, sort_scratch_bytes(n));
+    if (e != hipSuccess) return e;
+    ss.blob = p;
+    carve_sort(ss, p, n);
     return hipSuccess;
 }
 

@@ -124,6 +124,18 @@ int gr_scene_export_bvh(gr_scene *scene, float *bbox6, uint32_t *offset, uint32_
  * tests (the reference-order traversal's work, for the roofline's bytes). */
 int gr_intersect_closest(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
                          uint32_t *d_counts);
+/* gr_intersect_closest without any allocation.  No counterpart in the
+ * reference (it has no allocator to avoid); for callers that cannot allocate:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments and results as gr_intersect_closest, with the
+ * scene's tuning (ray sort included).  The ray-sort scratch and the traversal's
+ * work counters live in one static device array per GPU (about 16.3 MiB, held
+ * until the process ends), so n is limited to GR_NOALLOC_MAX_RAYS (larger n:
+ * GR_ERR_INVALID).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams or host threads) race. */
+#define GR_NOALLOC_MAX_RAYS (1u << 20)
+int gr_intersect_closest_noalloc(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
+                                 uint32_t *d_counts);
 /* Bvh::visibility for n segments: d_visible[i] = 1 if target is visible from
  * start, else 0.  mode GR_OCCLUSION_EXACT runs the reference's closest-hit
  * traversal order and stops at the first hit that proves occlusion (result
