@@ -7,6 +7,15 @@ What pins what:
     trees that expose the reference's quirks (median split through the
     nth_element whose left recursion drops an element, src/geom/util.rs:39-41;
     multi-component leaves for coincident centroids, src/geom/bvh.rs:207-216).
+    The 12-bucket SAH path (bvh.rs:229-296) is pinned by trees derived by hand
+    from its cost arithmetic.  Each case is built so that one plausible
+    mistake changes the tree:
+      - volume instead of area (Q1): volume and area pick different buckets;
+      - the "leaf" branch's median split when min cost >= n;
+      - the n > 255 rule forcing a split;
+      - all-NaN costs on flat boxes;
+      - a NaN cost[0] kept by the strict-< minimum seeded with cost[0].
+    The library's builder must give the same trees.
   * intersect -- the reference's own brute force, Bvh::intersect_naive
     (src/geom/bvh.rs:385-400), equal except documented ties (|dt| <= 1e-6 t),
     and closed-form ray/triangle and ray/sphere distances.
@@ -127,6 +136,168 @@ def test_bvh_axis_choice_prefers_x_on_ties():
     assert t["axis"][0] == 0
     t = OracleScene(_spheres_scene([(0, 0, 0), (1, 2, 2)])).export_bvh()
     assert t["axis"][0] == 1
+
+
+def _subtree_prims(t, node):
+    """Prims of the components under `node` (DFS layout: a leaf's range, or
+    both children's)."""
+    if t["count"][node] > 0:
+        o = t["offset"][node]
+        return t["comp_prim"][o:o + t["count"][node]].tolist()
+    return _subtree_prims(t, node + 1) + _subtree_prims(t, t["offset"][node])
+
+
+def _volume_spheres():
+    spec = _spheres_scene([(12, 0, 0), (4, 0, 0), (0, 0, 0), (5, 0, 0), (1, 0, 0)], radius=1.0)
+    for p, r in zip(spec.prims, (3.0, 3.0, 1.0, 3.0, 1.0)):
+        p.radius = r
+    return spec
+
+
+def test_bvh_sah_uses_volume():
+    """Five spheres on the x axis, given in the order x = 12, 4, 0, 5, 1 (prims
+    0..4) with radii 3, 3, 1, 3, 1.  Centroids span 0..12, so bucket =
+    floor(12 x / 12) = [11, 4, 0, 5, 1].  The reference's "surface area" is the
+    box volume (Q1, src/core/bbox.rs:73-76).  The node box is [-1,15] x [-3,3]^2,
+    volume 576.  Split costs 1 + (n0 V0 + n1 V1) / 576 after bucket:
+      0:     {0} V 8, {1,4,5,12} V 540      -> 1 + 2168/576 = 4.764
+      1..3:  {0,1} V 12, {4,5,12} V 504     -> 1 + 1536/576 = 3.667
+      4:     {0,1,4} V 288, {5,12} V 468    -> 1 + 1800/576 = 4.125
+      5..10: {0,1,4,5} V 324, {12} V 216    -> 1 + 1512/576 = 3.625  (minimum)
+    A true surface area (node 456) would give 3.825 after bucket 1 and 4.0
+    after bucket 5, and split {0,1} | {4,5,12} instead.  3.625 < 5 components,
+    so the reference partitions after bucket 5.  Its swap partition
+    (util.rs:4-15) leaves [4, 0, 5, 1 | 12].  The left four take the median
+    split, and nth_element's quirk (util.rs:39-41) leaves [0, 4, 1, 5]: halves
+    {0, 4} and {1, 5}."""
+    t = OracleScene(_volume_spheres()).export_bvh()
+    assert t["count"].tolist() == [0, 0, 0, 1, 1, 0, 1, 1, 1]
+    assert t["offset"].tolist() == [8, 5, 4, 0, 1, 7, 2, 3, 4]
+    assert t["axis"].tolist() == [0] * 9
+    # x = 0, 4, 1, 5 | 12  ->  prims 2, 1, 4, 3 | 0
+    assert t["comp_prim"].tolist() == [2, 1, 4, 3, 0]
+    np.testing.assert_array_equal(t["bbox"][1], [-1, -3, -3, 8, 3, 3])
+    np.testing.assert_array_equal(t["bbox"][2], [-1, -3, -3, 7, 3, 3])
+
+
+def test_bvh_sah_leaf_cost_takes_the_median_split():
+    """Six spheres of radius 100 at x = 3, 12, 0, 4, 1, 2 (prims 0..5): every box
+    spans almost the whole node, so no split is cheaper than a leaf.  Buckets
+    are floor(x) = [3, 11, 0, 4, 1, 2].  With equal y/z extents, the volume
+    costs are 1 + (n0 (200 + max x left) + n1 (212 - min x right)) / 212:
+    6.920, 6.858, 6.816, 6.792 after buckets 0..3, and 6.755 after bucket 4 (the
+    minimum).  That is >= 6 components, so the reference takes the "leaf" branch
+    (bvh.rs:276-296).  It splits at the median (mid 3) by nth_element rather
+    than at bucket 4, which would give {0,1,2,3,4} | {12}.  nth_element over
+    x = [3, 12, 0, 4, 1, 2] leaves [0, 2, 3, 1, 4, 12] (its left recursion skips
+    an element).  The halves are {0, 2, 3} | {1, 4, 12}, not {0, 1, 2} | {3, 4, 12}.
+    Each half of three then splits at its median: {0} | {2, 3} and {1} | {4, 12}."""
+    xs = [3, 12, 0, 4, 1, 2]
+    spec = _spheres_scene([(x, 0, 0) for x in xs], radius=100.0)
+    t = OracleScene(spec).export_bvh()
+    assert t["count"].tolist() == [0, 0, 1, 0, 1, 1, 0, 1, 0, 1, 1]
+    assert t["offset"].tolist() == [6, 3, 0, 5, 1, 2, 8, 3, 10, 4, 5]
+    # x = 0, 2, 3, 1, 4, 12  ->  prims 2, 5, 0, 4, 3, 1
+    assert t["comp_prim"].tolist() == [2, 5, 0, 4, 3, 1]
+
+
+def test_bvh_sah_more_than_255_components_always_split():
+    """300 spheres of radius 1e6: 250 at x = 0..249 and 50 at x = 1000..1049
+    (prims in that order, reversed so partitioning must move them).  Every
+    split costs about 1 + 300 * 0.9999 >= 300 components, so only the
+    MAX_COMPONENTS_PER_NODE rule (n > 255, bvh.rs:277) forces an interior
+    split at the cheapest bucket.  Buckets: floor(12 x / 1049) is 0, 1 or 2 for
+    x < 250 and 11 for x >= 1000.  Up to the shared 2e6 terms, the cost
+    numerators n0 max_left + n1 (1049 - min_right) are 211388, 139700 and 64700
+    after buckets 0, 1 and 2 (and after 3..10).  So the split falls after
+    bucket 2: the 250 near spheres | the 50 far ones.  A leaf-cost test without
+    the n > 255 rule would take the median split, 150 | 150."""
+    xs = list(range(250)) + list(range(1000, 1050))
+    xs = xs[::-1]
+    spec = _spheres_scene([(x, 0, 0) for x in xs], radius=1.0e6)
+    t = OracleScene(spec).export_bvh()
+    left = _subtree_prims(t, 1)
+    right = _subtree_prims(t, int(t["offset"][0]))
+    assert sorted(np.array(xs)[left].tolist()) == list(range(250))
+    assert sorted(np.array(xs)[right].tolist()) == list(range(1000, 1050))
+    check_invariants(spec, t)
+
+
+def _flat_triangles(xs):
+    """One small triangle per x in the plane z = 0.  Every box has zero
+    volume, so every SAH cost is 1 + 0/0 = NaN."""
+    s = scenes.SceneSpec(name="flat")
+    verts, tris = [], []
+    for k, x in enumerate(xs):
+        verts += [(x - 0.3, -0.2, 0.0), (x + 0.3, -0.2, 0.0), (x, 0.4, 0.0)]
+        tris.append((3 * k, 3 * k + 1, 3 * k + 2))
+    s.add_mesh(np.array(verts, np.float32), tris, scenes.disney())
+    return s
+
+
+def test_bvh_sah_nan_costs():
+    """All-NaN costs: the minimum search seeds with cost[0] and compares with a
+    strict < (bvh.rs:266-273), so it keeps NaN at bucket 0, and NaN < leaf cost
+    is false.  With <= 255 components that is the median split: 100 triangles at
+    x = 0..99 split 50 | 50.  Partitioning at bucket 0 would split
+    floor(12 x / 99) = 0, x < 8.25: 9 | 91.  With > 255 components the n > 255
+    rule partitions at bucket 0: 300 triangles at x = 0..299 split into
+    floor(12 x / 299) = 0, x < 24.92: 25 | 275.  A true surface area, non-zero
+    for these flat boxes, would give finite costs and split near the middle."""
+    for n, n_left in ((100, 50), (300, 25)):
+        spec = _flat_triangles(range(n))
+        t = OracleScene(spec).export_bvh()
+        assert t["count"][0] == 0 and t["axis"][0] == 0
+        # the left child's leaves come first in component order; triangle k sits at x = k
+        o1 = int(t["offset"][0])
+        assert int(sum(t["count"][1:o1])) == n_left
+        if n == 300:
+            assert sorted(t["comp_index"][:n_left].tolist()) == list(range(n_left))
+        check_invariants(spec, t)
+
+
+def _overflow_spheres():
+    # x = 2e14, 1e12, 2e13, 1.99e14, 0 (prims 0..4), radius 1e12
+    return _spheres_scene([(2e14, 0, 0), (1e12, 0, 0), (2e13, 0, 0), (1.99e14, 0, 0), (0, 0, 0)],
+                          radius=1.0e12)
+
+
+def test_bvh_sah_nan_first_cost_is_kept():
+    """Only cost[0] is NaN; the others are finite.  Boxes are 2e12 wide in y and z,
+    so a box longer than 8.5e13 in x has an infinite f32 volume.  The node
+    (x extent 2.02e14) is infinite.  After bucket 0 the right side {2e13,
+    1.99e14, 2e14} is infinite too, and inf / inf makes cost[0] NaN.  After
+    buckets 1..10 both sides are finite, so the cost is 1 + finite / inf = 1.
+    The reference seeds the minimum with cost[0] and compares with a strict <
+    (bvh.rs:266-273).  So it keeps NaN at bucket 0, NaN < 5 is false, and it
+    takes the median split.  nth_element over x = [2e14, 1e12, 2e13, 1.99e14, 0]
+    leaves [1e12, 0 | 2e13, 1.99e14, 2e14].  A search seeded with +inf would
+    pick cost 1 at bucket 1 and partition {0, 1e12, 2e13} | {1.99e14, 2e14}."""
+    t = OracleScene(_overflow_spheres()).export_bvh()
+    assert sorted(_subtree_prims(t, 1)) == [1, 4]
+    assert sorted(_subtree_prims(t, int(t["offset"][0]))) == [0, 2, 3]
+
+
+def _sah_cases():
+    return {
+        "volume": _volume_spheres,
+        "leaf_cost": lambda: _spheres_scene([(x, 0, 0) for x in [3, 12, 0, 4, 1, 2]], radius=100.0),
+        "over255": lambda: _spheres_scene([(x, 0, 0) for x in (list(range(250)) + list(range(1000, 1050)))[::-1]],
+                                          radius=1.0e6),
+        "flat100": lambda: _flat_triangles(range(100)),
+        "flat300": lambda: _flat_triangles(range(300)),
+        "nan_first": _overflow_spheres,
+    }
+
+
+@pytest.mark.parametrize("name", sorted(_sah_cases()))
+def test_library_builder_matches_oracle_on_sah_cases(name, host_ctx):
+    from gammaray_amd.api import Stage
+    spec = _sah_cases()[name]()
+    got = Stage(spec, ctx=host_ctx).export_bvh()
+    want = OracleScene(spec).export_bvh()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
 
 
 # --------------------------------------------------------------------------
