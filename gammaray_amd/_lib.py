@@ -32,6 +32,11 @@ class SceneInfo(ctypes.Structure):
                 ("build_seconds", ctypes.c_double)]
 
 
+class Stats(ctypes.Structure):
+    _fields_ = [("nonfinite_bsdf_samples", ctypes.c_uint64), ("nonfinite_radiance", ctypes.c_uint64),
+                ("bad_splats", ctypes.c_uint64), ("reserved", ctypes.c_uint64)]
+
+
 class Camera(ctypes.Structure):
     _fields_ = [("focal_length", ctypes.c_float), ("horizontal_aperture", ctypes.c_float),
                 ("vertical_aperture", ctypes.c_float), ("f_stop", ctypes.c_float),
@@ -56,7 +61,10 @@ class Job(ctypes.Structure):
 
 class BatchParams(ctypes.Structure):
     _fields_ = [("integrator", _u32), ("path_pool", _u32), ("steps_per_poll", _u32), ("use_graph", _u32),
-                ("tuning", Tuning)]
+                ("tuning", Tuning), ("job_order", _u32), ("sample_ring", _u32)]
+
+
+GR_JOBS_AS_GIVEN, GR_JOBS_FEWEST_FIRST = 0, 1
 
 
 class GuideParams(ctypes.Structure):
@@ -87,6 +95,7 @@ SIGNATURES = {
     "gr_ctx_stream": (_vp, [_vp]),
     "gr_ctx_synchronize": (_i, [_vp]),
     "gr_ctx_last_error": (ctypes.c_char_p, [_vp]),
+    "gr_ctx_take_stats": (_i, [_vp, ctypes.POINTER(Stats)]),
     "gr_scene_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "gr_scene_destroy": (None, [_vp]),
     "gr_scene_add_mesh": (_i, [_vp, _f32p, _u32, _f32p, _u32, _f32p, _u32, _u32p, _u32, _f32p, _u32p]),
@@ -116,6 +125,7 @@ SIGNATURES = {
     "gr_guide_infer": (_i, [_vp, _vp, _u32, _vp]),
     "gr_guide_gradient": (_i, [_vp, _vp, _u32, _vp]),
     "gr_guide_train": (_i, [_vp, _vp, _u32, _u32]),
+    "gr_guide_last_records": (_i, [_vp, _vp, _u32, _u32p]),
     "gr_guide_mixture_sample": (_i, [_vp, _vp, _vp, _u32, _vp]),
     "gr_guide_mixture_pdf": (_i, [_vp, _vp, _vp, _u32, _vp]),
     "gr_ctx_set_kernel_timing": (_i, [_vp, _i]),

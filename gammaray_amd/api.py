@@ -45,6 +45,13 @@ class Context:
     def synchronize(self):
         check(lib().gr_ctx_synchronize(self._h), self._h, "synchronize")
 
+    def take_stats(self):
+        """The device-side error counters of this context's render calls since
+        the previous call (gr_ctx_take_stats), as a dict; clears them."""
+        st = _lib.Stats()
+        check(lib().gr_ctx_take_stats(self._h, ctypes.byref(st)), self._h, "take_stats")
+        return {k: getattr(st, k) for k, _ in _lib.Stats._fields_ if k != "reserved"}
+
     KERNEL_KINDS = ("trace_closest", "trace_shadow", "ray_sort")
 
     def set_kernel_timing(self, enable=True):
@@ -350,6 +357,16 @@ class Guide:
                                       ctypes.c_void_p(out.data_ptr())), self.ctx.handle, "guide_gradient")
         return out
 
+    def last_records(self, cap=1 << 20):
+        """The last traced batch's training records as an (n, 12) float32 device
+        tensor (8 raw inputs, direction, value)."""
+        import torch
+        out = torch.empty((max(cap, 1), 12), dtype=torch.float32, device="cuda")
+        n = ctypes.c_uint32()
+        check(lib().gr_guide_last_records(self._h, ctypes.c_void_p(out.data_ptr()), cap, ctypes.byref(n)),
+              self.ctx.handle, "guide_last_records")
+        return out[:n.value]
+
     def train(self, records, steps):
         check(lib().gr_guide_train(self._h, ctypes.c_void_p(records.data_ptr()), records.shape[0], steps),
               self.ctx.handle, "guide_train")
@@ -402,11 +419,18 @@ def trace(stage, film, integrator="pt", spp=1, max_depth=0, light_depth=0, seed=
           stage.ctx.handle, "trace")
 
 
-def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, use_graph=True, tuning=None):
+JOB_ORDERS = {"as_given": _lib.GR_JOBS_AS_GIVEN, "fewest_first": _lib.GR_JOBS_FEWEST_FIRST}
+
+
+def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, use_graph=True, tuning=None,
+                job_order="as_given", sample_ring=0):
     """Render many jobs through one shared pool of in-flight paths.  jobs: list
     of dicts with stage, film, spp, max_depth, seed and optionally
-    sample_offset and camera (a CameraSpec; default: the stage's).  Returns the
-    per-job latency in ms (batch start to film final)."""
+    sample_offset and camera (a CameraSpec; default: the stage's).  job_order:
+    "as_given" or "fewest_first" (the order jobs take samples); sample_ring:
+    finished samples held before their film gather (0 = 16 x path_pool, at
+    most the batch's samples).
+    Returns the per-job latency in ms (batch start to film final)."""
     n = len(jobs)
     arr = (_lib.Job * n)()
     for i, j in enumerate(jobs):
@@ -422,6 +446,8 @@ def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, u
     p.path_pool = path_pool
     p.steps_per_poll = steps_per_poll
     p.use_graph = 1 if use_graph else 0
+    p.job_order = JOB_ORDERS[job_order]
+    p.sample_ring = sample_ring
     if tuning:
         t = dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=56, xcd_ranges=0)
         t.update(tuning)

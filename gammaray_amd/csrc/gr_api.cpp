@@ -95,6 +95,25 @@ static void drop_timed(gr_ctx* ctx) {
     ctx->timed.clear();
 }
 
+namespace gr {
+void ctx_retain(gr_ctx* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void ctx_release(gr_ctx* ctx) {
+    if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    if (ctx->device != GR_HOST_ONLY) {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        drop_timed(ctx);
+        if (ctx->own_stream) {
+            (void)hipStreamSynchronize(ctx->own_stream);
+            (void)hipStreamDestroy(ctx->own_stream);
+        }
+        (void)hipFree(ctx->d_stats);
+    }
+    delete ctx;
+}
+}  // namespace gr
+
 extern "C" {
 
 int gr_ctx_create(int device, gr_ctx** out) {
@@ -117,7 +136,11 @@ int gr_ctx_create(int device, gr_ctx** out) {
     // a caller's uploads often run.
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);
     if (e == hipSuccess) e = gr::noalloc_setup(device);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->d_stats), gr::kStats * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_stats, 0, gr::kStats * sizeof(unsigned long long), c->own_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->own_stream);
     if (e != hipSuccess) {
+        (void)hipFree(c->d_stats);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
         delete c;
         return GR_ERR_HIP;
@@ -129,18 +152,7 @@ int gr_ctx_create(int device, gr_ctx** out) {
 
 void gr_ctx_destroy(gr_ctx* ctx) {
     if (!ctx) return;
-    if (ctx->device == GR_HOST_ONLY) {
-        delete ctx;
-        return;
-    }
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    drop_timed(ctx);
-    if (ctx->own_stream) {
-        (void)hipStreamSynchronize(ctx->own_stream);
-        (void)hipStreamDestroy(ctx->own_stream);
-    }
-    delete ctx;
+    gr::ctx_release(ctx);
 }
 
 int gr_ctx_set_stream(gr_ctx* ctx, void* stream) {
@@ -197,23 +209,43 @@ int gr_ctx_take_kernel_times(gr_ctx* ctx, double* ms, uint64_t* launches) {
     return GR_OK;
 }
 
+int gr_ctx_take_stats(gr_ctx* ctx, gr_stats* out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    unsigned long long v[gr::kStats] = {0};
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(v, ctx->d_stats, sizeof(v), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->d_stats, 0, sizeof(v), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "stats readback");
+    std::memset(out, 0, sizeof(*out));
+    out->nonfinite_bsdf_samples = v[gr::kStatNonfiniteBsdf];
+    out->nonfinite_radiance = v[gr::kStatNonfiniteRadiance];
+    out->bad_splats = v[gr::kStatBadSplats];
+    return GR_OK;
+}
+
 int gr_scene_create(gr_ctx* ctx, gr_scene** out) {
     if (!ctx || !out) return GR_ERR_INVALID;
     gr_scene* s = new (std::nothrow) gr_scene();
     if (!s) return fail(ctx, GR_ERR_NOMEM, "out of host memory%s");
     s->ctx = ctx;
+    gr::ctx_retain(ctx);
     *out = s;
     return GR_OK;
 }
 
 void gr_scene_destroy(gr_scene* s) {
     if (!s) return;
-    if (s->ctx->device != GR_HOST_ONLY) {
-        (void)hipSetDevice(s->ctx->device);
-        (void)hipStreamSynchronize(s->ctx->stream);
+    gr_ctx* ctx = s->ctx;
+    if (ctx->device != GR_HOST_ONLY) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
         free_device(s);
     }
     delete s;
+    gr::ctx_release(ctx);
 }
 
 int gr_scene_add_mesh(gr_scene* s, const float* verts, uint32_t nverts, const float* normals,
@@ -530,12 +562,14 @@ int gr_film_create(gr_ctx* ctx, uint32_t width, uint32_t height, gr_film** out) 
         delete f;
         return hip_fail(ctx, e, "film allocation");
     }
+    gr::ctx_retain(ctx);
     *out = f;
     return GR_OK;
 }
 
 void gr_film_destroy(gr_film* f) {
     if (!f) return;
+    (void)hipSetDevice(f->ctx->device);
     (void)hipStreamSynchronize(f->ctx->stream);
     if (f->copy_stream) (void)hipStreamSynchronize(f->copy_stream);
     (void)hipFree(f->d_snapshot);
@@ -545,7 +579,9 @@ void gr_film_destroy(gr_film* f) {
     if (f->copy_stream) (void)hipStreamDestroy(f->copy_stream);
     (void)hipFree(f->d_pixels);
     release_render_buffers(f->rb);
+    gr_ctx* ctx = f->ctx;
     delete f;
+    gr::ctx_release(ctx);
 }
 
 int gr_film_clear(gr_film* f) {
@@ -638,6 +674,8 @@ int gr_render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr_bat
     if (prm->tuning.lds_stack != 0 && prm->tuning.lds_stack != 8 && prm->tuning.lds_stack != 16 &&
         prm->tuning.lds_stack != 32)
         return fail(ctx, GR_ERR_INVALID, "tuning out of range%s");
+    if (prm->job_order != GR_JOBS_AS_GIVEN && prm->job_order != GR_JOBS_FEWEST_FIRST)
+        return fail(ctx, GR_ERR_INVALID, "unknown job order%s");
     for (uint32_t j = 0; j < njobs; ++j) {
         const gr_job& jb = jobs[j];
         if (!jb.scene || !jb.film) return fail(ctx, GR_ERR_INVALID, "job without scene or film%s");

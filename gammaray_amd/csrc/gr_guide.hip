@@ -829,6 +829,7 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
         delete g;
         return gr::hip_error(ctx, e, "guide initialisation");
     }
+    gr::ctx_retain(ctx);
     *out = g;
     return GR_OK;
 }
@@ -840,7 +841,9 @@ void gr_guide_destroy(gr_guide* g) {
     (void)hipFree(g->blob);
     (void)hipFree(g->work);
     (void)hipFree(g->recs_blob);
+    gr_ctx* ctx = g->ctx;
     delete g;
+    gr::ctx_release(ctx);
 }
 
 int gr_guide_get_params(gr_guide* g, float* host, uint32_t n) {
@@ -899,6 +902,23 @@ int gr_guide_train(gr_guide* g, const float* d_records, uint32_t n, uint32_t ste
     if (e == hipSuccess) e = gr::guide_record_stats(g, d_records, n, nullptr, s);
     if (e == hipSuccess) e = gr::guide_adam_steps(g, d_records, n, nullptr, steps, s);
     return e == hipSuccess ? GR_OK : gr::hip_error(g->ctx, e, "guide training");
+}
+
+int gr_guide_last_records(gr_guide* g, float* d_out, uint32_t cap, uint32_t* n_out) {
+    if (!g || !n_out || (cap && !d_out)) return GR_ERR_INVALID;
+    hipStream_t s = g->ctx->stream;
+    uint32_t count = 0;
+    hipError_t e = g->recs ? hipMemcpyAsync(&count, g->count, sizeof(count), hipMemcpyDeviceToHost, s) : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return gr::hip_error(g->ctx, e, "record count");
+    uint32_t n = g->recs ? count : 0u;
+    n = std::min<uint32_t>(n, (uint32_t)std::min<size_t>(g->recs_cap, 0xffffffffu));
+    n = std::min(n, cap);
+    if (n) e = hipMemcpyAsync(d_out, g->recs, (size_t)n * gr::kRecordFloats * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return gr::hip_error(g->ctx, e, "record copy");
+    *n_out = n;
+    return GR_OK;
 }
 
 int gr_guide_mixture_sample(gr_ctx* ctx, const float* d_mix, const float* d_u, uint32_t n, float* d_out) {

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -144,6 +145,9 @@ struct TraceTuning {
 // Ray counters of the persistent traversal: one per XCD range.
 constexpr uint32_t kRayCounters = 8;
 
+// Device-side error counters of a context (gr_stats, in this order).
+enum StatIndex : int { kStatNonfiniteBsdf = 0, kStatNonfiniteRadiance = 1, kStatBadSplats = 2, kStats = 4 };
+
 // Everything a traversal launch needs besides its rays.  With `scenes` set,
 // every ray names its scene in the pad word after its origin (an index into
 // `scenes`, nscenes entries); sv then only supplies the work counter, the
@@ -197,11 +201,16 @@ struct ShadeView {
 
 // ---- ABI objects ----------------------------------------------------------
 struct gr_ctx {
+    // One reference for the caller's handle and one per live scene, film and
+    // guide: gr_ctx_destroy drops the handle's, and the context is freed when
+    // the last reference goes, so objects that outlive the handle stay valid.
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
     bool timing = false;
+    unsigned long long* d_stats = nullptr;  // gr::kStats error counters
     struct TimedLaunch {
         int kind;
         hipEvent_t begin, end;
@@ -302,6 +311,9 @@ struct gr_film {
 };
 
 namespace gr {
+// Context references held by scenes, films and guides (gr_api.cpp).
+void ctx_retain(gr_ctx* ctx);
+void ctx_release(gr_ctx* ctx);
 inline TraceSetup trace_setup(gr_scene* s) {
     return TraceSetup{s->view, s->tuning, &s->sort, s->ctx->timing ? s->ctx : nullptr};
 }

@@ -481,7 +481,25 @@ struct RenderArgs {
     float wx, wy;
     RenderBuffers rb;
     GuideArgs gd;
+    unsigned long long* stats;  // the context's error counters (kStat*)
+    // Film gather from a ring of samples (batched jobs): iteration `it`'s samples
+    // start at ring slot (ring_base + it * W * H) mod ring_cap.  ring_cap 0:
+    // rb.L / rb.st hold the iterations one after another from index 0.
+    uint64_t ring_base, ring_cap;
 };
+
+__device__ __forceinline__ bool finite3(vec3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// Counts a rare event in the context's error counters.
+__device__ __forceinline__ void note_stat(unsigned long long* stats, int which, bool event) {
+    if (event && stats) atomicAdd(stats + which, 1ull);
+}
+
+// The conditions the reference asserts on every material sample
+// (debug_assert!s, src/material/material.rs:224-226).
+__device__ __forceinline__ bool bad_sample(const dev::MatSample& smp) {
+    return !(finite3(smp.radiance) && isfinite(smp.pdf) && smp.pdf > 0.0f);
+}
 
 // Lobes whose reported pdf is the density their sampling draws from: the
 // guide is mixed with these only.  The clearcoat pdf lacks the cos(theta_h)
@@ -629,6 +647,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
             const uint4 r0 = draws(seed, path, kStreamCamera, depth, 0);
             if (!NEE) {
                 const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
                 L = L + mul(T, smp.emission);
                 T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
                 new_o = hp + smp.outgoing * kRayPush;
@@ -681,6 +700,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                     const bool split = GUIDED && gs != kNone && lobe_consistent(m.lobes[pick]);
                     if (!split) {
                         const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                        note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
                         T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
                         a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
                         out_dir = smp.outgoing;
@@ -711,6 +731,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                         vec3 g = mk(0.0f, 0.0f, 0.0f);
                         if (s_all > 0.0f && dens > 0.0f)
                             g = dev::f_world(m, wi, out_dir, f, true) * ((s_c / s_all) * fabsf(dot(f.n, out_dir)) / dens);
+                        note_stat(a.stats, kStatNonfiniteBsdf, !finite3(g));
                         T = mul(T, g);
                         a.rb.aux[p] = make_float4(s_all / (float)m.nlobes, 0.0f, 0.0f, 0.0f);
                     }
@@ -930,10 +951,13 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
     const int last_col = W - 1, last_row = H - 1;
     const uint32_t npx = a.width * a.height;
     for (uint32_t it = 0; it < iters; ++it) {
+        const uint64_t it0 = a.ring_cap ? (a.ring_base + (uint64_t)it * npx) % a.ring_cap : (uint64_t)it * npx;
         for (int k = threadIdx.x; k < kSpan * kSpan; k += kTile * kTile) {
             const int sx = x0 - kApron + k % kSpan, sy = y0 - kApron + k / kSpan;
             if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
-                const size_t p = (size_t)it * npx + (size_t)sy * W + sx;
+                uint64_t q = it0 + (uint64_t)sy * W + sx;
+                if (a.ring_cap && q >= a.ring_cap) q -= a.ring_cap;  // q < npx + ring_cap <= 2 ring_cap
+                const size_t p = (size_t)q;
                 const float2 st = a.rb.st[p];
                 const float4 L = a.rb.L[p];
                 const float col_cont = 0.0f + (0.5f * (st.x + 1.0f)) * (wf - 0.0f);
@@ -948,6 +972,9 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
                 s_r[k] = L.x;
                 s_g[k] = L.y;
                 s_b[k] = L.z;
+                // counted once, by the tile that holds the sample's own pixel
+                const bool home = sx >= x0 && sx < x0 + kTile && sy >= y0 && sy < y0 + kTile;
+                note_stat(a.stats, kStatNonfiniteRadiance, home && !finite3(mk(L.x, L.y, L.z)));
             }
         }
         __syncthreads();
@@ -1134,6 +1161,7 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
             const GpuMaterial& m = a.sh.mats[h.prim];
             const uint4 r0 = draws(a.seed, a.path_base + p, CAMERA ? kStreamCamera : kStreamLight, k, 0);
             const dev::MatSample smp = dev::sample_world(m, wi, f, CAMERA, r0.x, u01(r0.y), u01(r0.z));
+            note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
             const bool conn = dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0;
             float a_fwd = 0.0f;
             if (b.mis && k > 0) {
@@ -1504,6 +1532,13 @@ __global__ __launch_bounds__(256) void k_lt_splat(RenderArgs a, BdptArgs b, uint
         const uint32_t s = b.lt_base + l - 1;
         const float4 v = b.slots[(size_t)s * chunk + q];
         if (v.w == 0.0f || !(b.mask[(size_t)p * b.mwords + s / 32] & (1u << (s % 32)))) continue;
+        // a non-finite contribution is dropped; one beyond the fixed-point range
+        // (|v| wt * 2^32 > 9.2e18 with wt <= 1) is clamped
+        if (!finite3(mk(v.x, v.y, v.z))) {
+            note_stat(a.stats, kStatBadSplats, true);
+            continue;
+        }
+        note_stat(a.stats, kStatBadSplats, fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) > 2.0e9f);
         const float4 pb = b.lig[(size_t)(l - 1) * a.npaths + p].b;
         float cd, rd, cos_t;
         vec3 to_cam;
@@ -1912,6 +1947,8 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     a.wx = cam.horizontal_aperture / (cam.focal_length * 2.0f);  // window_max, camera.rs:76-79
     a.wy = cam.vertical_aperture / (cam.focal_length * 2.0f);
     a.rb = film->rb;
+    a.stats = sc->ctx->d_stats;
+    a.ring_base = a.ring_cap = 0;
     std::memset(&a.gd, 0, sizeof(a.gd));
     const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE || guided;
     uint32_t nslots = 0;
@@ -2066,9 +2103,8 @@ struct JobDev {
     uint32_t width, height, max_depth, scene;
     uint64_t seed, path_base0;  // path id of the job's sample 0: sample_offset * width * height
     uint64_t first;             // index of the job's first sample among all jobs' samples
-    uint32_t samples, pad;
-    float4* L;                  // per sample: radiance (iteration-major, as gr_trace's batches)
-    float2* st;                 // per sample: film position
+    uint32_t samples;
+    uint32_t iter0;             // index of the job's first iteration among all jobs' iterations
 };
 
 struct BatchDev {
@@ -2091,14 +2127,23 @@ struct BatchDev {
     // [1] paths started this step, [3] their first queue position, [4] paths retired this step
     uint32_t* ctr;
     unsigned long long* issued;  // [0] samples started so far, [1] first sample of this step
-    uint32_t* done;              // per job: kDoneSpread counters of finished samples (summed by the host)
+    // Finished samples wait in a ring indexed by their global sample number:
+    // radiance and film position at slot (job first + sample) mod ring.
+    float4* ring_L;
+    float2* ring_st;
+    unsigned long long ring;
+    // Samples may start only below *limit (set by the host: the first sample
+    // not yet gathered into its film, plus ring), so the ring never overruns.
+    const unsigned long long* limit;
+    uint32_t* done;              // per iteration of every job: kDoneSpread counters of finished samples
+    uint32_t niters;
 };
-constexpr uint32_t kDoneSpread = 64;  // spreads the counter updates of one job over 64 addresses
+constexpr uint32_t kDoneSpread = 16;  // spreads the counter updates of one iteration over 16 addresses
 constexpr uint32_t kNextSide = 0x80000000u;  // shadow-ray owner flag: the path's state is in the next queue
 
 __global__ void k_batch_init(RenderArgs a, BatchDev b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < b.njobs * kDoneSpread) b.done[i] = 0u;
+    if (i < b.niters * kDoneSpread) b.done[i] = 0u;
     if (i == 0) {
         b.ctr[1] = b.ctr[3] = b.ctr[4] = 0u;
         b.issued[0] = b.issued[1] = 0ull;
@@ -2113,7 +2158,10 @@ __global__ void k_batch_plan(RenderArgs a, BatchDev b, uint32_t cur) {
     if (threadIdx.x != 0) return;
     const uint32_t q = a.rb.counts[cur];  // continuing paths at queue positions [0, q)
     const unsigned long long issued = b.issued[0];
-    const unsigned long long rem = b.total - issued;
+    const unsigned long long lim = *b.limit;
+    unsigned long long rem = b.total - issued;
+    const unsigned long long avail = lim > issued ? lim - issued : 0ull;
+    rem = avail < rem ? avail : rem;
     const uint32_t room = b.pool - q;
     const uint32_t k = rem < (unsigned long long)room ? (uint32_t)rem : room;
     b.ctr[1] = k;
@@ -2218,28 +2266,29 @@ __global__ __launch_bounds__(256) void k_batch_shadow_accum(RenderArgs a, BatchD
 }
 
 // Finished paths (their last light sample already added): radiance and film
-// position to the job.
+// position to the sample ring, one count to their iteration.
 __global__ __launch_bounds__(256) void k_batch_retire(RenderArgs a, BatchDev b, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < b.ctr[4];
-    uint32_t j = kNone;
+    uint32_t it = kNone;
     if (ok) {
         const uint32_t p = b.retire[i];
         const uint4 m = b.meta[cur][p];
-        j = m.x;
-        const JobDev& J = b.jobs[j];
-        J.L[m.y] = b.L[cur][p];
-        J.st[m.y] = b.st[cur][p];
+        const JobDev& J = b.jobs[m.x];
+        const size_t slot = (size_t)((J.first + m.y) % b.ring);
+        b.ring_L[slot] = b.L[cur][p];
+        b.ring_st[slot] = b.st[cur][p];
+        it = J.iter0 + m.y / (J.width * J.height);
     }
-    // one counter update per distinct job in the wave (finished paths come in
-    // job order, so a wave usually holds one or two jobs)
+    // one counter update per distinct iteration in the wave (finished paths come
+    // roughly in sample order, so a wave usually holds one or two iterations)
     uint64_t left = __ballot(ok);
     while (left) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(left);
-        const uint32_t lj = __shfl(j, (int)lead);
-        const uint64_t same = __ballot(ok && j == lj);
+        const uint32_t li = __shfl(it, (int)lead);
+        const uint64_t same = __ballot(ok && it == li);
         if ((threadIdx.x & 63u) == lead)
-            atomicAdd(&b.done[lj * kDoneSpread + blockIdx.x % kDoneSpread], (uint32_t)__popcll(same));
+            atomicAdd(&b.done[(size_t)li * kDoneSpread + blockIdx.x % kDoneSpread], (uint32_t)__popcll(same));
         left &= ~same;
     }
 }
@@ -2250,16 +2299,24 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     const bool nee = prm.integrator == GR_INTEGRATOR_PT_NEE;
     const uint32_t pool = prm.path_pool ? prm.path_pool : (1u << 23);
     const uint32_t poll = prm.steps_per_poll ? prm.steps_per_poll : 16u;
-    // distinct scenes, job table
+    // generation order: the jobs as given, or fewest samples first (stable)
+    std::vector<uint32_t> order(njobs);
+    for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
+    auto job_samples = [&](uint32_t j) { return (uint64_t)jobs[j].film->width * jobs[j].film->height * jobs[j].spp; };
+    if (prm.job_order == GR_JOBS_FEWEST_FIRST)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return job_samples(x) < job_samples(y); });
+    // distinct scenes, job table (in generation order: entry g is job order[g])
     std::vector<gr_scene*> scenes;
     std::vector<JobDev> hj(njobs);
-    uint64_t total = 0;
-    for (uint32_t j = 0; j < njobs; ++j) {
-        const gr_job& jb = jobs[j];
+    uint64_t total = 0, max_npx = 0;
+    uint32_t niters = 0;
+    for (uint32_t g = 0; g < njobs; ++g) {
+        const gr_job& jb = jobs[order[g]];
         uint32_t si = 0;
         while (si < scenes.size() && scenes[si] != jb.scene) ++si;
         if (si == scenes.size()) scenes.push_back(jb.scene);
-        JobDev& d = hj[j];
+        JobDev& d = hj[g];
         std::memset(&d, 0, sizeof(d));
         d.sv = jb.scene->view;
         d.sh = jb.scene->shade;
@@ -2272,11 +2329,21 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         d.scene = si;
         d.seed = jb.seed;
         const uint64_t npx = (uint64_t)d.width * d.height;
+        max_npx = std::max(max_npx, npx);
         d.path_base0 = jb.sample_offset * npx;
         d.first = total;
         d.samples = (uint32_t)(npx * jb.spp);
+        d.iter0 = niters;
         total += d.samples;
+        niters += jb.spp;
     }
+    // The sample ring: at least one whole iteration of the largest film, so the
+    // oldest ungathered iteration always fits below the generation limit.  The
+    // limit moves only at polls, and a poll interval generates about
+    // steps_per_poll * pool / (mean path length) samples, so the default holds
+    // 16 pools' worth (no more than the batch's samples).
+    uint64_t ring = prm.sample_ring ? prm.sample_ring : std::min<uint64_t>(16ull * pool, total);
+    ring = std::max(ring, max_npx);
     std::vector<SceneRef> hs(scenes.size());
     for (size_t k = 0; k < scenes.size(); ++k) {
         const SceneView& v = scenes[k]->view;
@@ -2298,13 +2365,13 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         tu.xcd_ranges = prm.tuning.xcd_ranges;
     }
 
-    // device memory: job and scene tables, per-sample outputs, slot state, counters
+    // device memory: job and scene tables, the sample ring, path state, counters
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t off = 0;
     const size_t o_jobs = off; off = align(off + njobs * sizeof(JobDev));
     const size_t o_scenes = off; off = align(off + scenes.size() * sizeof(SceneRef));
-    const size_t o_L = off; off = align(off + total * sizeof(float4));
-    const size_t o_st = off; off = align(off + total * sizeof(float2));
+    const size_t o_L = off; off = align(off + ring * sizeof(float4));
+    const size_t o_st = off; off = align(off + ring * sizeof(float2));
     // the second queue side's path state (the first side uses the film buffers' T, L, aux, st)
     const size_t o_T1 = off; off = align(off + (size_t)pool * 16);
     const size_t o_L1 = off; off = align(off + (size_t)pool * 16);
@@ -2316,36 +2383,32 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     const size_t o_ret = off; off = align(off + (size_t)pool * 4);
     const size_t o_ctr = off; off = align(off + 64);
     const size_t o_iss = off; off = align(off + 16);
-    const size_t o_done = off; off = align(off + (size_t)njobs * kDoneSpread * 4);
+    const size_t o_lim = off; off = align(off + 8);
+    const size_t o_done = off; off = align(off + (size_t)niters * kDoneSpread * 4);
     char* blob = nullptr;
     RenderBuffers rb;
     SortScratch ss;
     uint32_t* h_done = nullptr;
+    unsigned long long* h_limit = nullptr;
     hipStream_t s = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev_start = nullptr, ev_poll = nullptr, ev_ctx = nullptr;
-    std::vector<hipEvent_t> ev_job(njobs, nullptr);
+    std::vector<hipEvent_t> ev_job(njobs, nullptr);  // by generation index
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&blob), off);
     if (e == hipSuccess) e = reserve(rb, pool);
     if (e == hipSuccess && tu.sort_bits) e = reserve_sort(ss, pool);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), (size_t)njobs * kDoneSpread * 4);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), (size_t)niters * kDoneSpread * 4);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_limit), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ev_poll);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_ctx, hipEventDisableTiming);
     for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) e = hipEventCreate(&ev_job[j]);
-    std::vector<float4*> job_L(njobs);
-    std::vector<float2*> job_st(njobs);
     std::vector<unsigned long long> job_first(njobs);
-    for (uint32_t j = 0; j < njobs; ++j) job_first[j] = hj[j].first;
+    for (uint32_t g = 0; g < njobs; ++g) job_first[g] = hj[g].first;
     if (e == hipSuccess) {
-        for (uint32_t j = 0; j < njobs; ++j) {
-            hj[j].L = reinterpret_cast<float4*>(blob + o_L) + hj[j].first;
-            hj[j].st = reinterpret_cast<float2*>(blob + o_st) + hj[j].first;
-            job_L[j] = hj[j].L;
-            job_st[j] = hj[j].st;
-        }
+        *h_limit = ring;  // nothing gathered yet
         // start after the work already on the context's stream
         e = hipEventRecord(ev_ctx, ctx->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, ev_ctx, 0);
@@ -2355,10 +2418,12 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             e = hipMemcpyAsync(blob + o_scenes, hs.data(), hs.size() * sizeof(SceneRef), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
             e = hipMemcpyAsync(blob + o_first, job_first.data(), njobs * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(blob + o_lim, h_limit, 8, hipMemcpyHostToDevice, s);
     }
     RenderArgs a;
     std::memset(&a, 0, sizeof(a));
     a.rb = rb;
+    a.stats = ctx->d_stats;
     BatchDev b;
     b.jobs = reinterpret_cast<const JobDev*>(blob + o_jobs);
     b.njobs = njobs;
@@ -2378,7 +2443,12 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     b.retire = reinterpret_cast<uint32_t*>(blob + o_ret);
     b.ctr = reinterpret_cast<uint32_t*>(blob + o_ctr);
     b.issued = reinterpret_cast<unsigned long long*>(blob + o_iss);
+    b.ring_L = reinterpret_cast<float4*>(blob + o_L);
+    b.ring_st = reinterpret_cast<float2*>(blob + o_st);
+    b.ring = ring;
+    b.limit = reinterpret_cast<const unsigned long long*>(blob + o_lim);
     b.done = reinterpret_cast<uint32_t*>(blob + o_done);
+    b.niters = niters;
 
     TraceSetup ts{scenes[0]->view, tu, &ss, nullptr, reinterpret_cast<const SceneRef*>(blob + o_scenes),
                   (uint32_t)scenes.size()};
@@ -2405,7 +2475,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         return hipGetLastError();
     };
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(1u, njobs * kDoneSpread)), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(k_batch_init, blocks_for(std::max(1u, niters * kDoneSpread)), dim3(256), 0, s, a, b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev_start, s);
@@ -2422,12 +2492,13 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     }
-    std::vector<char> gathered(njobs, 0);
+    std::vector<uint32_t> gathered(njobs, 0);  // iterations gathered, by generation index
     uint32_t pending = njobs;
     uint64_t pairs = 0;
     // Every path ends: past 20 bounces Russian roulette keeps at most 3/4 of
-    // them per bounce.  A workload needs about (samples / pool) * depth steps;
-    // a count far beyond that means the counters are wrong -- stop with an error.
+    // them per bounce.  A workload needs about (samples / pool) * depth steps
+    // (more when the ring holds generation back); a count far beyond that means
+    // the counters are wrong -- stop with an error.
     const uint64_t max_pairs = (total / pool + 2) * 4096ull;
     while (e == hipSuccess && pending > 0) {
         for (uint32_t k = 0; k < (poll + 1) / 2 && e == hipSuccess; ++k) {
@@ -2440,34 +2511,55 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             ++pairs;
         }
         if (e == hipSuccess)
-            e = hipMemcpyAsync(h_done, b.done, (size_t)njobs * kDoneSpread * 4, hipMemcpyDeviceToHost, s);
+            e = hipMemcpyAsync(h_done, b.done, (size_t)niters * kDoneSpread * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipEventRecord(ev_poll, s);
         if (e == hipSuccess) e = hipEventSynchronize(ev_poll);
-        for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) {
-            if (gathered[j]) continue;
-            uint64_t fin = 0;
-            for (uint32_t k = 0; k < kDoneSpread; ++k) fin += h_done[(size_t)j * kDoneSpread + k];
-            if (fin < hj[j].samples) continue;
-            RenderArgs g = a;
-            g.width = hj[j].width;
-            g.height = hj[j].height;
-            g.rb.L = job_L[j];
-            g.rb.st = job_st[j];
-            const dim3 grid((g.width + kTile - 1) / kTile, (g.height + kTile - 1) / kTile);
-            hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, s, g, jobs[j].film->d_pixels,
-                               jobs[j].spp, nullptr, nullptr);
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipEventRecord(ev_job[j], s);
-            gathered[j] = 1;
-            --pending;
+        // each job's iterations whose samples are all in, in order, into its film
+        uint64_t lo = total;  // first sample not yet gathered
+        for (uint32_t g = 0; g < njobs && e == hipSuccess; ++g) {
+            const JobDev& d = hj[g];
+            const gr_job& jb = jobs[order[g]];
+            const uint32_t npx = d.width * d.height;
+            uint32_t k1 = gathered[g];
+            while (k1 < jb.spp) {
+                uint64_t fin = 0;
+                for (uint32_t c = 0; c < kDoneSpread; ++c) fin += h_done[(size_t)(d.iter0 + k1) * kDoneSpread + c];
+                if (fin < npx) break;
+                ++k1;
+            }
+            if (k1 > gathered[g]) {
+                RenderArgs ga = a;
+                ga.width = d.width;
+                ga.height = d.height;
+                ga.rb.L = b.ring_L;
+                ga.rb.st = b.ring_st;
+                ga.ring_base = d.first + (uint64_t)gathered[g] * npx;
+                ga.ring_cap = ring;
+                const dim3 grid((ga.width + kTile - 1) / kTile, (ga.height + kTile - 1) / kTile);
+                hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, s, ga, jb.film->d_pixels,
+                                   k1 - gathered[g], nullptr, nullptr);
+                e = hipGetLastError();
+                if (e == hipSuccess && k1 == jb.spp) {
+                    e = hipEventRecord(ev_job[g], s);
+                    --pending;
+                }
+                gathered[g] = k1;
+            }
+            if (k1 < jb.spp) lo = std::min(lo, d.first + (uint64_t)k1 * npx);
+        }
+        if (e == hipSuccess && pending > 0) {
+            // the gathers above run before the next steps (same stream), so their
+            // ring slots may be reused from here on
+            *h_limit = lo + ring;
+            e = hipMemcpyAsync(blob + o_lim, h_limit, 8, hipMemcpyHostToDevice, s);
         }
         if (e == hipSuccess && pairs > max_pairs) e = hipErrorLaunchTimeOut;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    for (uint32_t j = 0; j < njobs && e == hipSuccess && job_ms; ++j) {
+    for (uint32_t g = 0; g < njobs && e == hipSuccess && job_ms; ++g) {
         float ms = 0.0f;
-        e = hipEventElapsedTime(&ms, ev_start, ev_job[j]);
-        job_ms[j] = ms;
+        e = hipEventElapsedTime(&ms, ev_start, ev_job[g]);
+        job_ms[order[g]] = ms;
     }
     // the context's stream continues after the batch
     if (ev_ctx && s) {
@@ -2484,6 +2576,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     if (ev_ctx) (void)hipEventDestroy(ev_ctx);
     if (s) (void)hipStreamDestroy(s);
     if (h_done) (void)hipHostFree(h_done);
+    if (h_limit) (void)hipHostFree(h_limit);
     release_sort(ss);
     release_render_buffers(rb);
     (void)hipFree(blob);

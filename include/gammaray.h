@@ -52,6 +52,9 @@ typedef struct { float sx, sy, sz, pad0, tx, ty, tz, pad1; } gr_segment;
  * can be built and their tree exported, but nothing can be traced. */
 #define GR_HOST_ONLY (-1)
 int gr_ctx_create(int device, gr_ctx **out);
+/* Releases the caller's handle.  Scenes, films and guides created on the
+ * context keep it alive: it is freed when the last of them is destroyed, so
+ * they may be destroyed after the context in any order. */
 void gr_ctx_destroy(gr_ctx *ctx);
 /* Run subsequent work on an existing HIP stream (hipStream_t; NULL is the
  * legacy default stream).  gr_ctx_use_own_stream() returns to the context's
@@ -61,6 +64,23 @@ int gr_ctx_use_own_stream(gr_ctx *ctx);
 void *gr_ctx_stream(gr_ctx *ctx);
 int gr_ctx_synchronize(gr_ctx *ctx);
 const char *gr_ctx_last_error(gr_ctx *ctx);
+
+/* Device-side error counters of the context's render calls (gr_trace,
+ * gr_render_jobs), accumulated since the previous gr_ctx_take_stats.  They
+ * stand in for the reference's debug assertions; a release build of the
+ * reference lets the values through, and so does this library (a non-finite
+ * sample reaches the film as in the reference). */
+typedef struct {
+    uint64_t nonfinite_bsdf_samples; /* material samples whose radiance or pdf is not finite, or whose
+                                        pdf is negative (debug_assert!s, src/material/material.rs:224-226) */
+    uint64_t nonfinite_radiance;     /* camera samples whose radiance is not finite when they reach the
+                                        film (Film::report_samples, src/render/film.rs:78-114) */
+    uint64_t bad_splats;             /* light-tracing contributions not finite (dropped) or beyond the
+                                        fixed-point splat range (clamped); GR_INTEGRATOR_BDPT_LT only */
+    uint64_t reserved;
+} gr_stats;
+/* Waits for the context's stream, returns the counters and clears them. */
+int gr_ctx_take_stats(gr_ctx *ctx, gr_stats *out);
 
 /* ---- scene (Stage::new / Bvh::build) ------------------------------------ */
 /* Material record: 16 floats, rec[0] = kind:
@@ -265,9 +285,11 @@ int gr_film_write_exr(gr_film *film, const char *path);
  * from a global counter over all jobs, one closest-hit launch traverses every
  * job's scene (each ray names its scene), shading and shadow rays follow,
  * finished paths hand their radiance to their job.  The step is captured once as a hipGraph
- * and replayed.  A job's film gets the Mitchell splat of its samples, in the
- * reference's order, as soon as its last sample finishes; it equals what
- * gr_trace renders for the same camera, seed, spp, depth and sample_offset. */
+ * and replayed.  Finished samples wait in a ring of sample_ring entries (24 bytes each); each
+ * job's film gets the Mitchell splat of its iterations, in order, as soon as an iteration's
+ * samples are all in, so memory is bounded by the pool and the ring, not by the jobs' total
+ * samples.  A job's film equals what gr_trace renders for the same camera, seed, spp, depth
+ * and sample_offset. */
 typedef struct {
     gr_scene *scene;
     gr_film *film;             /* the resolution is the film's */
@@ -281,7 +303,13 @@ typedef struct {
     uint32_t steps_per_poll;   /* bounce steps between completion checks; 0 = 16 */
     uint32_t use_graph;        /* 1: replay the captured bounce step as a hipGraph; 0: launch eagerly */
     gr_tuning tuning;          /* the shared traversal launches; lds_stack 0 = the first job's scene's setting */
+    uint32_t job_order;        /* order in which jobs take samples: GR_JOBS_AS_GIVEN or GR_JOBS_FEWEST_FIRST */
+    uint32_t sample_ring;      /* finished samples held before their film gather; 0 = 16 x path_pool
+                                  (at most the batch's samples); raised to the largest job's pixel
+                                  count if smaller */
 } gr_batch_params;
+#define GR_JOBS_AS_GIVEN 0
+#define GR_JOBS_FEWEST_FIRST 1 /* fewest samples first (shorter jobs finish sooner: lower median latency) */
 /* All scenes and films must belong to ctx.  job_ms (may be NULL, njobs entries)
  * receives each job's latency: ms from the start of the batch until its film
  * is final. */
@@ -326,6 +354,10 @@ int gr_guide_infer(gr_guide *guide, const float *d_raw, uint32_t n, float *d_mix
 int gr_guide_gradient(gr_guide *guide, const float *d_records, uint32_t n, float *d_grad);
 /* `steps` Adam steps on batches drawn with replacement from n records (device). */
 int gr_guide_train(gr_guide *guide, const float *d_records, uint32_t n, uint32_t steps);
+/* The training records of the last traced batch (GR_GUIDE_RECORD_FLOATS each): at
+ * most cap of them are copied to d_out (device memory) and their number is written
+ * to *n_out.  Waits for the context's stream. */
+int gr_guide_last_records(gr_guide *guide, float *d_out, uint32_t cap, uint32_t *n_out);
 /* Test hooks: draw directions from n mixtures with 3 uniforms each (out: x, y, z,
  * pdf), and evaluate mixture densities of given directions. */
 int gr_guide_mixture_sample(gr_ctx *ctx, const float *d_mix, const float *d_u, uint32_t n, float *d_out);

@@ -11,6 +11,7 @@ the oracle, so for one seed both render the same paths:
     order);
   * independent seeds agree statistically (per-pixel 3 sigma).
 """
+import functools
 import os
 
 import numpy as np
@@ -19,6 +20,7 @@ import pytest
 from gammaray_amd import scenes
 from gammaray_amd.api import Film, Stage, material_query, trace
 from oracle import oracle as O
+from statcheck import assert_three_sigma
 
 pytestmark = pytest.mark.gpu
 
@@ -152,13 +154,7 @@ def test_independent_seeds_agree_statistically(gpu_ctx, integrator, oi):
     g = _sample_radiance(stage, film, integrator, spp, max_depth=6, light_depth=5, seed=101)
     o = O.OracleScene(spec)
     _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=6, light_depth=5, seed=202, colors=True)
-    lg, lr = g.sum(2), r.sum(2)  # (spp, pixels)
-    mg, mr = lg.mean(0), lr.mean(0)
-    se = np.sqrt(lg.var(0, ddof=1) / spp + lr.var(0, ddof=1) / spp)
-    z = np.abs(mg - mr) / np.maximum(se, 1e-12)
-    assert np.mean(z > 3.0) < 0.02, np.mean(z > 3.0)
-    tot = abs(lg.mean() - lr.mean()) / np.sqrt(lg.var() / lg.size + lr.var() / lr.size)
-    assert tot < 4.0, tot
+    assert_three_sigma(g.sum(2), r.sum(2), integrator)  # (spp, pixels)
 
 
 def test_resume_is_bit_exact(gpu_ctx):
@@ -197,7 +193,7 @@ def _image(film_rgbw):
 
 
 @pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("pt_nee", O.PT_NEE), ("bdpt", O.BDPT)])
-def test_rmse_against_cpu_4096spp_reference_agrees(gpu_ctx, integrator, oi):
+def test_rmse_against_cpu_4096spp_reference_paired_seed(gpu_ctx, integrator, oi):
     """GPU and CPU 64-spp renders have the same RMSE against a 4096-spp CPU
     render, within 2 %.  Both use one seed (the same addressed random numbers),
     so the comparison isolates the GPU's film and estimator from seed noise:
@@ -362,3 +358,131 @@ def test_full_size_1080p_paired_samples(gpu_ctx, integrator, oi, spp):
     trace(stage, film2, integrator, spp=spp, max_depth=8, light_depth=5, seed=31)
     gw = film2.readback().reshape(-1, 4)[:, 3]
     assert np.array_equal(gw.view(np.uint32), np.asarray(f, np.float32).reshape(-1, 4)[:, 3].view(np.uint32))
+
+
+def _white_box():
+    """The box with every Disney surface a bright diffuse (base 0.9, no
+    specular lobe): a closed room where paths bounce until Russian roulette or
+    the light ends them, many past 20 intersections."""
+    spec = scenes.box_scene()
+    for p in spec.prims:
+        if p.material[0] == 0:
+            p.material = scenes.disney(base_color=(0.9, 0.9, 0.9), roughness=0.8, ior=1.0)
+    return spec
+
+
+@pytest.mark.parametrize("integrator,oi", [("pt", O.PT), ("bdpt", O.BDPT)])
+def test_reference_mode_paired_with_oracle(gpu_ctx, integrator, oi):
+    """max_depth 0 (and light_depth 0): the reference's own loops.
+    * PT has no depth cap; Russian roulette past 10 intersections uses
+      clamped_lerp(0.25, 1, lum), and past 20 clamped_lerp(0.10, 0.75, lum)
+      (integrators.rs:36-37, 69-89).
+    * BDPT caps subpaths at 16 vertices, with roulette from the 4th
+      (integrators.rs:102-103, 196-208).
+    Same seed, so the same paths: per-sample radiance as in
+    test_paired_samples_match_oracle.  The oracle with a cap of 11 (21)
+    intersections loses over 30 % (10 %) of the image's energy.  That energy
+    is carried by paths past both thresholds."""
+    print("case reference mode", integrator, flush=True)
+    spec = _white_box()
+    W, H, spp = 32, 32, 16
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=0, light_depth=0, seed=27)
+    o = O.OracleScene(spec)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=0, light_depth=0, seed=27, colors=True)
+    assert np.isfinite(g).all()
+    close = np.all(np.isclose(g, r, rtol=1e-3, atol=1e-5), axis=2)
+    print("paired fraction %.4f" % close.mean(), flush=True)
+    assert close.mean() > 0.98, close.mean()
+    np.testing.assert_allclose(g.reshape(-1, 3).mean(0), r.reshape(-1, 3).mean(0), rtol=3e-2)
+    if integrator == "pt":
+        for cap, frac in ((11, 0.3), (21, 0.1)):
+            _, rc = O.render(o, spec, W, H, oi, spp=spp, max_depth=cap, seed=27, colors=True)
+            lost = 1.0 - rc.mean() / r.mean()
+            n_longer = int(np.any(rc != r, axis=2).sum())
+            print("cap %d: %.3f of the energy lost, %d samples differ" % (cap, lost, n_longer), flush=True)
+            assert lost > frac and n_longer > 50, (cap, lost, n_longer)
+
+
+@pytest.mark.parametrize("integrator,oi", [("pt_nee", O.PT_NEE), ("bdpt", O.BDPT), ("bdpt_mis", O.BDPT_MIS)])
+def test_independent_seeds_on_the_1080p_scene(gpu_ctx, integrator, oi):
+    """The 1080p scene (100k-triangle mesh, 200 spheres, two lights, depth 8,
+    light depth 5) at 96 x 54: GPU and oracle renders with different seeds
+    agree per pixel within 3 sigma (binomial bound) and in total."""
+    print("case 3sigma 1080p scene", integrator, flush=True)
+    spec = scenes.showcase_scene()
+    W, H, spp = 96, 54, 256
+    stage = Stage(spec, ctx=gpu_ctx)
+    film = Film(gpu_ctx, W, H)
+    g = _sample_radiance(stage, film, integrator, spp, max_depth=8, light_depth=5, seed=301)
+    o = O.OracleScene(spec)
+    _, r = O.render(o, spec, W, H, oi, spp=spp, max_depth=8, light_depth=5, seed=302, colors=True, threads=16)
+    assert_three_sigma(g.sum(2), r.sum(2), "1080p scene " + integrator)
+
+
+def _lambert_box():
+    spec = scenes.box_scene()
+    for p in spec.prims:
+        if p.material[0] == 0:
+            p.material = scenes.diffuse()
+    return spec
+
+
+_RMSE_W = _RMSE_H = 128
+
+
+@functools.lru_cache(maxsize=1)
+def _lambert_reference():
+    """The CPU oracle's 4096-spp PT+NEE render of the Lambertian box, unbounded
+    depth: every unbiased integrator below converges to it (BDPT's 16-vertex
+    cap drops paths carrying about 0.5^16 of the light)."""
+    spec = _lambert_box()
+    o = O.OracleScene(spec)
+    ref, _ = O.render(o, spec, _RMSE_W, _RMSE_H, O.PT_NEE, spp=4096, max_depth=0, seed=23, threads=16)
+    return _image(ref)
+
+
+def _rmse(img, ref, trim):
+    """RMSE over pixels; trim > 0 drops that fraction of the largest squared
+    errors first."""
+    d = ((img - ref) ** 2).sum(1) / 3.0
+    if trim:
+        d = np.sort(d)[:len(d) - int(round(trim * len(d)))]
+    return float(np.sqrt(d.mean()))
+
+
+@pytest.mark.parametrize("integrator,oi,trim,seeds", [("pt", O.PT, 0.0, 8), ("pt_nee", O.PT_NEE, 0.0, 16),
+                                                     ("bdpt_mis", O.BDPT_MIS, 0.001, 16),
+                                                     ("bdpt", O.BDPT, 0.001, 16)])
+def test_rmse_against_cpu_4096spp_reference_independent_seeds(gpu_ctx, integrator, oi, trim, seeds):
+    """RMSE against a 4096-spp CPU render, as the acceptance criterion states it:
+    over independent seeds (GPU seeds 600.., CPU seeds 500..), the mean 64-spp
+    RMSE of the GPU's renders and of the CPU oracle's agree within 2 %.
+    * Scene: the Lambertian box at 128 x 128, unbounded depth.
+    * Spread of one render's RMSE between seeds: about 1 % (PT), 1.7 % (PT+NEE),
+      1.3 % (BDPT+MIS, trimmed) and 1 % (BDPT, trimmed).
+    * The seed counts put the 2 % bound at 3 sigma or more of the mean's difference.
+    * BDPT trims: one pixel's firefly can raise a render's RMSE by 70 % (the
+      uniform 1/(c+l) weights, seed 602 of the CPU oracle), and BDPT+MIS's by
+      several percent.  Those RMSEs drop the largest 0.1 % of pixel errors.
+    With glossy materials a render's RMSE varies 5-10 % between seeds; the
+    paired-seed test above covers them."""
+    print("case rmse independent seeds", integrator, flush=True)
+    spec = _lambert_box()
+    W, H = _RMSE_W, _RMSE_H
+    R = _lambert_reference()
+    o = O.OracleScene(spec)
+    stage = Stage(spec, ctx=gpu_ctx)
+    e_cpu, e_gpu = [], []
+    for k in range(seeds):
+        cpu, _ = O.render(o, spec, W, H, oi, spp=64, max_depth=0, light_depth=0, seed=500 + k, threads=16)
+        e_cpu.append(_rmse(_image(cpu), R, trim))
+        film = Film(gpu_ctx, W, H)
+        trace(stage, film, integrator, spp=64, max_depth=0, light_depth=0, seed=600 + k)
+        e_gpu.append(_rmse(_image(film.readback()), R, trim))
+        film.close()
+    a, b = np.mean(e_gpu), np.mean(e_cpu)
+    print("rmse gpu %.6f cpu %.6f (seed spread %.3f / %.3f)" % (a, b, np.std(e_gpu) / a, np.std(e_cpu) / b),
+          flush=True)
+    assert abs(a - b) / b < 0.02, (a, b)

@@ -541,3 +541,17 @@ def test_batch_rejects_host_only_context(host_ctx):
     no_film = type("NoFilm", (), {"handle": None})()
     with pytest.raises(GrError):
         render_jobs(host_ctx, [dict(stage=st, film=no_film, spp=1)])
+
+
+def test_context_outlived_by_its_scene(host_ctx):
+    """A scene keeps its context alive: destroying the context's handle first
+    and the scene afterwards is safe (the reference's Rust ownership makes
+    the opposite order impossible; a C caller or a garbage collector may
+    use either)."""
+    from gammaray_amd import _lib
+    from gammaray_amd.api import Context, Stage
+    ctx = Context(_lib.GR_HOST_ONLY)
+    st = Stage(scenes.box_scene(), ctx=ctx)
+    ctx.close()
+    assert st.info()["num_prims"] == len(scenes.box_scene().prims)
+    st.close()
