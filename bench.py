@@ -68,6 +68,59 @@ def cpu_model():
     return "unknown"
 
 
+def host_cpus():
+    """What the host offers the CPU baseline: nproc, the CPUs this process may
+    run on, and the cgroup CPU quota (CPUs' worth of time, None if unlimited)."""
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return dict(nproc=os.cpu_count() or 1, affinity=affinity, cgroup_quota_cpus=quota)
+
+
+# One GPU's share of the GPU box's CPUs: the CPU baseline uses at most this many
+# threads, whatever nproc shows.
+CPU_SHARE = 16
+
+
+def baseline_threads(hc):
+    n = min(hc["affinity"], CPU_SHARE)
+    if hc["cgroup_quota_cpus"]:
+        n = min(n, max(1, int(hc["cgroup_quota_cpus"])))
+    return max(1, n)
+
+
+_ORACLE_BUILD = "x86-64-v2 (prebuilt)"
+
+
+def use_native_oracle():
+    """Rebuild the oracle with -march=native on this host for the CPU-baseline
+    legs (its results are the same; only its speed changes)."""
+    global _ORACLE_BUILD
+    import tempfile
+    from oracle import oracle as O
+    path = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    t0 = time.time()
+    if O.build_native(path):
+        O.use_library(path)
+        _ORACLE_BUILD = "-O3 -march=native (built on this host in %.1f s)" % (time.time() - t0)
+    log("oracle for the CPU baseline: " + _ORACLE_BUILD)
+
+
+def cpu_record(threads):
+    hc = host_cpus()
+    return dict(cores=threads, cpu_model=cpu_model(), nproc=hc["nproc"], affinity=hc["affinity"],
+                cgroup_quota_cpus=hc["cgroup_quota_cpus"], oracle_build=_ORACLE_BUILD,
+                threads_rule="min(CPUs this process may use, %d = one GPU's CPU share, cgroup quota)" % CPU_SHARE)
+
+
 def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=5, single_sample=1 << 20):
     """The oracle's Bvh::intersect / Bvh::visibility restatement on `sample`
     rays and segments of the same batches, median of `repeats` passes after a
@@ -96,8 +149,8 @@ def cpu_baseline(spec, rays8, segs8, sample, threads, repeats=5, single_sample=1
     o.intersect(r6[:m], threads=1)
     o.visibility(s6[:m], threads=1)
     t1 = 2 * m / (time.perf_counter() - t0) / 1e9
-    return dict(value=2 * sample / (c + sh) / 1e9, unit="Grays/s", cores=threads, kind="port",
-                cpu_model=cpu_model(), single_thread_value=round(t1, 6),
+    return dict(value=2 * sample / (c + sh) / 1e9, unit="Grays/s", kind="port", **cpu_record(threads),
+                single_thread_value=round(t1, 6),
                 single_thread_sample="%d rays + %d segments on 1 thread" % (m, m),
                 sample="%d closest-hit rays + %d shadow segments of the timed batches "
                        "(oracle Bvh::intersect / Bvh::visibility restatement, %d threads; median of %d passes: "
@@ -155,8 +208,8 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
                          sample_offset=1 + it * cpu_spp, threads=threads)
                 times.append(time.perf_counter() - t0)
             dt = float(np.median(times))
-            rec["cpu_baseline"] = {"mpaths_s": round(W * H * cpu_spp / dt / 1e6, 4), "cores": threads,
-                                   "kind": "port", "cpu_model": cpu_model(),
+            rec["cpu_baseline"] = {"mpaths_s": round(W * H * cpu_spp / dt / 1e6, 4), "kind": "port",
+                                   **cpu_record(threads),
                                    "sample": "median of 5 timed passes of %d iteration(s) of the same frame "
                                              "after a warm-up" % cpu_spp}
             rec["gpu_over_cpu"] = round(rec["mpaths_s"] / rec["cpu_baseline"]["mpaths_s"], 1)
@@ -199,6 +252,15 @@ def bench_batch(ctx, threads, pool=1 << 23, steps=16):
     for f in films:
         f.clear()
     ctx.synchronize()
+    t = time.perf_counter()
+    lat_ff = render_jobs(ctx, jl, "pt_nee", path_pool=pool, steps_per_poll=steps, use_graph=True, tuning=tuning,
+                         job_order="fewest_first")
+    wall_ff = time.perf_counter() - t
+    log("batch, fewest samples first: %.3f s, p50 %.0f ms p99 %.0f ms" % (
+        wall_ff, np.percentile(lat_ff, 50), np.percentile(lat_ff, 99)))
+    for f in films:
+        f.clear()
+    ctx.synchronize()
     seq_lat = []
     t = time.perf_counter()
     for job in jl:
@@ -213,10 +275,22 @@ def bench_batch(ctx, threads, pool=1 << 23, steps=16):
             "samples": total, "mpaths_s": round(total / wall / 1e6, 2), "wall_s": round(wall, 4),
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "job_order": "as given",
+            "fewest_first": {"mpaths_s": round(total / wall_ff / 1e6, 2), "wall_s": round(wall_ff, 4),
+                             "latency_ms_p50": round(float(np.percentile(lat_ff, 50)), 2),
+                             "latency_ms_p99": round(float(np.percentile(lat_ff, 99)), 2)},
+            "sample_ring": min(16 * pool, total), "sample_ring_bytes": min(16 * pool, total) * 24,
             "path_pool": pool, "graph": True, "tuning": tuning, "max_depth": 8, "integrator": "pt_nee",
             "sequential": {"mpaths_s": round(total / seq / 1e6, 2), "wall_s": round(seq, 4),
                            "latency_ms_p50": round(float(np.percentile(seq_lat, 50)), 2),
                            "latency_ms_p99": round(float(np.percentile(seq_lat, 99)), 2)}}
+
+
+def _rmse(img, ref):
+    """Root mean square error over the pixels' channels (non-finite pixels of
+    either image excluded)."""
+    d = ((img - ref) ** 2).ravel()
+    return float(np.sqrt(d[np.isfinite(d)].mean()))
 
 
 def _rel_mse_trimmed(img, ref, trim=1e-3):
@@ -283,7 +357,9 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
     trace(st, plain, "pt_nee", spp=1, max_depth=8, seed=5)
     plain.clear()
     t_plain = timed(lambda: trace(st, plain, "pt_nee", spp=spp, max_depth=8, seed=7))
-    e_plain = _rel_mse_trimmed(image(plain), R)
+    img = image(plain)
+    e_plain = _rmse(img, R)
+    r_plain = _rel_mse_trimmed(img, R)
     plain.close()
     # the guide samples the first vertex only (the scene's error comes from long paths, which a guide
     # at later vertices does not reduce); batches of 8 spp like the plain render, 8 Adam steps each
@@ -292,24 +368,36 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
     g = Guide(ctx, **cfg)
     infer = guide_inference_rate(ctx, g)
     f = Film(ctx, W, H)
-    tot, curve, reached = 0.0, [], None
+    tot, curve = 0.0, []
     for k in range(0, 2 * spp, 8):
         tot += timed(lambda: trace(st, f, "guided", spp=8, max_depth=8, seed=9, sample_offset=k, batch=8, guide=g))
-        e = _rel_mse_trimmed(image(f), R)
-        curve.append((k + 8, round(tot, 4), round(e, 6)))
-        if reached is None and e <= e_plain:
-            t0, e0 = (curve[-2][1], curve[-2][2]) if len(curve) > 1 else (0.0, float("inf"))
-            reached = t0 + (tot - t0) * ((e0 - e_plain) / (e0 - e) if np.isfinite(e0) and e0 > e else 1.0)
-    log("guided: plain %d spp %.3f s relMSE %.5f; guided reaches it at %s s" % (
-        spp, t_plain, e_plain, "%.3f" % reached if reached else "-- (not within %d spp)" % (2 * spp)))
+        img = image(f)
+        curve.append((k + 8, round(tot, 4), round(_rmse(img, R), 6), round(_rel_mse_trimmed(img, R), 6)))
+
+    def reach(col, target):
+        # wall time at which the guided curve first reaches `target` (linear between batches)
+        prev = (0.0, float("inf"))
+        for row in curve:
+            if row[col] <= target:
+                t0, e0 = prev
+                return t0 + (row[1] - t0) * ((e0 - target) / (e0 - row[col]) if np.isfinite(e0) and e0 > row[col] else 1.0)
+            prev = (row[1], row[col])
+        return None
+    reached, reached_rel = reach(2, e_plain), reach(3, r_plain)
+    log("guided: plain %d spp %.3f s RMSE %.5f relMSE %.5f; guided reaches the RMSE at %s s, the relMSE at %s s" % (
+        spp, t_plain, e_plain, r_plain, "%.3f" % reached if reached else "--", "%.3f" % reached_rel if reached_rel else "--"))
     f.close()
     g.close()
     return {"scene": "showcase 1920x1080, PT+NEE depth 8", "reference_spp": ref_spp,
-            "reference_s": round(t_ref, 3), "error": "relMSE, 0.1% largest pixel errors dropped",
-            "plain_spp": spp, "plain_s": round(t_plain, 4), "plain_error": round(e_plain, 6),
-            "guided_config": cfg, "guided_curve": curve, "inference": infer,
-            "time_to_equal_error_s": round(reached, 4) if reached else None,
-            "time_ratio": round(reached / t_plain, 3) if reached else None}
+            "reference_s": round(t_ref, 3),
+            "error": "RMSE over pixels and channels (the time ratio); relMSE with the 0.1% largest pixel errors dropped beside it",
+            "plain_spp": spp, "plain_s": round(t_plain, 4), "plain_rmse": round(e_plain, 6),
+            "plain_relmse_trimmed": round(r_plain, 6),
+            "guided_config": cfg, "guided_curve": curve, "guided_curve_columns": ["spp", "s", "rmse", "relmse_trimmed"],
+            "inference": infer,
+            "time_to_equal_rmse_s": round(reached, 4) if reached else None,
+            "time_ratio": round(reached / t_plain, 3) if reached else None,
+            "time_ratio_relmse_trimmed": round(reached_rel / t_plain, 3) if reached_rel else None}
 
 
 def main():
@@ -346,6 +434,9 @@ def main():
               "by pixel tiles or iteration ranges, ray batches by ranges). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if not args.no_cpu_baseline and not args.guide_inference_only and not args.traffic_probe:
+        use_native_oracle()
 
     import torch
     from gammaray_amd import scenes
@@ -480,7 +571,7 @@ def main():
                     closest_call_ms=round(ms_closest, 4), shadow_call_ms=round(ms_shadow, 4))
 
     cpu = None
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = baseline_threads(host_cpus())
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, rays8, segs8, min(args.cpu_sample, n), threads)
         log("cpu baseline %.4f Grays/s on %d threads" % (cpu["value"], threads))

@@ -27,6 +27,27 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_native(out_path):
+    """Compile the oracle for this host's CPU (-march=native) into out_path,
+    for a CPU baseline timed on the machine it runs on.  Returns True on
+    success (False when no C compiler is available)."""
+    try:
+        r = subprocess.run(["make", "-s", "-B", "-C", _HERE, "ARCH=-march=native", "OUT=" + out_path],
+                           capture_output=True, text=True, timeout=300)
+    except (OSError, subprocess.TimeoutExpired):
+        return False
+    return r.returncode == 0 and os.path.exists(out_path)
+
+
+def use_library(path):
+    """Load the oracle from `path` instead of the prebuilt liboracle.so; must
+    be called before the oracle is first used."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:
