@@ -341,7 +341,7 @@ int gr_scene_commit(gr_scene* s) {
     if (e == hipSuccess) e = upload(&s->d_comp_ids, lay.comp_ids, ctx->stream);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_overflow), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemsetAsync(s->d_overflow, 0, sizeof(unsigned long long), ctx->stream);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_next_ray), gr::kRayCounters * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_next_ray), gr::kRayCounterBytes);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         free_device(s);
@@ -428,7 +428,7 @@ int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_v
 
 static bool valid_tuning(const gr_tuning& t) {
     return t.sort_bits <= 30 && (t.lds_stack == 8 || t.lds_stack == 16 || t.lds_stack == 32) && t.refill >= 1 &&
-           t.refill <= 64 && t.waves_per_cu <= 64 && t.xcd_ranges <= 1;
+           t.refill <= 64 && t.waves_per_cu <= 64 && t.xcd_ranges <= 1 && t.ray_chunk <= 65536;
 }
 
 int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
@@ -439,6 +439,7 @@ int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
     s->tuning.waves_per_cu = t->waves_per_cu;
     s->tuning.refill = t->refill;
     s->tuning.xcd_ranges = t->xcd_ranges;
+    s->tuning.ray_chunk = t->ray_chunk ? t->ray_chunk : gr::kDefaultRayChunk;
     return GR_OK;
 }
 
@@ -449,6 +450,7 @@ int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
     out->waves_per_cu = s->tuning.waves_per_cu;
     out->refill = s->tuning.refill;
     out->xcd_ranges = s->tuning.xcd_ranges;
+    out->ray_chunk = s->tuning.ray_chunk;
     return GR_OK;
 }
 
@@ -488,7 +490,8 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 56
+    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 48,
+                            // 128-ray chunks
     float best_ms = 0.0f;
     auto timed = [&](const gr::TraceTuning& tu, float* ms) {
         return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
@@ -525,10 +528,15 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
         c.waves_per_cu = (uint32_t)std::max(1, occ * frac / 4);
         if ((int)c.waves_per_cu < occ) consider(c);
     }
-    for (uint32_t rf : {16u, 32u, 40u, 48u, 60u}) {
+    for (uint32_t rf : {32u, 40u, 56u, 64u}) {
         gr::TraceTuning c = best;
         c.refill = rf;
         if (rf != best.refill) consider(c);
+    }
+    for (uint32_t ch : {64u, 128u, 512u, 1024u}) {
+        gr::TraceTuning c = best;
+        c.ray_chunk = ch;
+        if (ch != best.ray_chunk) consider(c);
     }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);

@@ -135,15 +135,20 @@ hipError_t sort_rays_multi(const SceneRef* scenes, uint32_t nscenes, const float
 
 // ---- launchers (gr_traverse.hip) -------------------------------------------
 // Settings of the traversal launch (gr_tuning).
+constexpr uint32_t kDefaultRayChunk = 128;
 struct TraceTuning {
     uint32_t sort_bits = 0;
     uint32_t lds_stack = 8;  // 8 waves per SIMD (deeper LDS stacks cap occupancy lower)
     uint32_t waves_per_cu = 0;
-    uint32_t refill = 56;
+    uint32_t refill = 48;
     uint32_t xcd_ranges = 0;
+    uint32_t ray_chunk = kDefaultRayChunk;  // rays a wave reserves per atomic on the work counter
 };
-// Ray counters of the persistent traversal: one per XCD range.
+// Ray counters of the persistent traversal: one per XCD range, each on its own
+// 128-byte line (atomics on one line serialise in one L2 channel).
 constexpr uint32_t kRayCounters = 8;
+constexpr uint32_t kCounterStride = 32;  // uint32 words between counters
+constexpr size_t kRayCounterBytes = (size_t)kRayCounters * kCounterStride * sizeof(uint32_t);
 
 // Device-side error counters of a context (gr_stats, in this order).
 enum StatIndex : int { kStatNonfiniteBsdf = 0, kStatNonfiniteRadiance = 1, kStatBadSplats = 2, kStats = 4 };

@@ -26,7 +26,7 @@ namespace {
 
 // Scratch layout: the traversal's work counters, then the sort scratch for
 // GR_NOALLOC_MAX_RAYS keys (4 key/value arrays, digit histograms, totals).
-constexpr size_t kCounterBytes = 256;
+constexpr size_t kCounterBytes = 1024;
 constexpr size_t kSortBytes = 16u * GR_NOALLOC_MAX_RAYS + 256u * 4u * (GR_NOALLOC_MAX_RAYS / 4096u) + 1024u;
 constexpr size_t kScratchBytes = kCounterBytes + kSortBytes;
 
@@ -86,4 +86,4 @@ extern "C" int gr_intersect_closest_noalloc(gr_scene* s, const gr_ray* d_rays, u
     return GR_OK;
 }
 
-static_assert(gr::kRayCounters * sizeof(uint32_t) <= kCounterBytes, "counter region");
+static_assert(gr::kRayCounterBytes <= kCounterBytes, "counter region");

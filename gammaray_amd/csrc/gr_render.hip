@@ -2363,6 +2363,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         tu.waves_per_cu = prm.tuning.waves_per_cu;
         tu.refill = prm.tuning.refill;
         tu.xcd_ranges = prm.tuning.xcd_ranges;
+        tu.ray_chunk = prm.tuning.ray_chunk ? prm.tuning.ray_chunk : kDefaultRayChunk;
     }
 
     // device memory: job and scene tables, the sample ring, path state, counters

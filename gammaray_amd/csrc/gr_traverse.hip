@@ -131,9 +131,9 @@ __device__ __forceinline__ GpuNode load_node(const GpuNode* nodes, uint32_t i) {
 
 // ---------------------------------------------------------------------------
 // Persistent traversal.  One 64-lane wave per block; every lane owns one ray
-// at a time.  Lanes whose ray finishes go idle and the wave refills them from
-// a global ray counter once `refill` are idle, so long and short rays share a
-// wave without leaving lanes empty.  Rays are taken in the order of `perm`
+// at a time.  Lanes whose ray finishes go idle and the wave refills them, once
+// `refill` are idle, from ray positions it reserved in chunks on a global work
+// counter, so long and short rays share a wave without leaving lanes empty.  Rays are taken in the order of `perm`
 // (a coherence sort) when given; results always go to the ray's own index.
 // The loop is while-while: all lanes step through interior nodes until each is
 // at a leaf (or done), then leaves are tested together.  The traversal stack
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                                                   typename Output<MODE>::type* __restrict__ out,
                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ next_ray,
                                                   const uint32_t* __restrict__ n_ptr, uint32_t refill,
-                                                  uint32_t parts) {
+                                                  uint32_t parts, uint32_t chunk) {
     constexpr int kSpill = kStackSize - LDS;
     // stack entries: (node ref, entry distance bits), one 8-B word per entry
     __shared__ uint2 s_e[LDS][kWave];
@@ -175,6 +175,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     if (n_ptr) {
         const uint32_t m = __builtin_amdgcn_readfirstlane(*n_ptr);
         n = m < n ? m : n;
+    }
+    // A chunk no larger than a quarter of each wave's even share, so that a
+    // short batch still spreads over every wave of the grid.
+    {
+        const uint32_t share = n / (gridDim.x * 4u);
+        const uint32_t c = share > kWave ? share : (uint32_t)kWave;
+        chunk = c < chunk ? c : chunk;
     }
 
     // With parts > 1 the batch is cut into `parts` contiguous ranges, one per
@@ -251,28 +258,46 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
         return false;
     };
 
+    // Wave-local pool of reserved ray positions [pool_lo + pool_k, pool_lo +
+    // pool_end).  A wave reserves `chunk` positions per atomic on its range's
+    // work counter and refills from the pool: one counter serves every wave of
+    // the launch, and atomics on one address serialise in the L2.
+    uint32_t pool_k = 0, pool_end = 0, pool_lo = 0;
+
     for (;;) {
         // ---- refill idle lanes (wave-uniform control flow)
-        if (!exhausted) {
+        if (!exhausted || pool_k < pool_end) {
             const uint64_t idle = __ballot(!active);
-            const int nidle = __popcll(idle);
-            if ((uint32_t)nidle >= refill) {
-                const uint32_t lo = r_lo, len = r_len;
-                uint32_t first = 0;
-                if (lane == 0) first = atomicAdd(next_ray + part, (uint32_t)nidle);
-                first = __shfl(first, 0);
-                if (first + (uint32_t)nidle >= len) {
-                    // this range is drained (counters only grow): go to the next
-                    if (++drained == parts) exhausted = true;
-                    part = part + 1 == parts ? 0u : part + 1;
-                    set_range();
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= refill) {
+                const uint32_t avail = pool_end - pool_k;
+                uint32_t nfirst = 0, nsize = 0, nlo = 0;
+                if (avail < nidle && !exhausted) {
+                    const uint32_t lo = r_lo, len = r_len;
+                    uint32_t first = 0;
+                    if (lane == 0) first = atomicAdd(next_ray + part * kCounterStride, chunk);
+                    first = __builtin_amdgcn_readfirstlane(first);
+                    if (first < len) {
+                        nfirst = first;
+                        nsize = len - first < chunk ? len - first : chunk;
+                        nlo = lo;
+                    }
+                    if ((uint64_t)first + chunk >= len) {
+                        // this range is drained (counters only grow): go to the next
+                        if (++drained == parts) exhausted = true;
+                        part = part + 1 == parts ? 0u : part + 1;
+                        set_range();
+                    }
                 }
                 if (!active) {
-                    const uint32_t k = first + (uint32_t)__popcll(idle & lanes_below());
-                    const uint32_t idx = lo + k;
+                    // idle lanes take the pool's positions first, then the new chunk's
+                    const uint32_t rank = (uint32_t)__popcll(idle & lanes_below());
+                    uint32_t idx = n;
+                    if (rank < avail) idx = pool_lo + pool_k + rank;
+                    else if (rank - avail < nsize) idx = nlo + nfirst + (rank - avail);
                     // A permutation entry outside [0, n) is skipped (never
                     // dereferenced), so a faulty sort shows as missing results.
-                    const uint32_t pick = k < len ? (perm ? perm[idx] : idx) : n;
+                    const uint32_t pick = idx < n ? (perm ? perm[idx] : idx) : n;
                     if (pick < n) {
                         ray = pick;
                         const float4 a = in[2 * (size_t)ray], b = in[2 * (size_t)ray + 1];
@@ -325,10 +350,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                         }
                     }
                 }
+                if (nidle <= avail) {
+                    pool_k += nidle;
+                } else if (nsize) {
+                    const uint32_t used = nidle - avail < nsize ? nidle - avail : nsize;
+                    pool_lo = nlo;
+                    pool_k = nfirst + used;
+                    pool_end = nfirst + nsize;
+                } else {
+                    pool_k = pool_end;
+                }
             }
         }
         if (__ballot(active) == 0) {
-            if (exhausted) break;
+            if (exhausted && pool_k >= pool_end) break;
             continue;
         }
 
@@ -483,7 +518,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
 }
 
 __global__ void k_reset_counter(uint32_t* c) {
-    for (uint32_t i = 0; i < kRayCounters; ++i) c[i] = 0u;
+    for (uint32_t i = 0; i < kRayCounters; ++i) c[i * kCounterStride] = 0u;
 }
 
 template <int MODE, bool COUNT, int LDS, bool MULTI>
@@ -521,13 +556,14 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     const uint32_t blocks = needed < grid ? needed : grid;
     const uint32_t refill = ts.tu.refill < 1 ? 1u : (ts.tu.refill > 64 ? 64u : ts.tu.refill);
     const uint32_t parts = ts.tu.xcd_ranges ? kRayCounters : 1u;
+    const uint32_t chunk = ts.tu.ray_chunk < 1 ? 1u : ts.tu.ray_chunk;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);
     hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
     hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS, MULTI>), dim3(blocks), dim3(kWave), 0, s, ts.sv, ts.scenes,
                        ts.nscenes,
                        reinterpret_cast<const float4*>(in), n, perm,
                        reinterpret_cast<typename Output<MODE>::type*>(out), counts, ts.sv.next_ray, n_ptr, refill,
-                       parts);
+                       parts, chunk);
     if (t0) timing_end(ts.timing, MODE == kClosest ? GR_KERNEL_TRACE_CLOSEST : GR_KERNEL_TRACE_SHADOW, t0, s);
     return hipGetLastError();
 }

@@ -519,14 +519,17 @@ def test_tuning_validation_and_rounding(host_ctx):
     from gammaray_amd.api import Stage
     from gammaray_amd._lib import GrError
     st = Stage(scenes.box_scene(), ctx=host_ctx)
-    assert st.get_tuning() == dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=56, xcd_ranges=0)
+    assert st.get_tuning() == dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=48, xcd_ranges=0,
+                                   ray_chunk=128)
     for want, got in [(0, 0), (5, 0), (6, 6), (8, 6), (20, 18), (30, 30)]:
         assert st.set_tuning(sort_bits=want)["sort_bits"] == got
     for bad in [dict(sort_bits=31), dict(lds_stack=12), dict(refill=0), dict(refill=65), dict(waves_per_cu=65),
-                dict(xcd_ranges=2)]:
+                dict(xcd_ranges=2), dict(ray_chunk=65537)]:
         with pytest.raises(GrError):
             st.set_tuning(**bad)
     assert st.get_tuning()["sort_bits"] == 30
+    assert st.set_tuning(ray_chunk=1)["ray_chunk"] == 1
+    assert st.set_tuning(ray_chunk=65536)["ray_chunk"] == 65536
     with pytest.raises(GrError):
         import torch
         st.autotune(torch.zeros((4, 8)))
