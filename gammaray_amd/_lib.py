@@ -45,7 +45,7 @@ class Camera(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("sort_bits", _u32), ("lds_stack", _u32), ("waves_per_cu", _u32), ("refill", _u32),
-                ("xcd_ranges", _u32), ("ray_chunk", _u32)]
+                ("xcd_ranges", _u32), ("ray_chunk", _u32), ("vis_refill", _u32)]
 
 
 class TraceParams(ctypes.Structure):

@@ -428,7 +428,8 @@ int gr_occluded(gr_scene* s, const gr_segment* d_segs, uint32_t n, uint32_t* d_v
 
 static bool valid_tuning(const gr_tuning& t) {
     return t.sort_bits <= 30 && (t.lds_stack == 8 || t.lds_stack == 16 || t.lds_stack == 32) && t.refill >= 1 &&
-           t.refill <= 64 && t.waves_per_cu <= 64 && t.xcd_ranges <= 1 && t.ray_chunk <= 65536;
+           t.refill <= 64 && t.waves_per_cu <= 64 && t.xcd_ranges <= 1 && t.ray_chunk <= 65536 &&
+           t.vis_refill <= 64;
 }
 
 int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
@@ -440,6 +441,7 @@ int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
     s->tuning.refill = t->refill;
     s->tuning.xcd_ranges = t->xcd_ranges;
     s->tuning.ray_chunk = t->ray_chunk ? t->ray_chunk : gr::kDefaultRayChunk;
+    s->tuning.vis_refill = t->vis_refill ? t->vis_refill : t->refill;
     return GR_OK;
 }
 
@@ -451,6 +453,7 @@ int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
     out->refill = s->tuning.refill;
     out->xcd_ranges = s->tuning.xcd_ranges;
     out->ray_chunk = s->tuning.ray_chunk;
+    out->vis_refill = s->tuning.vis_refill;
     return GR_OK;
 }
 
@@ -490,8 +493,8 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 48,
-                            // 128-ray chunks
+    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 32 (40 for
+                            // visibility), 128-ray chunks
     float best_ms = 0.0f;
     auto timed = [&](const gr::TraceTuning& tu, float* ms) {
         return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
@@ -528,10 +531,17 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
         c.waves_per_cu = (uint32_t)std::max(1, occ * frac / 4);
         if ((int)c.waves_per_cu < occ) consider(c);
     }
-    for (uint32_t rf : {32u, 40u, 56u, 64u}) {
+    for (uint32_t rf : {24u, 40u, 48u, 56u}) {
         gr::TraceTuning c = best;
         c.refill = rf;
         if (rf != best.refill) consider(c);
+    }
+    if (m) {
+        for (uint32_t rf : {24u, 32u, 48u, 56u}) {
+            gr::TraceTuning c = best;
+            c.vis_refill = rf;
+            if (rf != best.vis_refill) consider(c);
+        }
     }
     for (uint32_t ch : {64u, 128u, 512u, 1024u}) {
         gr::TraceTuning c = best;

@@ -140,7 +140,8 @@ struct TraceTuning {
     uint32_t sort_bits = 0;
     uint32_t lds_stack = 8;  // 8 waves per SIMD (deeper LDS stacks cap occupancy lower)
     uint32_t waves_per_cu = 0;
-    uint32_t refill = 48;
+    uint32_t refill = 32;      // closest-hit kernels
+    uint32_t vis_refill = 40;  // visibility kernels (their segments end at the first occluder)
     uint32_t xcd_ranges = 0;
     uint32_t ray_chunk = kDefaultRayChunk;  // rays a wave reserves per atomic on the work counter
 };

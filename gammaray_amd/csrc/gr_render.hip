@@ -2362,6 +2362,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         tu.lds_stack = prm.tuning.lds_stack;
         tu.waves_per_cu = prm.tuning.waves_per_cu;
         tu.refill = prm.tuning.refill;
+        tu.vis_refill = prm.tuning.vis_refill ? prm.tuning.vis_refill : prm.tuning.refill;
         tu.xcd_ranges = prm.tuning.xcd_ranges;
         tu.ray_chunk = prm.tuning.ray_chunk ? prm.tuning.ray_chunk : kDefaultRayChunk;
     }

@@ -107,38 +107,17 @@ __device__ __forceinline__ float hit_sphere(const GpuComp& g, const RayFrame& r)
     return 0.0f;
 }
 
-__device__ __forceinline__ GpuComp load_comp(const GpuComp* comps, uint32_t slot) {
-    const float4* p = reinterpret_cast<const float4*>(comps + slot);
-    GpuComp g;
-    g.q0 = p[0];
-    g.q1 = p[1];
-    g.q2 = p[2];
-    return g;
-}
-
-__device__ __forceinline__ GpuNode load_node(const GpuNode* nodes, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    GpuNode n;
-    n.q0 = p[0];
-    n.q1 = p[1];
-    n.q2 = p[2];
-    const float4 w = p[3];
-    n.q3 = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z),
-                      __float_as_uint(w.w));
-    return n;
-}
-
-
 // ---------------------------------------------------------------------------
 // Persistent traversal.  One 64-lane wave per block; every lane owns one ray
 // at a time.  Lanes whose ray finishes go idle and the wave refills them, once
 // `refill` are idle, from ray positions it reserved in chunks on a global work
-// counter, so long and short rays share a wave without leaving lanes empty.  Rays are taken in the order of `perm`
-// (a coherence sort) when given; results always go to the ray's own index.
-// The loop is while-while: all lanes step through interior nodes until each is
-// at a leaf (or done), then leaves are tested together.  The traversal stack
-// keeps its top LDS entries per lane in LDS and spills older entries to
-// private memory, up to the reference's 64 entries.
+// counter, so long and short rays share a wave without leaving lanes empty.
+// Rays are taken in the order of `perm` (a coherence sort) when given; results
+// always go to the ray's own index.  Each pass of the loop advances every
+// active lane by one step, an interior node or one primitive of a leaf,
+// through one shared fetch.  The traversal stack keeps its top LDS entries per
+// lane in LDS and spills older entries to private memory, up to the
+// reference's 64 entries.
 constexpr int kWave = 64;
 
 template <int MODE>
@@ -367,150 +346,89 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
             continue;
         }
 
-        if constexpr (MODE == kClosest) {
-            // ---- one step for every active lane: an interior node, or one
-            // primitive of its leaf.  Both fetch through the same four loads (a
-            // primitive record is 48 bytes; its fourth load repeats the first), so
-            // a wave whose lanes are at different kinds of nodes still waits on one
-            // round trip per step.
-            if (active) {
-                const bool at_leaf = (cur & kLeafBit) != 0;
-                const uint32_t slot = cur & ~kLeafBit;
-                const float4* p = at_leaf ? reinterpret_cast<const float4*>(comps + slot)
-                                          : reinterpret_cast<const float4*>(nodes + cur);
-                const float4 q0 = p[0], q1 = p[1], q2 = p[2];
-                const float4 q3 = *(at_leaf ? p : p + 3);
-                if (!at_leaf) {
-                    if (COUNT) c_box += 2;
-                    float t0 = 0.0f, t1 = 0.0f;
-                    const bool s0 = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
-                    const bool s1 = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
-                    const uint32_t axis = __float_as_uint(q3.z);
-                    const bool flip = ((r.sign >> axis) & 1u) != 0u;
-                    const uint32_t near_ref = __float_as_uint(flip ? q3.y : q3.x);
-                    const uint32_t far_ref = __float_as_uint(flip ? q3.x : q3.y);
-                    const bool near_ok = flip ? s1 : s0;
-                    const bool far_ok = flip ? s0 : s1;
-                    const float near_t = flip ? t1 : t0;
-                    const float far_t = flip ? t0 : t1;
-                    // A far child whose entry distance is already >= closest would fail
-                    // the deferred test at its pop (closest only decreases): not pushed.
-                    const bool push = far_ok && far_t < closest;
-                    if (push && sp == kStackSize) {
-                        finish(false, true);
-                    } else {
-                        if (push) {
-                            if (sp - base == LDS) {
-                                spill[base] = s_e[base & (LDS - 1)][lane];
-                                ++base;
-                            }
-                            s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(far_t));
-                            ++sp;
-                        }
-                        if (near_ok && near_t < closest) {
-                            cur = near_ref;
-                        } else if (!pop()) {
-                            finish(false, false);
-                        }
-                    }
-                } else {
-                    GpuComp g;
-                    g.q0 = q0;
-                    g.q1 = q1;
-                    g.q2 = q2;
-                    const uint32_t tag = __float_as_uint(g.q2.y);
-                    float dist;
-                    if (tag & kTagSphere) {
-                        if (COUNT) c_sph += 1;
-                        dist = hit_sphere(g, r);
-                    } else {
-                        if (COUNT) c_tri += 1;
-                        dist = hit_triangle(g, r);
-                    }
-                    bool done = false;
-                    if (dist != 0.0f && dist < closest) {
-                        closest = dist;
-                        best = slot;
-                        done = MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit);
-                    }
-                    if (done) finish(true, false);
-                    else if (!(tag & kTagLast)) cur = kLeafBit | (slot + 1);
-                    else if (!pop()) finish(false, false);
-                }
-            }
-        } else {
-            // Visibility segments keep the while-while loop (measured faster for them).
-            // ---- interior nodes until every lane sits on a leaf or is done
-            while (active && !(cur & kLeafBit)) {
-                const GpuNode nd = load_node(nodes, cur);
-                // The reference tests the near child's box next; the far child's
-                // box is tested when it is popped (counted in pop()).
-                if (COUNT) c_box += 1;
+        // ---- one step for every active lane: an interior node, or one
+        // primitive of its leaf.  Both fetch through the same four loads (a
+        // primitive record is 48 bytes; its fourth load repeats the first), so
+        // a wave whose lanes are at different kinds of nodes still waits on one
+        // round trip per step.
+        if (active) {
+            const bool at_leaf = (cur & kLeafBit) != 0;
+            const uint32_t slot = cur & ~kLeafBit;
+            const float4* p = at_leaf ? reinterpret_cast<const float4*>(comps + slot)
+                                      : reinterpret_cast<const float4*>(nodes + cur);
+            const float4 q0 = p[0], q1 = p[1], q2 = p[2];
+            const float4 q3 = *(at_leaf ? p : p + 3);
+            // All four loads complete here, before the interior/leaf branch:
+            // left to itself the compiler sinks the loads only the interior
+            // path reads into that path, which costs a wave holding both
+            // kinds of lanes a second round trip after the leaf path's wait.
+            asm volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w), "v"(q1.x), "v"(q1.y), "v"(q1.z),
+                         "v"(q1.w), "v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(q2.w), "v"(q3.x), "v"(q3.y),
+                         "v"(q3.z), "v"(q3.w));
+            if (!at_leaf) {
+                // Closest hit counts both child boxes here; visibility counts the
+                // near child's here and the far child's at its pop (pop()).
+                if (COUNT) c_box += MODE == kClosest ? 2 : 1;
                 float t0 = 0.0f, t1 = 0.0f;
-                const bool s0 = slab(nd.q0.x, nd.q0.y, nd.q0.z, nd.q0.w, nd.q1.x, nd.q1.y, r, sv.pad, &t0);
-                const bool s1 = slab(nd.q1.z, nd.q1.w, nd.q2.x, nd.q2.y, nd.q2.z, nd.q2.w, r, sv.pad, &t1);
-                const uint32_t axis = nd.q3.z;
+                const bool s0 = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
+                const bool s1 = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
+                const uint32_t axis = __float_as_uint(q3.z);
                 const bool flip = ((r.sign >> axis) & 1u) != 0u;
-                const uint32_t near_ref = flip ? nd.q3.y : nd.q3.x;
-                const uint32_t far_ref = flip ? nd.q3.x : nd.q3.y;
+                const uint32_t near_ref = __float_as_uint(flip ? q3.y : q3.x);
+                const uint32_t far_ref = __float_as_uint(flip ? q3.x : q3.y);
                 const bool near_ok = flip ? s1 : s0;
                 const bool far_ok = flip ? s0 : s1;
                 const float near_t = flip ? t1 : t0;
                 const float far_t = flip ? t0 : t1;
                 // A far child whose entry distance is already >= closest would fail
                 // the deferred test at its pop (closest only decreases): not pushed.
-                // The counting variant pushes every far child, as the reference
-                // does, so that a traversal ending at its first occluder counts
-                // exactly the boxes the reference tests before it stops (those
-                // still on the stack then are never tested); a child whose slab
-                // test failed carries an infinite entry distance, failing at pop.
-                if (COUNT || (far_ok && far_t < closest)) {
-                    if (sp == kStackSize) {
-                        finish(false, true);
-                        break;
-                    }
-                    if (sp - base == LDS) {
-                        spill[base] = s_e[base & (LDS - 1)][lane];
-                        ++base;
-                    }
-                    const float push_t = far_ok ? far_t : __builtin_inff();
-                    s_e[sp & (LDS - 1)][lane] = make_uint2(far_ref, __float_as_uint(push_t));
-                    ++sp;
-                }
-                if (near_ok && near_t < closest) {
-                    cur = near_ref;
-                } else if (!pop()) {
-                    finish(false, false);
-                }
-            }
-
-            // ---- leaves
-            if (active) {
-                uint32_t slot = cur & ~kLeafBit;
-                bool done = false;
-                for (;;) {
-                    const GpuComp g = load_comp(comps, slot);
-                    const uint32_t tag = __float_as_uint(g.q2.y);
-                    float dist;
-                    if (tag & kTagSphere) {
-                        if (COUNT) c_sph += 1;
-                        dist = hit_sphere(g, r);
-                    } else {
-                        if (COUNT) c_tri += 1;
-                        dist = hit_triangle(g, r);
-                    }
-                    if (dist != 0.0f && dist < closest) {
-                        closest = dist;
-                        best = slot;
-                        if (MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit)) {
-                            done = true;
-                            break;
+                // The visibility kernels' counting variant pushes every far child,
+                // as the reference does, so that a traversal ending at its first
+                // occluder counts exactly the boxes the reference tests before it
+                // stops (those still on the stack then are never tested); a child
+                // whose slab test failed carries an infinite entry distance and
+                // fails at its pop.
+                const bool push = (COUNT && MODE != kClosest) || (far_ok && far_t < closest);
+                if (push && sp == kStackSize) {
+                    finish(false, true);
+                } else {
+                    if (push) {
+                        if (sp - base == LDS) {
+                            spill[base] = s_e[base & (LDS - 1)][lane];
+                            ++base;
                         }
+                        s_e[sp & (LDS - 1)][lane] =
+                            make_uint2(far_ref, __float_as_uint(far_ok ? far_t : __builtin_inff()));
+                        ++sp;
                     }
-                    if (tag & kTagLast) break;
-                    ++slot;
+                    if (near_ok && near_t < closest) {
+                        cur = near_ref;
+                    } else if (!pop()) {
+                        finish(false, false);
+                    }
+                }
+            } else {
+                GpuComp g;
+                g.q0 = q0;
+                g.q1 = q1;
+                g.q2 = q2;
+                const uint32_t tag = __float_as_uint(g.q2.y);
+                float dist;
+                if (tag & kTagSphere) {
+                    if (COUNT) c_sph += 1;
+                    dist = hit_sphere(g, r);
+                } else {
+                    if (COUNT) c_tri += 1;
+                    dist = hit_triangle(g, r);
+                }
+                bool done = false;
+                if (dist != 0.0f && dist < closest) {
+                    closest = dist;
+                    best = slot;
+                    done = MODE == kOcclusionAnyHit || (MODE == kOcclusionExact && dist < limit);
                 }
                 if (done) finish(true, false);
+                else if (!(tag & kTagLast)) cur = kLeafBit | (slot + 1);
                 else if (!pop()) finish(false, false);
             }
         }
@@ -554,7 +472,8 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     const uint32_t grid = (uint32_t)(num_cus() * per_cu);
     const uint32_t needed = (n + kWave - 1) / kWave;
     const uint32_t blocks = needed < grid ? needed : grid;
-    const uint32_t refill = ts.tu.refill < 1 ? 1u : (ts.tu.refill > 64 ? 64u : ts.tu.refill);
+    const uint32_t want = MODE == kClosest ? ts.tu.refill : ts.tu.vis_refill;
+    const uint32_t refill = want < 1 ? 1u : (want > 64 ? 64u : want);
     const uint32_t parts = ts.tu.xcd_ranges ? kRayCounters : 1u;
     const uint32_t chunk = ts.tu.ray_chunk < 1 ? 1u : ts.tu.ray_chunk;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);

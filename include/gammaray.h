@@ -181,17 +181,20 @@ typedef struct {
                               octant above the m-bit-per-axis Morton code of the origin */
     uint32_t lds_stack;    /* traversal stack entries per lane kept in LDS: 8 (default), 16 or 32 */
     uint32_t waves_per_cu; /* resident traversal waves per CU; 0 = occupancy limit */
-    uint32_t refill;       /* idle lanes (1..64) at which a wave fetches new rays (default 48) */
+    uint32_t refill;       /* idle lanes (1..64) at which a closest-hit wave fetches new rays
+                              (default 32) */
     uint32_t xcd_ranges;   /* 1: the (sorted) batch is cut into one contiguous range per XCD,
                               each drained by the waves resident on that XCD first */
     uint32_t ray_chunk;    /* rays a wave reserves per atomic on the work counter (1..65536);
                               0 = 128 */
+    uint32_t vis_refill;   /* idle lanes (1..64) at which a visibility wave fetches new
+                              segments (default 40); 0 = refill */
 } gr_tuning;
 int gr_scene_set_tuning(gr_scene *scene, const gr_tuning *tuning);
 int gr_scene_get_tuning(gr_scene *scene, gr_tuning *out);
 /* Times candidate settings on n rays (device memory; closest hit) and keeps the
  * fastest: sort bits (12, 15, 18, 21, 24, 30), XCD ranges, LDS stack depth,
- * waves per CU, refill.
+ * waves per CU, refill, ray chunk, visibility refill.
  * d_hits: scratch for n hits.  The chosen setting is returned in *out. */
 int gr_scene_autotune(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
                       gr_tuning *out);
