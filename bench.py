@@ -28,23 +28,24 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 COPY_PROBE_BYTES = 1 << 29  # 512 MiB, read once and written once
 # HBM bytes per launch of the traversal kernels on the default workload
-# (icosphere level 8, 16,777,216 rays / segments, rays in input order, 8 LDS
-# stack entries, refill at 56 idle lanes), from two `rocprofv3 --pmc` runs of
-# `bench.py --traffic-probe`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the
-# factors being what the counters read for the probe's 512 MiB device copy.
-# Counters cannot be read inside this process, so the figure is reported only
-# when the timed launches run that exact configuration.
+# (icosphere level 8, 16,777,216 rays / segments, the library's default
+# setting: rays in input order, one range per XCD, 8 LDS stack entries, refill
+# at 32 idle lanes (40 for visibility), 128-ray chunks), from two
+# `rocprofv3 --pmc` runs of `bench.py --traffic-probe`: FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (KiB), the factors being what the counters read for the
+# probe's 512 MiB device copy.  Counters cannot be read inside this process, so
+# the figure is reported only when the timed launches run that exact setting.
 MEASURED_TRAFFIC = {
-    "closest": (7898149.1 * 2.0 + 581600.0) * 1024,
-    "shadow": (10830856.9 * 2.0 + 379299.5) * 1024,
+    "closest": (8304211.8 * 2.0 + 554926.9) * 1024,
+    "shadow": (10939109.8 * 2.0 + 656658.3) * 1024,
 }
 # TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) of the same launches (a third counter run)
 MEASURED_L2_HIT = {
-    "closest": 304816695.0 / (304816695.0 + 138665037.0),
-    "shadow": 218569677.0 / (218569677.0 + 179414871.0),
+    "closest": 149404016.0 / (149404016.0 + 145886791.0),
+    "shadow": 234197122.0 / (234197122.0 + 183486669.0),
 }
-MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=56, waves_per_cu=0,
-                                xcd_ranges=0, ray_chunk=None)
+MEASURED_TRAFFIC_SETTING = dict(level=8, rays=1 << 24, sort_bits=0, lds_stack=8, refill=32, waves_per_cu=0,
+                                xcd_ranges=1, ray_chunk=128, vis_refill=40)
 RAY_BYTES, HIT_BYTES, VIS_BYTES = 32, 16, 4
 NODE_BYTES, TRI_BYTES, SPHERE_BYTES = 32, 36, 16
 
@@ -421,8 +422,9 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
     ap.add_argument("--guide-inference-only", action="store_true",
                     help="for profiles: only the guide inference timing (4M queries), printed as JSON")
-    ap.add_argument("--tuning", default="auto",
-                    help="'auto' (autotune on the closest-hit batch before timing), 'default', or "
+    ap.add_argument("--tuning", default="default",
+                    help="'default' (the library's setting), 'auto' (autotune on the step's own mix "
+                         "before timing), or "
                          "k=v pairs, e.g. sort_bits=18,lds_stack=16,waves_per_cu=0,refill=16")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: one %d MiB device copy (calibration), then one launch "
@@ -556,7 +558,7 @@ def main():
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
     setting = dict(level=args.level, rays=n, **{k: tuning.get(k) for k in
                                                 ("sort_bits", "lds_stack", "refill", "waves_per_cu", "xcd_ranges",
-                                                 "ray_chunk")})
+                                                 "ray_chunk", "vis_refill")})
     traffic_dom = l2_hit = None
     if setting == MEASURED_TRAFFIC_SETTING and args.occlusion == "exact":
         traffic_dom = int(MEASURED_TRAFFIC[dominant])

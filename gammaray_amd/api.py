@@ -171,7 +171,7 @@ class Stage:
         check(lib().gr_scene_get_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "get_tuning")
         return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
 
-    def set_tuning(self, sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=0, ray_chunk=128,
+    def set_tuning(self, sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=1, ray_chunk=128,
                    vis_refill=40):
         t = _lib.Tuning(sort_bits, lds_stack, waves_per_cu, refill, xcd_ranges, ray_chunk, vis_refill)
         check(lib().gr_scene_set_tuning(self._h, ctypes.byref(t)), self.ctx.handle, "set_tuning")
@@ -450,7 +450,7 @@ def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, u
     p.job_order = JOB_ORDERS[job_order]
     p.sample_ring = sample_ring
     if tuning:
-        t = dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=0, ray_chunk=128, vis_refill=40)
+        t = dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=1, ray_chunk=128, vis_refill=40)
         t.update(tuning)
         p.tuning = _lib.Tuning(**t)
     ms = (ctypes.c_double * n)()

@@ -493,8 +493,8 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    gr::TraceTuning best{};  // the defaults: sort off, 8 LDS entries, occupancy limit, refill at 32 (40 for
-                            // visibility), 128-ray chunks
+    gr::TraceTuning best{};  // the defaults: sort off, XCD ranges, 8 LDS entries, occupancy limit,
+                            // refill at 32 (40 for visibility), 128-ray chunks
     float best_ms = 0.0f;
     auto timed = [&](const gr::TraceTuning& tu, float* ms) {
         return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
@@ -517,7 +517,7 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     }
     {
         gr::TraceTuning c = best;
-        c.xcd_ranges = 1;
+        c.xcd_ranges = best.xcd_ranges ? 0u : 1u;
         consider(c);
     }
     for (uint32_t lds : {8u, 16u, 32u}) {

@@ -142,7 +142,7 @@ struct TraceTuning {
     uint32_t waves_per_cu = 0;
     uint32_t refill = 32;      // closest-hit kernels
     uint32_t vis_refill = 40;  // visibility kernels (their segments end at the first occluder)
-    uint32_t xcd_ranges = 0;
+    uint32_t xcd_ranges = 1;
     uint32_t ray_chunk = kDefaultRayChunk;  // rays a wave reserves per atomic on the work counter
 };
 // Ray counters of the persistent traversal: one per XCD range, each on its own

@@ -183,8 +183,9 @@ typedef struct {
     uint32_t waves_per_cu; /* resident traversal waves per CU; 0 = occupancy limit */
     uint32_t refill;       /* idle lanes (1..64) at which a closest-hit wave fetches new rays
                               (default 32) */
-    uint32_t xcd_ranges;   /* 1: the (sorted) batch is cut into one contiguous range per XCD,
-                              each drained by the waves resident on that XCD first */
+    uint32_t xcd_ranges;   /* 1 (default): the (sorted) batch is cut into one contiguous range
+                              per XCD, each drained by the waves resident on that XCD first;
+                              0: one range */
     uint32_t ray_chunk;    /* rays a wave reserves per atomic on the work counter (1..65536);
                               0 = 128 */
     uint32_t vis_refill;   /* idle lanes (1..64) at which a visibility wave fetches new

@@ -519,7 +519,7 @@ def test_tuning_validation_and_rounding(host_ctx):
     from gammaray_amd.api import Stage
     from gammaray_amd._lib import GrError
     st = Stage(scenes.box_scene(), ctx=host_ctx)
-    assert st.get_tuning() == dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=0,
+    assert st.get_tuning() == dict(sort_bits=0, lds_stack=8, waves_per_cu=0, refill=32, xcd_ranges=1,
                                    ray_chunk=128, vis_refill=40)
     for want, got in [(0, 0), (5, 0), (6, 6), (8, 6), (20, 18), (30, 30)]:
         assert st.set_tuning(sort_bits=want)["sort_bits"] == got
