@@ -357,6 +357,7 @@ int gr_scene_commit(gr_scene* s) {
         v.root_hi[k] = lay.root_hi[k];
     }
     v.pad = slab_pad();
+    v.simple_boxes = lay.simple_boxes ? 1u : 0u;
     v.overflow = s->d_overflow;
     v.next_ray = s->d_next_ray;
     s->committed = true;

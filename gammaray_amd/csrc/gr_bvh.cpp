@@ -281,12 +281,19 @@ GpuLayout make_gpu_layout(const FlatTree& t, const std::vector<Prim>& prims) {
         return t.nodes[c].count ? (kLeafBit | t.nodes[c].offset) : rank[c];
     };
     g.nodes.resize(ni);
+    g.simple_boxes = true;
+    auto simple = [](const FlatNode& c) {
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(c.lo[k]) || !std::isfinite(c.hi[k]) || !(c.lo[k] <= c.hi[k])) return false;
+        return true;
+    };
     for (size_t i = 0; i < nn; ++i) {
         const FlatNode& nd = t.nodes[i];
         if (nd.count != 0) continue;
         const FlatNode& a = t.nodes[i + 1];
         const FlatNode& b = t.nodes[nd.offset];
         GpuNode& o = g.nodes[rank[i]];
+        g.simple_boxes = g.simple_boxes && simple(a) && simple(b);
         o.q0 = make_float4(a.lo[0], a.lo[1], a.lo[2], a.hi[0]);
         o.q1 = make_float4(a.hi[1], a.hi[2], b.lo[0], b.lo[1]);
         o.q2 = make_float4(b.lo[2], b.hi[0], b.hi[1], b.hi[2]);

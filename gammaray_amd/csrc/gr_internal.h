@@ -83,6 +83,7 @@ struct SceneView {
     uint32_t root_ref;
     float root_lo[3], root_hi[3];
     float pad;  // 1 + 2 gamma(3)
+    uint32_t simple_boxes;  // every child box finite with lo <= hi on each axis
     unsigned long long* overflow;
     uint32_t* next_ray;  // kRayCounters work counters of the persistent traversal kernels
 };
@@ -93,6 +94,7 @@ struct GpuLayout {
     std::vector<uint2> comp_ids;
     uint32_t root_ref;
     float root_lo[3], root_hi[3];
+    bool simple_boxes;  // every child box finite with lo <= hi on each axis
 };
 GpuLayout make_gpu_layout(const FlatTree& t, const std::vector<Prim>& prims);
 
@@ -103,6 +105,7 @@ struct SceneRef {
     const uint2* comp_ids;
     uint32_t root_ref;
     float root_lo[3], root_hi[3];
+    uint32_t simple_boxes;
 };
 
 // ---- ray sort (gr_sort.hip) -------------------------------------------------

@@ -2351,6 +2351,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         hs[k].comps = v.comps;
         hs[k].comp_ids = v.comp_ids;
         hs[k].root_ref = v.root_ref;
+        hs[k].simple_boxes = v.simple_boxes;
         for (int c = 0; c < 3; ++c) {
             hs[k].root_lo[c] = v.root_lo[c];
             hs[k].root_hi[c] = v.root_hi[c];

@@ -29,6 +29,7 @@ enum TraceMode { kClosest = 0, kOcclusionExact = 1, kOcclusionAnyHit = 2 };
 struct RayFrame {
     vec3 o, d, inv;
     bool nx, ny, nz;
+    bool plain;     // origin and 1/d finite, 1/d nonzero on every axis
     uint32_t sign;  // nx | ny << 1 | nz << 2: dir_is_neg by axis
 };
 
@@ -42,6 +43,8 @@ __device__ __forceinline__ RayFrame make_frame(vec3 o, vec3 d) {
     r.ny = d.y < 0.0f;
     r.nz = d.z < 0.0f;
     r.sign = (r.nx ? 1u : 0u) | (r.ny ? 2u : 0u) | (r.nz ? 4u : 0u);
+    r.plain = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(r.inv.x) && isfinite(r.inv.y) &&
+              isfinite(r.inv.z) && r.inv.x != 0.0f && r.inv.y != 0.0f && r.inv.z != 0.0f;
     return r;
 }
 
@@ -67,6 +70,30 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
     t_max = tz_max < t_max ? tz_max : t_max;
     *tmin = t_min;
     return ok_y & ok_z & (t_max > 0.0f);
+}
+
+// slab() for a plain ray (RayFrame::plain) and a box that is finite with
+// lo <= hi on each axis.  Then no operand is NaN and, per axis, the entry
+// distance is <= the exit distance, so the reference's staged comparisons
+// reduce to one interval test:
+//   pass  <=>  max(tx_min, ty_min, tz_min) <= E  and  E > 0,
+//   E = min(tx_max, ty_max, tz_max) * pad,
+// with the same entry distance max(tx_min, ty_min) (the z slab never raises
+// it).  Multiplying by pad > 0 is monotonic under rounding, so padding the
+// minimum equals the minimum of the padded exits bit for bit; a zero's sign
+// can differ from the staged form, which only feeds comparisons.
+__device__ __forceinline__ bool slab_plain(float lx, float ly, float lz, float hx, float hy, float hz,
+                                           const RayFrame& r, float pad, float* tmin) {
+    const float tx_min = ((r.nx ? hx : lx) - r.o.x) * r.inv.x;
+    const float tx_max = ((r.nx ? lx : hx) - r.o.x) * r.inv.x;
+    const float ty_min = ((r.ny ? hy : ly) - r.o.y) * r.inv.y;
+    const float ty_max = ((r.ny ? ly : hy) - r.o.y) * r.inv.y;
+    const float tz_min = ((r.nz ? hz : lz) - r.o.z) * r.inv.z;
+    const float tz_max = ((r.nz ? lz : hz) - r.o.z) * r.inv.z;
+    const float entry = __builtin_fmaxf(tx_min, ty_min);
+    const float exit = __builtin_fminf(__builtin_fminf(tx_max, ty_max), tz_max) * pad;
+    *tmin = entry;
+    return (__builtin_fmaxf(entry, tz_min) <= exit) & (exit > 0.0f);
 }
 
 // Mesh::intersect_world distance (src/geom/mesh.rs:278-304) from the stored
@@ -186,9 +213,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     uint32_t c_box = 0, c_tri = 0, c_sph = 0;
     RayFrame r;
     r.o = r.d = r.inv = mk(0.0f, 0.0f, 0.0f);
-    r.nx = r.ny = r.nz = false;
+    r.nx = r.ny = r.nz = r.plain = false;
     r.sign = 0u;
     // the lane's scene (the launch's scene unless MULTI)
+    uint32_t scene_plain = sv.simple_boxes;
+    bool plain = false;  // slab_plain() applies to this lane's ray and scene
     const GpuNode* nodes = sv.nodes;
     const GpuComp* comps = sv.comps;
     const uint2* comp_ids = sv.comp_ids;
@@ -317,9 +346,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                                 comps = sr.comps;
                                 comp_ids = sr.comp_ids;
                                 root = sr.root_ref;
+                                scene_plain = sr.simple_boxes;
                                 lo0 = sr.root_lo[0], lo1 = sr.root_lo[1], lo2 = sr.root_lo[2];
                                 hi0 = sr.root_hi[0], hi1 = sr.root_hi[1], hi2 = sr.root_hi[2];
                             }
+                            plain = r.plain && scene_plain != 0u;
                             if (slab(lo0, lo1, lo2, hi0, hi1, hi2, r, sv.pad, &tmin) && tmin < closest) {
                                 cur = root;
                                 active = true;
@@ -370,8 +401,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
                 // near child's here and the far child's at its pop (pop()).
                 if (COUNT) c_box += MODE == kClosest ? 2 : 1;
                 float t0 = 0.0f, t1 = 0.0f;
-                const bool s0 = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
-                const bool s1 = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
+                bool s0, s1;
+                if (plain) {
+                    s0 = slab_plain(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
+                    s1 = slab_plain(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
+                } else {
+                    s0 = slab(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, r, sv.pad, &t0);
+                    s1 = slab(q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r, sv.pad, &t1);
+                }
                 const uint32_t axis = __float_as_uint(q3.z);
                 const bool flip = ((r.sign >> axis) & 1u) != 0u;
                 const uint32_t near_ref = __float_as_uint(flip ? q3.y : q3.x);

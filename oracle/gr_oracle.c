@@ -203,6 +203,23 @@ static int bb_intersect(const obbox *b, ovec o, ovec inv, const int neg[3], floa
     return t_min < max_dist && t_max > 0.0f;
 }
 
+/* BBox::intersect on n (box, ray) pairs: boxes as lo.xyz hi.xyz, rays as
+ * o.xyz d.xyz, 1/d and the direction signs as Ray::compute_intersection_data
+ * (src/core/ray.rs:31-42) forms them.  out[i] = 1 on a hit before max_dist[i]. */
+void orc_bbox_intersect(const float *boxes, const float *rays, const float *max_dist, int64_t n,
+                        uint8_t *out)
+{
+    const float pad = slab_pad();
+    for (int64_t i = 0; i < n; ++i) {
+        const float *b = boxes + 6 * i, *r = rays + 6 * i;
+        const obbox bb = {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}};
+        const ovec o = v_new(r[0], r[1], r[2]);
+        const ovec inv = v_new(1.0f / r[3], 1.0f / r[4], 1.0f / r[5]);
+        const int neg[3] = {r[3] < 0.0f, r[4] < 0.0f, r[5] < 0.0f};
+        out[i] = (uint8_t)bb_intersect(&bb, o, inv, neg, max_dist[i], pad);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Prim bounds and intersection                                             */
 /* ------------------------------------------------------------------------ */

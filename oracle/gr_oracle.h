@@ -116,6 +116,9 @@ size_t orc_exr_encode(const float *rgbw, uint32_t width, uint32_t height, uint8_
 
 /* Philox4x32-10 block function (counter-based RNG shared by the GPU path). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* BBox::intersect per (box, ray, max_dist); boxes lo.xyz hi.xyz, rays o.xyz d.xyz. */
+void orc_bbox_intersect(const float *boxes, const float *rays, const float *max_dist, int64_t n,
+                        uint8_t *out);
 
 #ifdef __cplusplus
 }

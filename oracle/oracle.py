@@ -76,6 +76,7 @@ def lib():
         L.orc_visibility.argtypes = [_vp, _f32p, ctypes.c_int64, _u8p, _u32p, ctypes.c_int, ctypes.c_int]
         L.orc_visibility.restype = ctypes.c_int64
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_bbox_intersect.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int64, _u8p]
         L.orc_camera_matrix.argtypes = [_f32p, _f32p, _f32p]
         L.orc_camera_rays.argtypes = [_f32p, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_int64, _f32p]
         L.orc_mitchell1.argtypes = [ctypes.c_float]
@@ -105,6 +106,19 @@ def philox4x32_10(ctr, key):
     out = np.zeros(4, dtype=np.uint32)
     lib().orc_philox4x32_10(_p(c, _u32p), _p(k, _u32p), _p(out, _u32p))
     return out
+
+
+def bbox_intersect(boxes, rays, max_dist):
+    """BBox::intersect (src/core/bbox.rs:78-113) on (n, 6) boxes (lo, hi), (n, 6)
+    rays (o, d) and n max distances; returns n bools."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    m = np.ascontiguousarray(max_dist, dtype=np.float32)
+    n = len(b)
+    assert b.shape == (n, 6) and r.shape == (n, 6) and m.shape == (n,)
+    out = np.zeros(n, dtype=np.uint8)
+    lib().orc_bbox_intersect(_p(b, _f32p), _p(r, _f32p), _p(m, _f32p), n, _p(out, _u8p))
+    return out.astype(bool)
 
 
 def default_threads():
