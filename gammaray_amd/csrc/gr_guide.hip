@@ -32,6 +32,7 @@ constexpr int kAFrags = 29;  // layer-input fragments per 16-query tile: 13 + 4 
 constexpr int kZFrags = 18;  // dZ fragments per tile: 4 * 4 + 2
 constexpr int kDwTiles = 108;
 constexpr int kDwSplit = 16;
+constexpr int kGuideBiases = 4 * kGuideWidth + kGuideOut;  // 288, in the forward pass's order
 constexpr float kGradClip = 1.0e4f;
 constexpr float kValueClip = 20.0f;
 
@@ -342,8 +343,9 @@ __device__ __forceinline__ half4_t to_half4(float4_t v) {
     return h;
 }
 
-// Sum over the 16 queries of a C-layout tile, added to grad[off + 4(lane/16) + c].
-__device__ __forceinline__ void add_bias_grad(float4_t v, float* grad, int off, uint32_t lane) {
+// Sum over the 16 queries of a C-layout tile, stored to the tile's bias
+// partials at [off + 4(lane/16) + c] (biases in the forward pass's order).
+__device__ __forceinline__ void add_bias_grad(float4_t v, float* tile_bias, int off, uint32_t lane) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float s = v[c];
@@ -351,8 +353,13 @@ __device__ __forceinline__ void add_bias_grad(float4_t v, float* grad, int off, 
         s += __shfl_xor(s, 2);
         s += __shfl_xor(s, 4);
         s += __shfl_xor(s, 8);
-        if ((lane & 15u) == 0) atomicAdd(grad + off + 4 * (lane >> 4) + c, s);
+        if ((lane & 15u) == 0) tile_bias[off + 4 * (lane >> 4) + c] = s;
     }
+}
+
+// Parameter index of bias i in the forward pass's order (b1..b4, then b5).
+__host__ __device__ __forceinline__ int bias_param(int i) {
+    return i < 4 * kGuideWidth ? guide_b_off(i / kGuideWidth) + i % kGuideWidth : kB5 + (i - 4 * kGuideWidth);
 }
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
@@ -365,13 +372,13 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 }
 
 // Forward + backward of one 16-query tile per wave.  Writes the dW operands
-// (layer inputs and dZ, transposed into fragment order) and adds the bias
+// (layer inputs and dZ, transposed into fragment order) and the tile's bias
 // gradients.  Records: the first n of `recs`; batch entry i uses record i
 // (sequential) or a hashed draw (training).
 __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* __restrict__ fwd,
                                                                const uint16_t* __restrict__ bwd,
                                                                const float* __restrict__ params,
-                                                               float* __restrict__ grad,
+                                                               float* __restrict__ bias_part,
                                                                const float* __restrict__ recs, uint32_t n_max,
                                                                const uint32_t* __restrict__ n_ptr,
                                                                const float* __restrict__ stats, uint32_t batch,
@@ -447,6 +454,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
 
     uint16_t* at = at_buf + (size_t)tile * kAFrags * kFragHalves;
     uint16_t* dz = dz_buf + (size_t)tile * kZFrags * kFragHalves;
+    float* tb = bias_part + (size_t)tile * kGuideBiases;
     _Float16* lds = s_t[wave];
 
     // layer 5 (index 4): dZ = dz5
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
     for (int ob = 0; ob < 2; ++ob) {
         zh[ob] = to_half4(dz5[ob]);
         store_transposed(zh[ob], lds, dz + (16 + ob) * kFragHalves, lane);
-        add_bias_grad(dz5[ob], grad, kB5 + ob * 16, lane);
+        add_bias_grad(dz5[ob], tb, 4 * kGuideWidth + ob * 16, lane);
     }
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) store_transposed(h[3][kb], lds, at + (25 + kb) * kFragHalves, lane);
@@ -482,7 +490,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
         for (int ib = 0; ib < 4; ++ib) {
             zh[ib] = to_half4(d[ib]);
             store_transposed(zh[ib], lds, dz + (4 * l + ib) * kFragHalves, lane);
-            add_bias_grad(d[ib], grad, guide_b_off(l) + ib * 16, lane);
+            add_bias_grad(d[ib], tb, l * kGuideWidth + ib * 16, lane);
         }
         if (l > 0) {
 #pragma unroll
@@ -501,11 +509,12 @@ __global__ __launch_bounds__(kTrainThreads) void k_guide_train(const uint16_t* _
     }
 }
 
-// dW_l[in][out] += sum over the batch of A_{l-1}[q][in] dZ_l[q][out]: one
-// 16x16 output tile per block, the batch split kDwSplit ways.
+// dW_l[in][out] = sum over split y's tiles of A_{l-1}[q][in] dZ_l[q][out]: one
+// 16x16 output tile per block, the batch split kDwSplit ways (tiles y, y +
+// kDwSplit, ...), each split's sum stored to part[y].
 __global__ __launch_bounds__(64) void k_guide_dw(const uint16_t* __restrict__ at_buf,
                                                  const uint16_t* __restrict__ dz_buf, uint32_t tiles,
-                                                 float* __restrict__ grad) {
+                                                 float* __restrict__ part) {
     const uint32_t lane = threadIdx.x;
     const int id = (int)blockIdx.x;
     int l, ib, ob;
@@ -531,8 +540,28 @@ __global__ __launch_bounds__(64) void k_guide_dw(const uint16_t* __restrict__ at
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int in = ib * 16 + 4 * (int)(lane >> 4) + c;
-        if (acc[c] != 0.0f) atomicAdd(grad + guide_w_off(l) + in * nout + out, acc[c]);
+        part[(size_t)blockIdx.y * kGuideParams + guide_w_off(l) + in * nout + out] = acc[c];
     }
+}
+
+// Bias gradients of split y: the sum of its tiles' partials, in tile order.
+__global__ __launch_bounds__(64) void k_guide_bias_reduce(const float* __restrict__ bias_part, uint32_t tiles,
+                                                          float* __restrict__ part) {
+    const int b = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (b >= kGuideBiases) return;
+    float s = 0.0f;
+    for (uint32_t t = blockIdx.y; t < tiles; t += gridDim.y) s += bias_part[(size_t)t * kGuideBiases + b];
+    part[(size_t)blockIdx.y * kGuideParams + bias_param(b)] = s;
+}
+
+// grad += the splits' partial gradients, in split order.
+__global__ __launch_bounds__(256) void k_guide_grad_reduce(const float* __restrict__ part, float* __restrict__ grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kGuideParams) return;
+    float g = grad[p];
+#pragma unroll
+    for (int y = 0; y < kDwSplit; ++y) g += part[(size_t)y * kGuideParams + p];
+    grad[p] = g;
 }
 
 __device__ __forceinline__ void store_fragments(uint16_t* fwd, uint16_t* bwd, uint16_t* fwd32, int p, float w) {
@@ -581,9 +610,10 @@ __global__ __launch_bounds__(256) void k_guide_fill(float* __restrict__ a, uint3
     if (i < n) a[i] = value;
 }
 
+// Per-block sums of the first n record values (a fixed tree per block).
 __global__ __launch_bounds__(256) void k_guide_value_sum(const float* __restrict__ recs, uint32_t n_max,
                                                          const uint32_t* __restrict__ n_ptr,
-                                                         float* __restrict__ stats) {
+                                                         float* __restrict__ vpart) {
     __shared__ float s[256];
     uint32_t n = n_max;
     if (n_ptr) {
@@ -597,7 +627,22 @@ __global__ __launch_bounds__(256) void k_guide_value_sum(const float* __restrict
         if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && s[0] != 0.0f) atomicAdd(stats, s[0]);
+    if (threadIdx.x == 0) vpart[blockIdx.x] = s[0];
+}
+
+// stats[0] += the sum of the block sums (one block, a fixed order).
+__global__ __launch_bounds__(256) void k_guide_value_total(const float* __restrict__ vpart, uint32_t nblocks,
+                                                           float* __restrict__ stats) {
+    __shared__ float s[256];
+    float run = 0.0f;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256u) run += vpart[b];
+    s[threadIdx.x] = run;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) stats[0] += s[0];
 }
 
 __global__ __launch_bounds__(256) void k_guide_scale_copy(const float* __restrict__ grad, float inv,
@@ -648,12 +693,15 @@ hipError_t reserve_work(gr_guide* g, uint32_t batch) {
     g->work = nullptr;
     g->work_tiles = 0;
     const size_t a = (size_t)tiles * kAFrags * kFragHalves * 2, z = (size_t)tiles * kZFrags * kFragHalves * 2;
+    const size_t bp = (size_t)tiles * kGuideBiases * 4, pt = (size_t)kDwSplit * kGuideParams * 4;
     char* p = nullptr;
-    hipError_t e = hipMalloc(&p, a + z);
+    hipError_t e = hipMalloc(&p, a + z + bp + pt);
     if (e != hipSuccess) return e;
     g->work = p;
     g->at = reinterpret_cast<uint16_t*>(p);
     g->dz = reinterpret_cast<uint16_t*>(p + a);
+    g->bias_part = reinterpret_cast<float*>(p + a + z);
+    g->part = reinterpret_cast<float*>(p + a + z + bp);
     g->work_tiles = tiles;
     return hipSuccess;
 }
@@ -665,9 +713,13 @@ hipError_t accumulate_gradient(gr_guide* g, const float* recs, uint32_t n_max, c
     if (e != hipSuccess) return e;
     const uint32_t tiles = (batch + 15) / 16;
     const uint32_t blocks = (tiles + kTrainThreads / 64 - 1) / (kTrainThreads / 64);
-    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(kTrainThreads), 0, s, g->fwd, g->bwd, g->params, g->grad,
-                       recs, n_max, n_ptr, g->stats, batch, draw_key, sequential ? 1u : 0u, g->at, g->dz);
-    hipLaunchKernelGGL(k_guide_dw, dim3(kDwTiles, kDwSplit), dim3(64), 0, s, g->at, g->dz, tiles, g->grad);
+    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(kTrainThreads), 0, s, g->fwd, g->bwd, g->params,
+                       g->bias_part, recs, n_max, n_ptr, g->stats, batch, draw_key, sequential ? 1u : 0u, g->at,
+                       g->dz);
+    hipLaunchKernelGGL(k_guide_dw, dim3(kDwTiles, kDwSplit), dim3(64), 0, s, g->at, g->dz, tiles, g->part);
+    hipLaunchKernelGGL(k_guide_bias_reduce, dim3((kGuideBiases + 63) / 64, kDwSplit), dim3(64), 0, s,
+                       g->bias_part, tiles, g->part);
+    hipLaunchKernelGGL(k_guide_grad_reduce, dim3(blocks256(kGuideParams)), dim3(256), 0, s, g->part, g->grad);
     return hipGetLastError();
 }
 
@@ -747,7 +799,17 @@ hipError_t guide_zero_stats(gr_guide* g, hipStream_t s) {
 
 hipError_t guide_record_stats(gr_guide* g, const float* recs, uint32_t n_max, const uint32_t* n_ptr, hipStream_t s) {
     if (n_max == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_value_sum, dim3(blocks256(n_max)), dim3(256), 0, s, recs, n_max, n_ptr, g->stats);
+    const uint32_t nb = blocks256(n_max);
+    if (nb > g->vpart_cap) {
+        (void)hipFree(g->vpart);
+        g->vpart = nullptr;
+        g->vpart_cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&g->vpart), (size_t)nb * sizeof(float));
+        if (e != hipSuccess) return e;
+        g->vpart_cap = nb;
+    }
+    hipLaunchKernelGGL(k_guide_value_sum, dim3(nb), dim3(256), 0, s, recs, n_max, n_ptr, g->vpart);
+    hipLaunchKernelGGL(k_guide_value_total, dim3(1), dim3(256), 0, s, g->vpart, nb, g->stats);
     return hipGetLastError();
 }
 
@@ -775,9 +837,12 @@ hipError_t guide_reserve_records(gr_guide* g, size_t n) {
     (void)hipFree(g->recs_blob);
     g->recs_blob = nullptr;
     g->recs_cap = 0;
-    hipError_t e = hipMalloc(&g->recs_blob, n * kRecordFloats * sizeof(float));
+    const size_t nb = (n + 255) / 256, rec_bytes = (n * kRecordFloats * sizeof(float) + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&g->recs_blob, rec_bytes + 2 * nb * sizeof(uint32_t));
     if (e != hipSuccess) return e;
     g->recs = reinterpret_cast<float*>(g->recs_blob);
+    g->rec_bcount = reinterpret_cast<uint32_t*>(static_cast<char*>(g->recs_blob) + rec_bytes);
+    g->rec_boff = g->rec_bcount + nb;
     g->recs_cap = n;
     return hipSuccess;
 }
@@ -841,6 +906,7 @@ void gr_guide_destroy(gr_guide* g) {
     (void)hipFree(g->blob);
     (void)hipFree(g->work);
     (void)hipFree(g->recs_blob);
+    (void)hipFree(g->vpart);
     gr_ctx* ctx = g->ctx;
     delete g;
     gr::ctx_release(ctx);

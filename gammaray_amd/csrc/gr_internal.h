@@ -257,14 +257,22 @@ struct gr_guide {
     float* stats = nullptr;                   // [0] sum of record values
     uint32_t* count = nullptr;                // records available for training
     uint64_t steps = 0;                       // Adam steps taken
-    // dW operands of one training batch, grown on demand
+    // dW operands of one training batch, grown on demand, with the gradient's
+    // partial sums: per-tile bias sums and one full gradient per split of the
+    // batch, summed in a fixed order (no float atomics: training is reproducible)
     void* work = nullptr;
     uint32_t work_tiles = 0;
     uint16_t *at = nullptr, *dz = nullptr;
-    // training records of one traced batch (compacted), grown on demand
+    float *bias_part = nullptr, *part = nullptr;
+    // training records of one traced batch (compacted in slot order), grown on
+    // demand, with the compaction's per-block counts and offsets
     void* recs_blob = nullptr;
     size_t recs_cap = 0;
     float* recs = nullptr;
+    uint32_t *rec_bcount = nullptr, *rec_boff = nullptr;
+    // per-block partial sums of record values, grown on demand
+    float* vpart = nullptr;
+    size_t vpart_cap = 0;
 };
 
 namespace gr {

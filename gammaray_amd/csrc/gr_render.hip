@@ -864,37 +864,89 @@ __global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t dept
     }
 }
 
-// Vertex records -> training records (raw, direction, f cos Li / pdf as
+// Vertex record s -> training record (raw, direction, f cos Li / pdf as
 // luminance: the radiance the path gathered after the vertex, divided by the
-// throughput it arrived with).  Clears the records for the next batch.
-__global__ __launch_bounds__(256) void k_guide_finalize(RenderArgs a, uint32_t nslots) {
+// throughput it arrived with).  *had: the slot held a record.
+__device__ __forceinline__ bool guide_record(const RenderArgs& a, uint32_t s, float4* r0, float4* r1, float4* r2,
+                                             float* v, bool* had) {
+    const float4* vr = a.gd.vrec + (size_t)s * kVrecFloat4;
+    const float4 r3 = vr[3];
+    *had = r3.w != 0.0f;
+    if (!*had) return false;
+    *r0 = vr[0];
+    *r1 = vr[1];
+    *r2 = vr[2];
+    const uint32_t p = (s / a.gd.drec) * a.gd.train_stride;
+    const float4 L = p < a.npaths ? a.rb.L[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const vec3 dl = mk(L.x - r2->w, L.y - r3.x, L.z - r3.y);
+    const float tl = r3.z;
+    *v = tl > 0.0f ? fmaxf(dev::luminance(dl), 0.0f) / tl : 0.0f;
+    return (*v == *v) && *v < 3.0e38f;
+}
+
+// Training records are compacted in slot order (count, scan, write), so the
+// record list -- and the training batches drawn from it -- do not depend on
+// the order in which blocks run.
+__global__ __launch_bounds__(256) void k_guide_rec_count(RenderArgs a, uint32_t nslots, uint32_t* bcount) {
+    __shared__ uint32_t s_c[4];
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool ok = false;
-    float4 r0, r1, r2, r3;
+    float4 r0, r1, r2;
     float v = 0.0f;
-    if (s < nslots) {
-        float4* vr = a.gd.vrec + (size_t)s * kVrecFloat4;
-        r3 = vr[3];
-        if (r3.w != 0.0f) {
-            r0 = vr[0];
-            r1 = vr[1];
-            r2 = vr[2];
-            vr[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const uint32_t p = (s / a.gd.drec) * a.gd.train_stride;
-            const float4 L = p < a.npaths ? a.rb.L[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const vec3 dl = mk(L.x - r2.w, L.y - r3.x, L.z - r3.y);
-            const float tl = r3.z;
-            v = tl > 0.0f ? fmaxf(dev::luminance(dl), 0.0f) / tl : 0.0f;
-            ok = (v == v) && v < 3.0e38f;
-        }
+    bool had = false;
+    const bool ok = s < nslots && guide_record(a, s, &r0, &r1, &r2, &v, &had);
+    const uint64_t m = __ballot(ok);
+    if ((threadIdx.x & 63u) == 0) s_c[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+// Exclusive scan of the per-block counts (one block); *total = their sum.
+__global__ __launch_bounds__(1024) void k_guide_rec_scan(const uint32_t* bcount, uint32_t nblocks, uint32_t* boff,
+                                                         uint32_t* total) {
+    __shared__ uint32_t s_sum[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblocks + 1023u) / 1024u;
+    const uint32_t b0 = t * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    uint32_t run = 0;
+    for (uint32_t b = b0; b < b1; ++b) run += bcount[b];
+    s_sum[t] = run;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t add = t >= off ? s_sum[t - off] : 0u;
+        __syncthreads();
+        s_sum[t] += add;
+        __syncthreads();
     }
-    const uint32_t k = block_append(ok, a.gd.rec_count);
+    run = t ? s_sum[t - 1] : 0u;
+    for (uint32_t b = b0; b < b1; ++b) {
+        boff[b] = run;
+        run += bcount[b];
+    }
+    if (t == 1023u) *total = s_sum[1023];
+}
+
+// Writes each record at its block's offset plus its rank in the block, and
+// clears the vertex records for the next batch.
+__global__ __launch_bounds__(256) void k_guide_rec_write(RenderArgs a, uint32_t nslots, const uint32_t* boff) {
+    __shared__ uint32_t s_c[4];
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    float4 r0, r1, r2;
+    float v = 0.0f;
+    bool had = false;
+    const bool ok = s < nslots && guide_record(a, s, &r0, &r1, &r2, &v, &had);
+    const uint64_t m = __ballot(ok);
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) s_c[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t k = boff[blockIdx.x] + (uint32_t)__popcll(m & lanes_below());
+    for (uint32_t w = 0; w < wave; ++w) k += s_c[w];
     if (ok && k < a.gd.rec_cap) {
         float* o = a.gd.recs + (size_t)k * kRecordFloats;
         o[0] = r0.x, o[1] = r0.y, o[2] = r0.z, o[3] = r0.w;
         o[4] = r1.x, o[5] = r1.y, o[6] = r1.z, o[7] = r1.w;
         o[8] = r2.x, o[9] = r2.y, o[10] = r2.z, o[11] = v;
     }
+    if (had) a.gd.vrec[(size_t)s * kVrecFloat4 + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_export_samples(RenderArgs a, float4* out) {
@@ -2056,10 +2108,13 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         if (e != hipSuccess) break;
         if (guided) {
             // train on this batch's records before the next batch samples the guide
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, guide->count, 0u);
             e = guide_zero_stats(guide, s);
             if (e != hipSuccess) break;
-            hipLaunchKernelGGL(k_guide_finalize, blocks_for(nslots), dim3(256), 0, s, a, nslots);
+            const dim3 rb = blocks_for(nslots);
+            hipLaunchKernelGGL(k_guide_rec_count, rb, dim3(256), 0, s, a, nslots, guide->rec_bcount);
+            hipLaunchKernelGGL(k_guide_rec_scan, dim3(1), dim3(1024), 0, s, guide->rec_bcount, rb.x,
+                               guide->rec_boff, guide->count);
+            hipLaunchKernelGGL(k_guide_rec_write, rb, dim3(256), 0, s, a, nslots, guide->rec_boff);
             e = guide_record_stats(guide, guide->recs, (uint32_t)guide->recs_cap, guide->count, s);
             if (e == hipSuccess)
                 e = guide_adam_steps(guide, guide->recs, (uint32_t)guide->recs_cap, guide->count,

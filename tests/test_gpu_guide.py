@@ -243,6 +243,30 @@ def test_guided_render_has_the_plain_mean(variant, gpu_ctx):
     assert_three_sigma(a[32:].sum(2), b[32:].sum(2), "guided " + variant)
 
 
+def test_guided_render_is_bit_reproducible(gpu_ctx):
+    """Training records are compacted in slot order and every gradient sum is
+    taken in a fixed order (no float atomics), so a guided render -- the
+    records, the Adam steps trained on them, the guided samples and the film
+    -- repeats bit for bit."""
+    print("case guided reproducible", flush=True)
+    spec = scenes.indirect_room_scene()
+    W, H = 48, 40
+    stage = Stage(spec, ctx=gpu_ctx)
+    out = []
+    for _ in range(2):
+        g = Guide(gpu_ctx, seed=5, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=4, alpha=0.5,
+                  guided_depth=2)
+        f = Film(gpu_ctx, W, H)
+        trace(stage, f, "guided", spp=24, max_depth=6, seed=61, batch=4, guide=g)
+        out.append((f.readback(), g.get_params(), g.last_records().cpu().numpy(), g.steps()))
+    (fa, pa, ra, sa), (fb, pb, rb, sb) = out
+    assert sa == sb == 4 * 24 // 4
+    assert ra.shape == rb.shape and ra.shape[0] > 100
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(np.asarray(pa, np.float32).view(np.uint32), np.asarray(pb, np.float32).view(np.uint32))
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+
+
 def test_guide_reduces_variance_under_indirect_light(gpu_ctx):
     print("case guided variance", flush=True)
     spec = scenes.indirect_room_scene()
