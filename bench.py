@@ -404,6 +404,81 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
             "time_ratio_rmse_untrimmed": round(reached / t_plain, 3) if reached else None}
 
 
+def _without_inconsistent_lobes(spec):
+    """The scene with clearcoat and specular transmission set to zero on every
+    Disney material: all remaining lobes report their sampling densities."""
+    for prim in spec.prims:
+        m = prim.material
+        if m[0] == 0:
+            m[8] = 0.0   # specular_trans
+            m[12] = 0.0  # clearcoat
+    return spec
+
+
+GUIDE_SWEEP = [("alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1)),
+               ("alpha 0.5, first vertex", dict(alpha=0.5, guided_depth=1)),
+               ("alpha 0.7, first vertex", dict(alpha=0.7, guided_depth=1)),
+               ("alpha 0.3, two vertices", dict(alpha=0.3, guided_depth=2)),
+               ("alpha 0.5, three vertices", dict(alpha=0.5, guided_depth=3)),
+               ("alpha 0.3, first vertex, 16 Adam steps per batch", dict(alpha=0.3, guided_depth=1, steps_per_pass=16))]
+
+
+def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128):
+    """Guide settings on the 1080p showcase scene and on the same scene without
+    its clearcoat and transmission lobes: trimmed relMSE against a ref_spp
+    PT+NEE render and wall time after every 8-spp batch, for plain PT+NEE and
+    for each setting in GUIDE_SWEEP (online training included)."""
+    from gammaray_amd import scenes
+    from gammaray_amd.api import Film, Guide, Stage, trace
+
+    def image(f):
+        px = f.readback()
+        return px[..., :3] / px[..., 3:4]
+
+    def timed(fn):
+        ctx.synchronize()
+        t = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        return time.perf_counter() - t
+
+    def curve(st, R, integrator, guide=None):
+        f = Film(ctx, W, H)
+        if guide is None:
+            trace(st, f, "pt_nee", spp=1, max_depth=8, seed=5)
+            f.clear()
+        tot, rows = 0.0, []
+        for k in range(0, spp, 8):
+            tot += timed(lambda: trace(st, f, integrator, spp=8, max_depth=8, seed=9, sample_offset=k, batch=8,
+                                       guide=guide))
+            rows.append((k + 8, round(tot, 4), round(_rel_mse_trimmed(image(f), R), 6)))
+        f.close()
+        return rows
+
+    out = {}
+    for name, spec in (("showcase", scenes.showcase_scene()),
+                       ("showcase without clearcoat and transmission",
+                        _without_inconsistent_lobes(scenes.showcase_scene()))):
+        st = Stage(spec, ctx=ctx)
+        ref = Film(ctx, W, H)
+        trace(st, ref, "pt_nee", spp=ref_spp, max_depth=8, seed=1000)
+        R = image(ref)
+        ref.close()
+        res = {"plain PT+NEE": curve(st, R, "pt_nee")}
+        log("guide sweep, %s: plain %s" % (name, res["plain PT+NEE"][-1]))
+        for label, cfg in GUIDE_SWEEP:
+            c = dict(alpha=0.3, guided_depth=1, steps_per_pass=8, batch=16384, train_paths=1 << 16,
+                     warmup_steps=8, learning_rate=5e-3)
+            c.update(cfg)
+            g = Guide(ctx, **c)
+            res[label] = curve(st, R, "guided", g)
+            g.close()
+            log("guide sweep, %s: %s %s" % (name, label, res[label][-1]))
+        out[name] = res
+        st.close()
+    return {"reference_spp": ref_spp, "columns": ["spp", "s", "relmse_trimmed"], "curves": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,6 +495,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
+    ap.add_argument("--guide-sweep", action="store_true",
+                    help="only the guide-settings sweep on the 1080p scene (with and without its clearcoat and "
+                         "transmission lobes), printed as JSON")
     ap.add_argument("--guide-inference-only", action="store_true",
                     help="for profiles: only the guide inference timing (4M queries), printed as JSON")
     ap.add_argument("--tuning", default="default",
@@ -440,7 +518,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if not args.no_cpu_baseline and not args.guide_inference_only and not args.traffic_probe:
+    if not args.no_cpu_baseline and not args.guide_inference_only and not args.traffic_probe \
+            and not args.guide_sweep:
         use_native_oracle()
 
     import torch
@@ -457,6 +536,11 @@ def main():
         ctx = Context(0)
         ctx.set_stream(stream.cuda_stream)
         print(json.dumps(guide_inference_rate(ctx, Guide(ctx))), flush=True)
+        return
+    if args.guide_sweep:
+        ctx = Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        print(json.dumps(bench_guide_sweep(ctx)), flush=True)
         return
     t0 = time.time()
     spec = scenes.icosphere_scene(level=args.level, seed=2)
