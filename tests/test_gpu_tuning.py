@@ -1,6 +1,6 @@
 """Traversal settings -- coherence sort of the rays, LDS stack depth, resident
-waves per CU, refill threshold -- change the order in which rays run, never a
-result.  Every setting must reproduce the oracle's hits bit for bit, write
+waves per CU, refill thresholds, ray chunks per work-counter atomic, XCD
+ranges -- change the order in which rays run, never a result.  Every setting must reproduce the oracle's hits bit for bit, write
 every output slot exactly once, do the same reference-order work per ray, and
 leave rendered films bit-identical.
 """
@@ -22,6 +22,10 @@ TUNINGS = [
     dict(sort_bits=24, waves_per_cu=2, refill=4),
     dict(sort_bits=18, xcd_ranges=1),
     dict(xcd_ranges=1, refill=1, waves_per_cu=1),
+    # one ray per work-counter atomic; a chunk beyond every wave's share (clamped)
+    dict(xcd_ranges=0, ray_chunk=1, refill=1, vis_refill=1),
+    dict(ray_chunk=65536, refill=64, vis_refill=64),
+    dict(sort_bits=15, xcd_ranges=1, ray_chunk=7, vis_refill=13),
 ]
 SENTINEL = 0x3F3F3F3F
 
@@ -91,12 +95,12 @@ def test_tuned_visibility_exact(tu, gpu_ctx):
 
 
 @pytest.mark.parametrize("n", [1, 7, 63, 64, 4095, 4097])
-@pytest.mark.parametrize("xcd", [0, 1])
-def test_sorted_small_batches(n, xcd, gpu_ctx):
-    print("case sorted small", n, xcd, flush=True)
+@pytest.mark.parametrize("xcd,chunk", [(0, 128), (1, 128), (1, 1)])
+def test_sorted_small_batches(n, xcd, chunk, gpu_ctx):
+    print("case sorted small", n, xcd, chunk, flush=True)
     spec = scenes.box_scene()
     stage = Stage(spec, ctx=gpu_ctx)
-    stage.set_tuning(sort_bits=30, xcd_ranges=xcd)
+    stage.set_tuning(sort_bits=30, xcd_ranges=xcd, ray_chunk=chunk)
     o = OracleScene(spec)
     rays = scenes.ray_batch(spec, n, seed=n)
     g = _closest(stage, rays)
