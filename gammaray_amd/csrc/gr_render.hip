@@ -2446,27 +2446,36 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     char* blob = nullptr;
     RenderBuffers rb;
     SortScratch ss;
-    uint32_t* h_done = nullptr;
-    unsigned long long* h_limit = nullptr;
-    hipStream_t s = nullptr;
+    // Polls are double-buffered: the steps of the next interval are queued
+    // before the host reads the previous interval's counters, and film gathers
+    // run on a stream of their own, so the GPU never waits for the host and
+    // gathers overlap the bounce steps.
+    uint32_t* h_done[2] = {nullptr, nullptr};
+    unsigned long long* h_limit = nullptr;  // two words, used alternately
+    hipStream_t s = nullptr, sg = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev_start = nullptr, ev_poll = nullptr, ev_ctx = nullptr;
+    hipEvent_t ev_start = nullptr, ev_ctx = nullptr, ev_poll[2] = {nullptr, nullptr}, ev_gath[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_job(njobs, nullptr);  // by generation index
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&blob), off);
     if (e == hipSuccess) e = reserve(rb, pool);
     if (e == hipSuccess && tu.sort_bits) e = reserve_sort(ss, pool);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_done), (size_t)niters * kDoneSpread * 4);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_limit), sizeof(unsigned long long));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+        e = hipHostMalloc(reinterpret_cast<void**>(&h_done[k]), (size_t)niters * kDoneSpread * 4);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&h_limit), 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sg, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
-    if (e == hipSuccess) e = hipEventCreate(&ev_poll);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipEventCreateWithFlags(&ev_poll[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_gath[k], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_ctx, hipEventDisableTiming);
     for (uint32_t j = 0; j < njobs && e == hipSuccess; ++j) e = hipEventCreate(&ev_job[j]);
     std::vector<unsigned long long> job_first(njobs);
     for (uint32_t g = 0; g < njobs; ++g) job_first[g] = hj[g].first;
     if (e == hipSuccess) {
-        *h_limit = ring;  // nothing gathered yet
+        h_limit[0] = h_limit[1] = ring;  // nothing gathered yet
         // start after the work already on the context's stream
         e = hipEventRecord(ev_ctx, ctx->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, ev_ctx, 0);
@@ -2558,22 +2567,47 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
     // (more when the ring holds generation back); a count far beyond that means
     // the counters are wrong -- stop with an error.
     const uint64_t max_pairs = (total / pool + 2) * 4096ull;
-    while (e == hipSuccess && pending > 0) {
-        for (uint32_t k = 0; k < (poll + 1) / 2 && e == hipSuccess; ++k) {
+    // One interval of steps, then a copy of the iteration counters into
+    // h_done[k] and ev_poll[k] behind it.
+    auto interval = [&](int k) -> hipError_t {
+        hipError_t r = hipSuccess;
+        for (uint32_t i = 0; i < (poll + 1) / 2 && r == hipSuccess; ++i) {
             if (exec) {
-                e = hipGraphLaunch(exec, s);
+                r = hipGraphLaunch(exec, s);
             } else {
-                e = step(0);
-                if (e == hipSuccess) e = step(1);
+                r = step(0);
+                if (r == hipSuccess) r = step(1);
             }
             ++pairs;
         }
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(h_done, b.done, (size_t)niters * kDoneSpread * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipEventRecord(ev_poll, s);
-        if (e == hipSuccess) e = hipEventSynchronize(ev_poll);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(h_done[k], b.done, (size_t)niters * kDoneSpread * 4, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipEventRecord(ev_poll[k], s);
+        return r;
+    };
+    // The limit moves to (first ungathered sample + ring) only once the gathers
+    // that freed those ring slots are complete: one poll behind the gathers.
+    uint64_t lo_issued = 0;  // first ungathered sample when the last gathers were queued
+    bool gath_pending = false;
+    int cur = 0;
+    if (e == hipSuccess) e = interval(0);
+    while (e == hipSuccess && pending > 0) {
+        e = interval(cur ^ 1);  // the GPU keeps working while the host reads poll `cur`
+        if (e == hipSuccess) e = hipEventSynchronize(ev_poll[cur]);
+        if (e == hipSuccess && gath_pending) {
+            // the previous poll's gathers have had a whole interval to finish
+            e = hipEventSynchronize(ev_gath[cur ^ 1]);
+            if (e == hipSuccess) {
+                // ev_poll[cur] follows the last copy from this word, so it may be rewritten
+                h_limit[cur] = lo_issued + ring;
+                e = hipMemcpyAsync(blob + o_lim, h_limit + cur, 8, hipMemcpyHostToDevice, s);
+            }
+        }
         // each job's iterations whose samples are all in, in order, into its film
+        // (on the gather stream, after the steps that wrote them)
+        if (e == hipSuccess) e = hipStreamWaitEvent(sg, ev_poll[cur], 0);
         uint64_t lo = total;  // first sample not yet gathered
+        const uint32_t* done = h_done[cur];
         for (uint32_t g = 0; g < njobs && e == hipSuccess; ++g) {
             const JobDev& d = hj[g];
             const gr_job& jb = jobs[order[g]];
@@ -2581,7 +2615,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             uint32_t k1 = gathered[g];
             while (k1 < jb.spp) {
                 uint64_t fin = 0;
-                for (uint32_t c = 0; c < kDoneSpread; ++c) fin += h_done[(size_t)(d.iter0 + k1) * kDoneSpread + c];
+                for (uint32_t c = 0; c < kDoneSpread; ++c) fin += done[(size_t)(d.iter0 + k1) * kDoneSpread + c];
                 if (fin < npx) break;
                 ++k1;
             }
@@ -2594,25 +2628,24 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
                 ga.ring_base = d.first + (uint64_t)gathered[g] * npx;
                 ga.ring_cap = ring;
                 const dim3 grid((ga.width + kTile - 1) / kTile, (ga.height + kTile - 1) / kTile);
-                hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, s, ga, jb.film->d_pixels,
+                hipLaunchKernelGGL(k_film_gather, grid, dim3(kTile * kTile), 0, sg, ga, jb.film->d_pixels,
                                    k1 - gathered[g], nullptr, nullptr);
                 e = hipGetLastError();
                 if (e == hipSuccess && k1 == jb.spp) {
-                    e = hipEventRecord(ev_job[g], s);
+                    e = hipEventRecord(ev_job[g], sg);
                     --pending;
                 }
                 gathered[g] = k1;
             }
             if (k1 < jb.spp) lo = std::min(lo, d.first + (uint64_t)k1 * npx);
         }
-        if (e == hipSuccess && pending > 0) {
-            // the gathers above run before the next steps (same stream), so their
-            // ring slots may be reused from here on
-            *h_limit = lo + ring;
-            e = hipMemcpyAsync(blob + o_lim, h_limit, 8, hipMemcpyHostToDevice, s);
-        }
+        if (e == hipSuccess) e = hipEventRecord(ev_gath[cur], sg);
+        lo_issued = lo;
+        gath_pending = true;
+        cur ^= 1;
         if (e == hipSuccess && pairs > max_pairs) e = hipErrorLaunchTimeOut;
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(sg);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (uint32_t g = 0; g < njobs && e == hipSuccess && job_ms; ++g) {
         float ms = 0.0f;
@@ -2624,16 +2657,26 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
         (void)hipEventRecord(ev_ctx, s);
         (void)hipStreamWaitEvent(ctx->stream, ev_ctx, 0);
     }
+    if (ev_ctx && sg) {
+        (void)hipEventRecord(ev_ctx, sg);
+        (void)hipStreamWaitEvent(ctx->stream, ev_ctx, 0);
+    }
     if (s) (void)hipStreamSynchronize(s);
+    if (sg) (void)hipStreamSynchronize(sg);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t ev : ev_job)
         if (ev) (void)hipEventDestroy(ev);
     if (ev_start) (void)hipEventDestroy(ev_start);
-    if (ev_poll) (void)hipEventDestroy(ev_poll);
+    for (int k = 0; k < 2; ++k) {
+        if (ev_poll[k]) (void)hipEventDestroy(ev_poll[k]);
+        if (ev_gath[k]) (void)hipEventDestroy(ev_gath[k]);
+    }
     if (ev_ctx) (void)hipEventDestroy(ev_ctx);
     if (s) (void)hipStreamDestroy(s);
-    if (h_done) (void)hipHostFree(h_done);
+    if (sg) (void)hipStreamDestroy(sg);
+    for (int k = 0; k < 2; ++k)
+        if (h_done[k]) (void)hipHostFree(h_done[k]);
     if (h_limit) (void)hipHostFree(h_limit);
     release_sort(ss);
     release_render_buffers(rb);
