@@ -127,10 +127,33 @@ def test_edge_case_rays(gpu_ctx):
     pts[: n // 8, 0] = 0.0
     pts[: n // 8, 2] = 0.0
     dirs[-10:] = 0.0
+    # negative zeros (1/d = -inf) on the axis-aligned rays' other axes
+    neg = dirs[n // 2:n // 2 + 3000]
+    neg[neg == 0.0] = -0.0
     rays6 = np.concatenate([pts, dirs], 1)
-    g = _gpu_hits(stage, _pad8(rays6))
-    r = o.intersect(rays6)
+    # Waves mix rays the kernels test with the interval form of the box test
+    # (finite nonzero 1/d) and rays that need the staged form (infinite 1/d):
+    # hits and the reference-order work counts must both match.
+    g = _gpu_hits(stage, _pad8(rays6), counts=True)
+    r = o.intersect(rays6, counts=True)
     _assert_same_hits(g, r)
+    assert np.array_equal(g["counts"], r["counts"])
+
+
+def test_scene_with_unbounded_boxes_uses_the_staged_box_test(gpu_ctx):
+    """A sphere whose box overflows to infinity: that scene's boxes are not all
+    finite, so every ray takes the staged box test; hits and work counts still
+    match the oracle."""
+    spec = scenes.box_scene()
+    spec.add_sphere((3.0e38, 0.5, 0.0), 1.0e38, scenes.disney())
+    print("case unbounded boxes", flush=True)
+    stage = Stage(spec, ctx=gpu_ctx)
+    o = OracleScene(spec)
+    rays = scenes.ray_batch(scenes.box_scene(), 50_000, seed=14)
+    g = _gpu_hits(stage, rays, counts=True)
+    r = o.intersect(scenes.rays6(rays), counts=True)
+    _assert_same_hits(g, r)
+    assert np.array_equal(g["counts"], r["counts"])
 
 
 @pytest.mark.parametrize("name", ["box", "showcase", "ico6", "mixed"])
