@@ -682,10 +682,13 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                         const vec3 le_l = dev::light_world(sh.mats[lp.prim], -wo, lp.f);
                         const float cos_l = fabsf(dot(lp.f.gn, wo));
                         if (!dev::is_zero(le_l) && cos_l > 0.0f && d2 > 0.0f) {
-                            const vec3 fr = dev::f_world(m, wi, wo, f, true);
+                            // f_world and pdf_world in one pass over the lobes (the same
+                            // values; the reverse density it could give is not used)
+                            const dev::FPdf fp = dev::f_pdf_world(m, wi, wo, f, true);
+                            const vec3 fr = fp.f;
                             if (!dev::is_zero(fr)) {
                                 const float pdf_light = lp.point_pdf * d2 / cos_l;
-                                const float pdf_bsdf = dev::pdf_world(m, wi, wo, f);
+                                const float pdf_bsdf = fp.pdf_fwd;
                                 const float w = power_heuristic(pdf_light, pdf_bsdf);
                                 c_nee = mul(mul(T, fr), le_l) * (fabsf(dot(f.n, wo)) * w / pdf_light);
                                 seg_to = lp.p;
