@@ -182,6 +182,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8, 8))) v
         const uint32_t m = __builtin_amdgcn_readfirstlane(*n_ptr);
         n = m < n ? m : n;
     }
+    // The grid is sized for the queue's capacity; when fewer rays are queued,
+    // waves beyond one per 64 rays have nothing to take.  They leave at once
+    // instead of each draining every range's work counter with an atomic.
+    if ((uint64_t)blockIdx.x * kWave >= n) return;
     // A chunk no larger than a quarter of each wave's even share, so that a
     // short batch still spreads over every wave of the grid.
     {
