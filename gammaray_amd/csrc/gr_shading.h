@@ -421,6 +421,50 @@ __device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool
     }
 }
 
+// lobe_f(l, i, o, c2l) (when want_f) and lobe_pdf(l, i, o) in one evaluation,
+// the values of the two calls operation for operation, with the reflection
+// lobes' half vector, microfacet density and incoming masking term computed
+// once.
+__device__ __forceinline__ void lobe_f_pdf(const GpuLobe& l, vec3 i, vec3 o, bool c2l, bool want_f, vec3* f,
+                                           float* pdf) {
+    switch (l.type) {
+        case kLobeDisneySpecular:
+        case kLobeClearcoat: {
+            *f = mk(0.0f, 0.0f, 0.0f);
+            *pdf = 0.0f;
+            const bool sh = same_hemi(i, o);
+            if (!sh && !want_f) return;
+            const vec3 hu = i + o;
+            const vec3 h = normalized(hu);
+            const float d = refl_d(l, h);
+            const bool cc = l.type == kLobeClearcoat;
+            const float lam_i = cc ? gtr1_lambda(i) : ggx_lambda(l.v[0], l.v[1], i);
+            if (want_f) {
+                const float ci = fabsf(i.z), co = fabsf(o.z);
+                if (!(is_zero(hu) || ci == 0.0f || co == 0.0f)) {
+                    const float lam_o = cc ? gtr1_lambda(o) : ggx_lambda(l.v[0], l.v[1], o);
+                    const vec3 fr = refl_fresnel(l, dot(o, h));
+                    const float g = 1.0f / (1.0f + lam_i + lam_o);
+                    *f = mul(refl_colour(l), fr) * (d * g / (4.0f * co * ci));
+                }
+            }
+            if (sh) {
+                if (cc) {
+                    *pdf = d / (4.0f * dot(i, h));
+                } else {
+                    const float g1 = i.z == 0.0f ? 0.0f : d * (1.0f / (1.0f + lam_i)) * fabsf(dot(i, h)) / fabsf(i.z);
+                    *pdf = g1 / (4.0f * dot(i, h));
+                }
+            }
+            return;
+        }
+        default:
+            *f = want_f ? lobe_f(l, i, o, c2l) : mk(0.0f, 0.0f, 0.0f);
+            *pdf = lobe_pdf(l, i, o);
+            return;
+    }
+}
+
 // ---- material (src/material/material.rs) ------------------------------------
 __device__ __forceinline__ vec3 light_world(const GpuMaterial& m, vec3 wi, const Frame& f) {
     if (!m.has_light) return mk(0.0f, 0.0f, 0.0f);
@@ -565,18 +609,20 @@ __device__ __forceinline__ MatSample sample_world(const GpuMaterial& m, vec3 wi,
     vec3 radiance = s.result;
     float pdf = s.pdf;
     if (!(kind & kLobeSpecular)) {
-        for (uint32_t k = 0; k < n; ++k)
-            if (k != r) pdf += lobe_pdf(m.lobes[k], il, s.outgoing);
-    }
-    pdf /= (float)n;
-    if (!(kind & kLobeSpecular)) {
         const bool refl = (dot(wi, f.gn) * dot(ow, f.gn)) > 0.0f;
         // the sampled lobe's kind gates the other lobes' f (material.rs:203-210)
         const bool take = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
-        if (take)
-            for (uint32_t k = 0; k < n; ++k)
-                if (k != r) radiance = radiance + lobe_f(m.lobes[k], il, s.outgoing, c2l);
+        // the other lobes' densities and (when taken) values, each summed in lobe order
+        for (uint32_t k = 0; k < n; ++k) {
+            if (k == r) continue;
+            vec3 fk;
+            float pk;
+            lobe_f_pdf(m.lobes[k], il, s.outgoing, c2l, take, &fk, &pk);
+            pdf += pk;
+            if (take) radiance = radiance + fk;
+        }
     }
+    pdf /= (float)n;
     if (pdf == 0.0f) {
         out.radiance = mk(0.0f, 0.0f, 0.0f);
         out.outgoing = ow;
