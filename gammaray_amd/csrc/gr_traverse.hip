@@ -511,12 +511,17 @@ hipError_t launch_lds(const TraceSetup& ts, const void* in, uint32_t n, const ui
     int per_cu = occupancy<MODE, COUNT, LDS, MULTI>();
     if (ts.tu.waves_per_cu > 0 && (int)ts.tu.waves_per_cu < per_cu) per_cu = (int)ts.tu.waves_per_cu;
     const uint32_t grid = (uint32_t)(num_cus() * per_cu);
-    const uint32_t needed = (n + kWave - 1) / kWave;
+    const uint32_t needed = (uint32_t)(((uint64_t)n + kWave - 1) / kWave);
     const uint32_t blocks = needed < grid ? needed : grid;
     const uint32_t want = MODE == kClosest ? ts.tu.refill : ts.tu.vis_refill;
     const uint32_t refill = want < 1 ? 1u : (want > 64 ? 64u : want);
     const uint32_t parts = ts.tu.xcd_ranges ? kRayCounters : 1u;
-    const uint32_t chunk = ts.tu.ray_chunk < 1 ? 1u : ts.tu.ray_chunk;
+    // A range's 32-bit work counter ends below len + (waves x chunk): each wave
+    // adds at most one chunk past the end before it moves on, so the chunk is
+    // capped to keep that below 2^32 for the largest batches.
+    uint32_t chunk = ts.tu.ray_chunk < 1 ? 1u : ts.tu.ray_chunk;
+    const uint64_t room = blocks ? (0xffffffffull - n) / blocks : 0xffffffffull;
+    if (chunk > room) chunk = room > 0 ? (uint32_t)room : 1u;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, ts.sv.next_ray);
     hipEvent_t t0 = ts.timing ? timing_begin(ts.timing, s) : nullptr;
     hipLaunchKernelGGL((k_trace<MODE, COUNT, LDS, MULTI>), dim3(blocks), dim3(kWave), 0, s, ts.sv, ts.scenes,
