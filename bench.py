@@ -420,7 +420,11 @@ GUIDE_SWEEP = [("alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1)),
                ("alpha 0.7, first vertex", dict(alpha=0.7, guided_depth=1)),
                ("alpha 0.3, two vertices", dict(alpha=0.3, guided_depth=2)),
                ("alpha 0.5, three vertices", dict(alpha=0.5, guided_depth=3)),
-               ("alpha 0.3, first vertex, 16 Adam steps per batch", dict(alpha=0.3, guided_depth=1, steps_per_pass=16))]
+               ("alpha 0.3, first vertex, 16 Adam steps per batch", dict(alpha=0.3, guided_depth=1, steps_per_pass=16)),
+               ("all lobes, alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1, lobes="all")),
+               ("all lobes, alpha 0.5, first vertex", dict(alpha=0.5, guided_depth=1, lobes="all")),
+               ("all lobes, alpha 0.3, two vertices", dict(alpha=0.3, guided_depth=2, lobes="all")),
+               ("all lobes, alpha 0.3, all vertices", dict(alpha=0.3, guided_depth=0, lobes="all"))]
 
 
 def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128):

@@ -312,10 +312,13 @@ class Guide:
     """The learned path guide (an MLP on the matrix cores, trained online)."""
 
     def __init__(self, ctx, seed=1, learning_rate=5e-3, alpha=0.5, batch=16384, steps_per_pass=4,
-                 train_paths=1 << 16, guided_depth=0, warmup_steps=8):
+                 train_paths=1 << 16, guided_depth=0, warmup_steps=8, lobes="consistent"):
         self.ctx = ctx
+        sets = {"consistent": _lib.GR_GUIDE_LOBES_CONSISTENT, "all": _lib.GR_GUIDE_LOBES_ALL}
+        if lobes not in sets:
+            raise ValueError("guided lobe set must be one of %s" % sorted(sets))
         p = _lib.GuideParams(seed, learning_rate, alpha, batch, steps_per_pass, train_paths, guided_depth,
-                             warmup_steps, 0)
+                             warmup_steps, sets[lobes])
         self._h = ctypes.c_void_p()
         check(lib().gr_guide_create(ctx.handle, ctypes.byref(p), ctypes.byref(self._h)), ctx.handle,
               "guide_create")

@@ -859,6 +859,8 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
     if (prm->batch == 0 || prm->batch % 16 != 0 || prm->batch > (1u << 22))
         return gr::set_error(ctx, GR_ERR_INVALID, "guide batch must be a positive multiple of 16 (<= 4M)");
     if (!(prm->alpha >= 0.0f && prm->alpha < 1.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "alpha must be in [0, 1)");
+    if (prm->lobes != GR_GUIDE_LOBES_CONSISTENT && prm->lobes != GR_GUIDE_LOBES_ALL)
+        return gr::set_error(ctx, GR_ERR_INVALID, "unknown guided lobe set");
     if (!(prm->learning_rate > 0.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "learning rate must be positive");
     gr_guide* g = new (std::nothrow) gr_guide();
     if (!g) return gr::set_error(ctx, GR_ERR_NOMEM, "out of host memory");

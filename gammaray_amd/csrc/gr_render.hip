@@ -468,6 +468,7 @@ struct GuideArgs {
     float* recs;       // compacted training records
     uint32_t* rec_count;
     uint32_t rec_cap, train_stride, train_paths, drec, guided_depth, active;
+    uint32_t lobes;  // GR_GUIDE_LOBES_*: the lobe picks the guide may redirect
     float alpha;
     float lo[3], inv_ext[3];
 };
@@ -501,20 +502,78 @@ __device__ __forceinline__ bool bad_sample(const dev::MatSample& smp) {
     return !(finite3(smp.radiance) && isfinite(smp.pdf) && smp.pdf > 0.0f);
 }
 
-// Lobes whose reported pdf is the density their sampling draws from: the
-// guide is mixed with these only.  The clearcoat pdf lacks the cos(theta_h)
-// Jacobian factor (src/material/util.rs:266-308) and the transmission pdf
-// uses an unnormalised half vector (src/material/lobes.rs:337-339); those
-// lobes, and the delta mirror, keep the reference's own sampling.
+// Lobes whose reported pdf is the density their sampling draws from.  The
+// clearcoat pdf lacks the cos(theta_h) Jacobian factor (src/material/util.rs:
+// 266-308) and the transmission pdf uses an unnormalised half vector
+// (src/material/lobes.rs:337-339); the delta mirror has no density.
 __device__ __forceinline__ bool lobe_consistent(const GpuLobe& l) {
     return l.type == kLobeDisneyDiffuse || l.type == kLobeDisneySpecular ||
            l.type == kLobePerfectDiffuse;
 }
 
-__device__ __forceinline__ uint32_t consistent_lobes(const GpuMaterial& m) {
+// Whether a pick of lobe l may be redirected by the guide, for the guide's
+// lobe set (GR_GUIDE_LOBES_*): the consistent lobes, or every lobe with a
+// density (all but the mirror).
+__device__ __forceinline__ bool lobe_guided(const GpuLobe& l, uint32_t set) {
+    return set == GR_GUIDE_LOBES_ALL ? l.type != kLobePerfectMirror : lobe_consistent(l);
+}
+
+__device__ __forceinline__ uint32_t guided_lobes(const GpuMaterial& m, uint32_t set) {
     uint32_t c = 0;
-    for (uint32_t k = 0; k < m.nlobes; ++k) c += lobe_consistent(m.lobes[k]) ? 1u : 0u;
+    for (uint32_t k = 0; k < m.nlobes; ++k) c += lobe_guided(m.lobes[k], set) ? 1u : 0u;
     return c;
+}
+
+// Weight of the guided estimator at direction wo for a pick in the guided
+// lobe set G (|G| of the material's n lobes), and the reported BSDF pdf there.
+//
+// sample_world's draw of lobe k at wo reports radiance F_k(wo) and pdf P(wo):
+// P is the mean of the n reported lobe pdfs, and F_k is lobe k's f plus, when
+// k's kind matches the reflect/transmit side of wo, every other lobe's f
+// (src/material/material.rs:177-211).  With s_k the density lobe k's draws
+// realise (lobe_density), the picks in G contribute
+//   (1/n) sum_{k in G} int s_k F_k cos / P dwo.
+// The guided vertex enters G with the reference's own pick (probability
+// |G|/n) and draws wo from D = alpha q + (1 - alpha) S_G / |G| (q: the guide's
+// mixture, S_G: the sum of s_k over G), so the weight
+//   sum_{k in G} s_k F_k cos / (P |G| D)
+// has the same mean.  A pick whose lobe draw failed (wo zero) weighs 0, as in
+// sample_world.
+struct GuidedWeight {
+    vec3 g;
+    float pdf;  // P(wo), for the next vertex's emission MIS weight
+};
+__device__ __forceinline__ GuidedWeight guided_weight(const GpuMaterial& m, vec3 wi, vec3 wo, const Frame& f,
+                                                      uint32_t set, float alpha, float q) {
+    GuidedWeight r;
+    r.g = mk(0.0f, 0.0f, 0.0f);
+    r.pdf = 0.0f;
+    if (dev::is_zero(wo)) return r;
+    const vec3 il = dev::to_local(wi, f), ol = dev::to_local(wo, f);
+    const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
+    vec3 f_sum = mk(0.0f, 0.0f, 0.0f), own = f_sum;  // own: sum over G of s_k f_k where F_k = f_k
+    float s_all = 0.0f, s_g = 0.0f, s_take = 0.0f;    // s_take: sum over G of s_k where F_k = f_sum
+    uint32_t ng = 0;
+    for (uint32_t k = 0; k < m.nlobes; ++k) {
+        const GpuLobe& l = m.lobes[k];
+        const vec3 fk = dev::lobe_f(l, il, ol, true);
+        f_sum = f_sum + fk;
+        if (il.z != 0.0f) s_all += dev::lobe_pdf(l, il, ol);
+        if (!lobe_guided(l, set)) continue;
+        ++ng;
+        const float sk = dev::lobe_density(l, il, ol);
+        s_g += sk;
+        const uint32_t kind = dev::lobe_kind(l);
+        if ((refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission)))
+            s_take += sk;
+        else
+            own = own + fk * sk;
+    }
+    r.pdf = s_all / (float)m.nlobes;
+    const float dens = alpha * q + (1.0f - alpha) * (s_g / (float)ng);
+    if (r.pdf > 0.0f && dens > 0.0f)
+        r.g = (f_sum * s_take + own) * (fabsf(dot(f.n, wo)) / (r.pdf * ((float)ng * dens)));
+    return r;
 }
 
 // Vertex-record slot of path p at this depth, or kNone when p records no
@@ -611,16 +670,14 @@ __global__ __launch_bounds__(256) void k_display(RenderArgs a) {
 // with NEE, light sampling and power-heuristic MIS.
 //
 // With GUIDED, the reference's BSDF estimator -- pick one of the n lobes
-// uniformly, sample it, weight f / ((1/n) sum_k pdf_k) -- is split by the
-// picked lobe.  A pick among the nc consistent lobes (see lobe_consistent)
-// estimates the integrand f * S_C / S_all (S_C, S_all: sums of the consistent
-// and of all reported lobe pdfs); there the direction comes from the guide's
-// mixture q with probability alpha and from the picked lobe otherwise, weighted
-// by the one-sample MIS density (nc/n)(alpha q + (1 - alpha) S_C / nc).  Any
-// other pick is the reference's sample as is.  Both parts keep their means, so
-// the guided estimator converges to the plain estimator's image, including the
-// reference's pdf quirks; the NEE and emission MIS weights use the reported
-// BSDF pdf exactly as without the guide.
+// uniformly, sample it, weight F / ((1/n) sum_k pdf_k) -- is split by the
+// picked lobe.  A pick in the guide's lobe set G (lobe_guided) takes its
+// direction from the guide's mixture q with probability alpha and from the
+// picked lobe otherwise, weighted as guided_weight derives so that the picks
+// in G keep the mean they have in the reference's estimator; any other pick
+// is the reference's sample as is.  So the guided estimator converges to the
+// plain estimator's image, including the reference's pdf quirks; the NEE and
+// emission MIS weights use the reported BSDF pdf exactly as without the guide.
 struct ShadeOut {
     bool alive, shadow;
     vec3 new_o, new_d, seg_to, c_nee, hp;
@@ -700,7 +757,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                     vec3 out_dir;
                     // the lobe the reference's sample_world picks with this draw
                     const uint32_t pick = (uint32_t)(((uint64_t)r0.x * m.nlobes) >> 32);
-                    const bool split = GUIDED && gs != kNone && lobe_consistent(m.lobes[pick]);
+                    const bool split = GUIDED && gs != kNone && lobe_guided(m.lobes[pick], a.gd.lobes);
                     if (!split) {
                         const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
                         note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
@@ -714,29 +771,11 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                         } else {
                             out_dir = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z)).outgoing;
                         }
-                        const vec3 il = dev::to_local(wi, f), ol = dev::to_local(out_dir, f);
-                        float s_c = 0.0f, s_all = 0.0f;
-                        uint32_t nc = 0;
-                        if (il.z != 0.0f) {
-                            for (uint32_t k = 0; k < m.nlobes; ++k) {
-                                const float pk = dev::lobe_pdf(m.lobes[k], il, ol);
-                                s_all += pk;
-                                if (lobe_consistent(m.lobes[k])) {
-                                    s_c += pk;
-                                    ++nc;
-                                }
-                            }
-                        } else {
-                            nc = consistent_lobes(m);
-                        }
                         const float q = mix ? mixture_pdf(mix, out_dir) : 0.0f;
-                        const float dens = ((float)nc / (float)m.nlobes) * (alpha * q + (1.0f - alpha) * (s_c / (float)nc));
-                        vec3 g = mk(0.0f, 0.0f, 0.0f);
-                        if (s_all > 0.0f && dens > 0.0f)
-                            g = dev::f_world(m, wi, out_dir, f, true) * ((s_c / s_all) * fabsf(dot(f.n, out_dir)) / dens);
-                        note_stat(a.stats, kStatNonfiniteBsdf, !finite3(g));
-                        T = mul(T, g);
-                        a.rb.aux[p] = make_float4(s_all / (float)m.nlobes, 0.0f, 0.0f, 0.0f);
+                        const GuidedWeight gw = guided_weight(m, wi, out_dir, f, a.gd.lobes, alpha, q);
+                        note_stat(a.stats, kStatNonfiniteBsdf, !finite3(gw.g));
+                        T = mul(T, gw.g);
+                        a.rb.aux[p] = make_float4(gw.pdf, 0.0f, 0.0f, 0.0f);
                     }
                     if (split) {
                         // training record: the direction and the radiance so far (this
@@ -841,7 +880,7 @@ __global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t dept
         const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
         if (h.prim != kNone && (a.gd.guided_depth == 0 || depth < a.gd.guided_depth)) {
             const GpuMaterial& m = a.sh.mats[h.prim];
-            if (consistent_lobes(m) != 0) {
+            if (guided_lobes(m, a.gd.lobes) != 0) {
                 eligible = true;
                 const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
                 const Frame f = hit_frame(a.sv, a.sh, h, o, d);
@@ -1092,7 +1131,8 @@ __global__ void k_material_query(ShadeView sh, uint32_t prim, const float* in, u
     r[10] = s.radiance.x; r[11] = s.radiance.y; r[12] = s.radiance.z;
     r[13] = s.outgoing.x; r[14] = s.outgoing.y; r[15] = s.outgoing.z;
     r[16] = s.pdf; r[17] = (float)s.kind;
-    r[18] = (float)dev::count_lobes(m, kLobeDiffuse | kLobeGlossy); r[19] = 0.0f;
+    r[18] = (float)dev::count_lobes(m, kLobeDiffuse | kLobeGlossy);
+    r[19] = dev::sample_density_world(m, wi, wo, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -2025,6 +2065,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         a.gd.train_paths = gp.train_paths;
         a.gd.drec = drec;
         a.gd.guided_depth = gp.guided_depth;
+        a.gd.lobes = gp.lobes;
         a.gd.alpha = gp.alpha;
         for (int k = 0; k < 3; ++k) {
             const float ext = sc->view.root_hi[k] - sc->view.root_lo[k];

@@ -373,6 +373,52 @@ __device__ __forceinline__ float lobe_pdf(const GpuLobe& l, vec3 i, vec3 o) {
     }
 }
 
+// Density over outgoing directions o of lobe_sample's draws from i: a
+// sub-probability, since draws that fail (reflected below the surface, total
+// internal reflection) return no direction.
+//  * Cosine-hemisphere (diffuse) and visible-normal GGX reflection draws: the
+//    reported pdf.
+//  * GTR1: the sampler sets cos^2 theta_h = 1 - (a^2)^(1-u) / (1 - a^2), so
+//    with t = cos^2 theta_h, dt/du = (1 - t) ln(a^2): the half vector has
+//    density |cos theta_h| / (pi |ln a^2| sin^2 theta_h) where
+//    sin^2 theta_h >= a^2 / (1 - a^2) (draws nearer the normal do not occur;
+//    the clamped ones lie on the horizon and reflect below), times the
+//    reflection Jacobian 1 / (4 i.h).  The reported pdf is D(h) / (4 i.h).
+//  * Transmission: refraction through a visible-normal GGX half vector h
+//    with i.h > 0 gives o with o.h < 0 and i + eta o parallel to h (Snell),
+//    so the density is the Walter et al. / PBRT pdf with h the normalised
+//    i + eta o oriented to i.h > 0, zero unless o.h < 0 and h lies in i's
+//    hemisphere -- also for o in i's own hemisphere, which grazing draws
+//    reach.  The reported pdf leaves h unnormalised when its z > 0 and is
+//    zero in i's hemisphere.
+//  * The mirror: a delta, no density.
+__device__ __forceinline__ float lobe_density(const GpuLobe& l, vec3 i, vec3 o) {
+    switch (l.type) {
+        case kLobeClearcoat: {
+            if (!same_hemi(i, o)) return 0.0f;
+            const vec3 h = normalized(i + o);
+            const float a2 = l.v[0] * l.v[0];
+            const float s2 = sin2_theta(h);
+            if (!(s2 >= a2 / (1.0f - a2))) return 0.0f;
+            return fabsf(h.z) / ((kPi * fabsf(logf(a2)) * s2) * (4.0f * dot(i, h)));
+        }
+        case kLobeSpecularTrans: {
+            if (i.z == 0.0f) return 0.0f;
+            const float eta = i.z > 0.0f ? l.v[2] : 1.0f / l.v[2];
+            const vec3 hu = i + o * eta;
+            if (is_zero(hu)) return 0.0f;
+            vec3 h = normalized(hu);
+            if (dot(i, h) < 0.0f) h = -h;
+            const float ih = dot(i, h), oh = dot(o, h);
+            if (!(ih > 0.0f && oh < 0.0f && same_hemi(h, i))) return 0.0f;
+            const float sd = ih + eta * oh;
+            return ggx_pdf(l.v[0], l.v[1], i, h) * fabsf((eta * eta * oh) / (sd * sd));
+        }
+        case kLobePerfectMirror: return 0.0f;
+        default: return lobe_pdf(l, i, o);
+    }
+}
+
 struct LobeSample {
     vec3 result, outgoing;
     float pdf;
@@ -489,6 +535,16 @@ __device__ __forceinline__ float pdf_world(const GpuMaterial& m, vec3 wi, vec3 w
     if (il.z == 0.0f) return 0.0f;
     float p = 0.0f;
     for (uint32_t k = 0; k < m.nlobes; ++k) p += lobe_pdf(m.lobes[k], il, ol);
+    return p / (float)m.nlobes;
+}
+
+// Density over world directions of sample_world's draws: the mean over the
+// lobes of lobe_density (a mirror lobe's draws are an atom, not counted).
+__device__ __forceinline__ float sample_density_world(const GpuMaterial& m, vec3 wi, vec3 wo, const Frame& f) {
+    if (m.nlobes == 0) return 0.0f;
+    const vec3 il = to_local(wi, f), ol = to_local(wo, f);
+    float p = 0.0f;
+    for (uint32_t k = 0; k < m.nlobes; ++k) p += lobe_density(m.lobes[k], il, ol);
     return p / (float)m.nlobes;
 }
 

@@ -326,9 +326,11 @@ int gr_render_jobs(gr_ctx *ctx, const gr_job *jobs, uint32_t njobs, const gr_bat
  * A 5-layer MLP (fp16 weights, fp32 accumulation on the matrix cores) maps a
  * surface vertex (position, outgoing direction, shading normal, roughness) to
  * a mixture of 8 von Mises-Fisher lobes over the incident direction.  During a
- * GR_INTEGRATOR_GUIDED trace, guided vertices draw their direction from the
- * mixture with probability alpha and from the BSDF otherwise, weighted by the
- * one-sample MIS density alpha*q + (1-alpha)*p_bsdf (unbiased for any network);
+ * GR_INTEGRATOR_GUIDED trace, a guided vertex whose lobe pick falls in the
+ * guided lobe set draws its direction from the mixture with probability alpha
+ * and from the picked lobe otherwise, weighted by the one-sample MIS density
+ * alpha*q + (1-alpha)*(mean of the set's lobe sampling densities), so the
+ * render keeps the plain PT+NEE mean for any network;
  * after each traced batch the network takes Adam steps on that batch's
  * (vertex, direction, f*cos*Li/pdf) records, minimising the KL divergence from
  * the product of BSDF and incident radiance to the mixture. */
@@ -341,8 +343,16 @@ typedef struct {
     uint32_t train_paths;     /* paths per traced batch that record training data */
     uint32_t guided_depth;    /* vertices at depth < guided_depth are guided; 0 = all */
     uint32_t warmup_steps;    /* Adam steps taken before the guide is sampled */
-    uint32_t reserved;
+    uint32_t lobes;           /* GR_GUIDE_LOBES_*: the lobe picks the guide may redirect */
 } gr_guide_params;
+/* Guided lobe sets.  CONSISTENT: Disney diffuse, GGX reflection and perfect
+ * diffuse, whose reported pdfs are their sampling densities.  ALL: every lobe
+ * but the mirror (a delta), clearcoat and transmission included, weighted with
+ * the densities their samplers realise (the reported pdfs of those two are
+ * not: the GTR1 pdf lacks cos(theta_h), the transmission pdf leaves its half
+ * vector unnormalised). */
+#define GR_GUIDE_LOBES_CONSISTENT 0u
+#define GR_GUIDE_LOBES_ALL 1u
 #define GR_GUIDE_PARAMS 27936  /* fp32 parameters: W1[208][64] b1 W2[64][64] b2 W3 b3 W4 b4 W5[64][32] b5 */
 #define GR_GUIDE_MIX_FLOATS 40 /* per query: 8 lobes x (weight, kappa, mean x, y, z) */
 #define GR_GUIDE_RECORD_FLOATS 12 /* training record: 8 raw inputs, direction (3), value */
@@ -373,7 +383,9 @@ int gr_guide_mixture_pdf(gr_ctx *ctx, const float *d_mix, const float *d_dirs, u
  * floats in d_in -- wi(3), wo(3), normal, tangent, binormal, geom_normal (12),
  * u_lobe (uint32 bits), u1, u2, camera_to_light, pad(2); 20 floats out per item
  * -- f_world(3), pdf_world, light_world(3), sample_world emission(3),
- * radiance(3), outgoing(3), pdf, kind, count_lobes(DIFFUSE|GLOSSY), pad. */
+ * radiance(3), outgoing(3), pdf, kind, count_lobes(DIFFUSE|GLOSSY), and the
+ * density of sample_world's draws at wo (the mean over the lobes of the
+ * density each lobe's sampler realises; the mirror contributes none). */
 int gr_material_query(gr_scene *scene, uint32_t prim, const float *d_in, uint32_t n, float *d_out);
 
 #ifdef __cplusplus

@@ -368,6 +368,63 @@ orc_lobe_sample orc_lobe_sample_f(const orc_lobe *l, ovec i, int c2l, float u1, 
     }
 }
 
+/* Density over outgoing directions of orc_lobe_sample_f's draws from i (no
+ * counterpart in the reference; the guided estimator's weights need it).  A
+ * sub-probability: draws that fail (reflected below the surface, total
+ * internal reflection) return no direction.
+ *  - cosine-hemisphere and visible-normal GGX reflection draws: the reported
+ *    pdf (orc_lobe_pdf);
+ *  - GTR1 (gtr1_sample_half): t = cos^2 theta_h = 1 - (a^2)^(1-u2) / (1 - a^2)
+ *    has dt/du2 = (1 - t) ln(a^2), so over the half-vector hemisphere the
+ *    density is |cos theta_h| / (pi |ln a^2| sin^2 theta_h) for
+ *    sin^2 theta_h >= a^2 / (1 - a^2) and zero nearer the normal; reflection
+ *    divides by 4 i.h;
+ *  - transmission: refract(i, h, eta') with i.h > 0 gives o.h < 0 and
+ *    i + eta o parallel to h (eta = 1 / eta'), so h = +-normalize(i + eta o)
+ *    with the sign that makes i.h > 0, valid when o.h < 0 and h lies in i's
+ *    hemisphere; the density is the visible-normal density of h times
+ *    eta^2 |o.h| / (i.h + eta o.h)^2;
+ *  - the mirror is a delta: no density. */
+float orc_lobe_density(const orc_lobe *l, ovec i, ovec o)
+{
+    switch (l->type) {
+    case OL_CLEARCOAT: {
+        if (!v_same_hemisphere(i, o)) return 0.0f;
+        ovec h = v_normalized(v_add(i, o));
+        float a2 = l->alpha * l->alpha;
+        float sin2 = fmaxf(0.0f, 1.0f - h.z * h.z);
+        if (!(sin2 >= a2 / (1.0f - a2))) return 0.0f;
+        return fabsf(h.z) / ((ORC_PI * fabsf(logf(a2)) * sin2) * (4.0f * v_dot(i, h)));
+    }
+    case OL_TRANSMISSION: {
+        if (i.z == 0.0f) return 0.0f;
+        float eta = i.z > 0.0f ? l->ior : 1.0f / l->ior;
+        ovec hu = v_add(i, v_scale(o, eta));
+        if (v_is_exactly_zero(hu)) return 0.0f;
+        ovec h = v_normalized(hu);
+        if (v_dot(i, h) < 0.0f) h = v_neg(h);
+        float ih = v_dot(i, h), oh = v_dot(o, h);
+        if (!(ih > 0.0f && oh < 0.0f && v_same_hemisphere(h, i))) return 0.0f;
+        float sd = ih + eta * oh;
+        return ggx_pdf(l, i, h) * fabsf((eta * eta * oh) / (sd * sd));
+    }
+    case OL_PERFECT_MIRROR: return 0.0f;
+    default: return orc_lobe_pdf(l, i, o);
+    }
+}
+
+/* Density over world directions of orc_sample_world's draws: the mean of the
+ * lobes' orc_lobe_density. */
+float orc_sample_density_world(const orc_material *m, ovec wi, ovec wo, const orc_frame *f)
+{
+    if (m->nlobes == 0) return 0.0f;
+    ovec il = v_world_to_local(wi, f->tangent, f->binormal, f->normal);
+    ovec ol = v_world_to_local(wo, f->tangent, f->binormal, f->normal);
+    float p = 0.0f;
+    for (int k = 0; k < m->nlobes; ++k) p += orc_lobe_density(&m->lobes[k], il, ol);
+    return p / (float)m->nlobes;
+}
+
 /* ---- material, src/material/material.rs ---------------------------------- */
 
 static ovec to_local(ovec v, const orc_frame *f)

@@ -926,7 +926,7 @@ size_t orc_exr_encode(const float *rgbw, uint32_t width, uint32_t height, uint8_
  * wi(3), wo(3), frame normal/tangent/binormal/geom_normal (12), u_lobe (u32
  * bits), u1, u2, camera_to_light, pad(2).  out: 20 floats per item --
  * f_world(3), pdf_world, light_world(3), sample emission(3), radiance(3),
- * outgoing(3), pdf, kind, count_lobes(DIFFUSE|GLOSSY), pad. */
+ * outgoing(3), pdf, kind, count_lobes(DIFFUSE|GLOSSY), density of the draws at wo. */
 void orc_material_query(const orc_scene *s, uint32_t prim, int64_t n, const float *in, float *out)
 {
     const orc_material *m = &s->prims[prim].mat;
@@ -952,6 +952,7 @@ void orc_material_query(const orc_scene *s, uint32_t prim, int64_t n, const floa
         r[10] = sm.radiance.x; r[11] = sm.radiance.y; r[12] = sm.radiance.z;
         r[13] = sm.outgoing.x; r[14] = sm.outgoing.y; r[15] = sm.outgoing.z;
         r[16] = sm.pdf; r[17] = (float)sm.kind;
-        r[18] = (float)orc_count_lobes(m, LOBE_DIFFUSE | LOBE_GLOSSY); r[19] = 0.0f;
+        r[18] = (float)orc_count_lobes(m, LOBE_DIFFUSE | LOBE_GLOSSY);
+        r[19] = orc_sample_density_world(m, wi, wo, &f);
     }
 }

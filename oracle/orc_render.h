@@ -50,10 +50,12 @@ typedef struct {
 int orc_lobe_kind(const orc_lobe *l);
 ovec orc_lobe_f(const orc_lobe *l, ovec i, ovec o, int c2l);
 float orc_lobe_pdf(const orc_lobe *l, ovec i, ovec o);
+float orc_lobe_density(const orc_lobe *l, ovec i, ovec o);
 orc_lobe_sample orc_lobe_sample_f(const orc_lobe *l, ovec i, int c2l, float u1, float u2);
 
 ovec orc_f_world(const orc_material *m, ovec wi, ovec wo, const orc_frame *f, int c2l);
 float orc_pdf_world(const orc_material *m, ovec wi, ovec wo, const orc_frame *f);
+float orc_sample_density_world(const orc_material *m, ovec wi, ovec wo, const orc_frame *f);
 ovec orc_light_world(const orc_material *m, ovec wi, const orc_frame *f);
 orc_mat_sample orc_sample_world(const orc_material *m, ovec wi, const orc_frame *f, int c2l,
                                 uint32_t u_lobe, float u1, float u2);

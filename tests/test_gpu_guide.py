@@ -221,18 +221,23 @@ def _sample_radiance(stage, film, integrator, spp, **kw):
     return buf.cpu().numpy()[:, :3].reshape(spp, H * W, 3)
 
 
-@pytest.mark.parametrize("variant", ["consistent", "clearcoat", "transmission"])
-def test_guided_render_has_the_plain_mean(variant, gpu_ctx):
+@pytest.mark.parametrize("variant,lobes", [("consistent", "consistent"), ("clearcoat", "consistent"),
+                                           ("transmission", "consistent"), ("clearcoat", "all"),
+                                           ("transmission", "all")])
+def test_guided_render_has_the_plain_mean(variant, lobes, gpu_ctx):
     """Guided and plain PT+NEE per-pixel means agree within 3 sigma (different
     seeds), with the guide sampled after warm-up -- also where clearcoat and
-    transmission lobes report pdfs that are not their sampling densities."""
-    print("case guided unbiased", variant, flush=True)
+    transmission lobes report pdfs that are not their sampling densities, and
+    with those lobes' picks guided too (lobes="all": weighted with the
+    densities their samplers realise and the radiance sample_world reports
+    for each pick, the extra lobe values of transmitted picks included)."""
+    print("case guided unbiased", variant, lobes, flush=True)
     spec = {"consistent": lambda: scenes.box_scene(consistent=True),
             "clearcoat": lambda: scenes.box_scene(),
             "transmission": lambda: scenes.box_scene(seed=2, full=True)}[variant]()
     W, H, spp = 32, 32, 256
     stage = Stage(spec, ctx=gpu_ctx)
-    g = Guide(gpu_ctx, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=8, alpha=0.5)
+    g = Guide(gpu_ctx, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=8, alpha=0.5, lobes=lobes)
     film = Film(gpu_ctx, W, H)
     a = _sample_radiance(stage, film, "guided", spp, max_depth=6, seed=31, batch=8, guide=g)
     assert g.steps() == 4 * spp // 8
@@ -240,7 +245,19 @@ def test_guided_render_has_the_plain_mean(variant, gpu_ctx):
     b = _sample_radiance(stage, film2, "pt_nee", spp, max_depth=6, seed=32)
     assert np.isfinite(a).all()
     # iterations traced with the guide sampled
-    assert_three_sigma(a[32:].sum(2), b[32:].sum(2), "guided " + variant)
+    assert_three_sigma(a[32:].sum(2), b[32:].sum(2), "guided %s %s" % (variant, lobes))
+
+
+def test_guide_rejects_an_unknown_lobe_set(gpu_ctx):
+    from gammaray_amd import _lib
+    from gammaray_amd.api import lib
+    import ctypes
+    p = _lib.GuideParams(1, 5e-3, 0.5, 4096, 4, 4096, 0, 8, 2)
+    h = ctypes.c_void_p()
+    assert lib().gr_guide_create(gpu_ctx.handle, ctypes.byref(p), ctypes.byref(h)) != 0
+    assert not h.value
+    with pytest.raises(ValueError):
+        Guide(gpu_ctx, lobes="clearcoat")
 
 
 def test_guided_render_is_bit_reproducible(gpu_ctx):
@@ -300,7 +317,8 @@ def _guide_loss(g, recs):
     return float(-(w * torch.log(q.clamp(min=1e-30))).mean().item()), mix
 
 
-def test_guided_full_1080p_scene(gpu_ctx):
+@pytest.mark.parametrize("lobes", ["consistent", "all"])
+def test_guided_full_1080p_scene(lobes, gpu_ctx):
     """Neural guiding on the full 1080p scene (100k-triangle mesh, 200 spheres,
     PT+NEE depth 8), with the bench's guide settings (alpha 0.3, first vertex,
     8-spp batches, 8 Adam steps per batch).  Checks:
@@ -310,12 +328,12 @@ def test_guided_full_1080p_scene(gpu_ctx):
     * 16 guided spp agree with 16 plain spp (independent seeds) per 16 x 16
       tile within 3 sigma (binomial bound) and in total."""
     import torch
-    print("case guided 1080p", flush=True)
+    print("case guided 1080p", lobes, flush=True)
     spec = scenes.showcase_scene()
     W, H = 1920, 1080
     stage = Stage(spec, ctx=gpu_ctx)
     kw = dict(seed=7, learning_rate=5e-3, alpha=0.3, batch=16384, steps_per_pass=8, train_paths=65536,
-              guided_depth=1, warmup_steps=8)
+              guided_depth=1, warmup_steps=8, lobes=lobes)
     g = Guide(gpu_ctx, **kw)
     fresh = Guide(gpu_ctx, **kw)  # same initial weights, never trained
     warm = Film(gpu_ctx, W, H)
@@ -337,4 +355,4 @@ def test_guided_full_1080p_scene(gpu_ctx):
                          batch=8, guide=g)
     b = _sample_radiance(stage, Film(gpu_ctx, W, H), "pt_nee", 16, max_depth=8, seed=54)
     assert np.isfinite(a).all()
-    assert_three_sigma_blocks(a.sum(2), b.sum(2), W, H, 16, "guided 1080p tiles")
+    assert_three_sigma_blocks(a.sum(2), b.sum(2), W, H, 16, "guided 1080p tiles " + lobes)

@@ -109,9 +109,9 @@ def test_material_functions_match_oracle(gpu_ctx):
         g = material_query(stage, prim, items)
         r = O.material_query(o, prim, items)
         has_cc = spec.prims[prim].material[0] == 0 and spec.prims[prim].material[12] > 0
-        # f_world, pdf_world, light_world and count_lobes: no transcendental
-        # call unless a clearcoat (ln) is present -> bit-exact.
-        cols = [0, 1, 2, 3, 4, 5, 6, 18]
+        # f_world, pdf_world, light_world, count_lobes and the draws' density:
+        # no transcendental call unless a clearcoat (ln) is present -> bit-exact.
+        cols = [0, 1, 2, 3, 4, 5, 6, 18, 19]
         if not has_cc:
             assert np.array_equal(g[:, cols].view(np.uint32), r[:, cols].view(np.uint32)), prim
         else:

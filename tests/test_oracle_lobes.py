@@ -459,3 +459,112 @@ def test_transmission_f_matches_walter():
         np.testing.assert_allclose(q[k, 0:3], want, rtol=tol, atol=1e-7)
         checked += fr < 1.0
     assert checked > 50
+
+
+# ---- the density of the draws ------------------------------------------------
+# The guided estimator weighs a direction by the density the lobes' samplers
+# realise there (material_query column 19).  Pinned against the oracle's own
+# sample_world: the draws of one incoming direction, binned on the sphere
+# (equal-area cells in z and azimuth), are compared by chi-square with the
+# density's integral over each cell (midpoint rule on a finer grid), and the
+# draws that return no direction with one minus the density's total.  A
+# density that used the reported pdfs (GTR1 without cos theta_h, the
+# transmission pdf's unnormalised half vector), Burley's GTR1 precedence,
+# dropped the reflection Jacobian, or left out the transmission draws that
+# land in the incoming hemisphere fails these bins.
+def _density_chi2(o, wi, n_draws=300000, kz=24, kphi=24, sub=24, seed=11):
+    rng = np.random.default_rng(seed)
+    u = np.zeros((n_draws, 3), np.uint32)
+    u[:, 0] = rng.integers(0, 1 << 32, n_draws, dtype=np.uint64).astype(np.uint32)
+    u[:, 1:3] = rng.random((n_draws, 2)).astype(np.float32).view(np.uint32)
+    wi = _unit(wi).astype(np.float32)
+    q = _query(o, np.tile(wi, (n_draws, 1)), np.tile([0, 0, 1.0], (n_draws, 1)), u)
+    d = q[:, 13:16].astype(np.float64)
+    drawn = np.any(d != 0.0, axis=1)
+    z = np.clip(d[drawn, 2], -1.0, 1.0)
+    phi = np.arctan2(d[drawn, 1], d[drawn, 0])
+    iz = np.minimum(((z + 1.0) * 0.5 * kz).astype(int), kz - 1)
+    ip = np.minimum(((phi + math.pi) / (2 * math.pi) * kphi).astype(int), kphi - 1)
+    counts = np.bincount(iz * kphi + ip, minlength=kz * kphi)
+    nz, nphi = kz * sub, kphi * sub
+    zc = -1.0 + (np.arange(nz) + 0.5) * (2.0 / nz)
+    pc = -math.pi + (np.arange(nphi) + 0.5) * (2.0 * math.pi / nphi)
+    zz, pp = np.meshgrid(zc, pc, indexing="ij")
+    ss = np.sqrt(1.0 - zz * zz)
+    dirs = np.stack([ss * np.cos(pp), ss * np.sin(pp), zz], -1).reshape(-1, 3)
+    dens = _query(o, np.tile(wi, (len(dirs), 1)), dirs)[:, 19].astype(np.float64)
+    assert np.all(dens >= 0.0)
+    cell = (2.0 / nz) * (2.0 * math.pi / nphi)
+    mass = dens.reshape(kz, sub, kphi, sub).sum(axis=(1, 3)).ravel() * cell
+    total = mass.sum()
+    assert total <= 1.0 + 2e-3, total
+    exp = np.append(mass, max(0.0, 1.0 - total)) * n_draws
+    obs = np.append(counts, n_draws - drawn.sum()).astype(np.float64)
+    small = exp < 5.0
+    exp = np.append(exp[~small], exp[small].sum())
+    obs = np.append(obs[~small], obs[small].sum())
+    chi = ((obs - exp) ** 2 / np.maximum(exp, 1e-12)).sum()
+    p = stats.chi2.sf(chi, len(exp) - 1)
+    return p, total, drawn.mean(), d[drawn], wi
+
+
+@pytest.mark.parametrize("gloss", [0.0, 0.3])
+@pytest.mark.parametrize("wz", [0.95, 0.3, -0.6])
+def test_clearcoat_draw_density(gloss, wz):
+    o = _clearcoat_only(gloss=gloss)
+    p, total, acc, _, _ = _density_chi2(o, [math.sqrt(1 - wz * wz), 0.0, wz])
+    assert p > 1e-4, (p, total, acc)
+    assert abs(total - acc) < 5e-3, (total, acc)
+
+
+@pytest.mark.parametrize("wz", [0.9, 0.25, -0.45, -0.9])
+@pytest.mark.parametrize("roughness", [0.35, 0.5])
+def test_transmission_draw_density(wz, roughness):
+    """Two lobes, GGX reflection and transmission: the mean of their densities.
+    The GGX sampler inverts the slope distribution with PBRT's rational fit
+    (src/material/util.rs:194-223), so its draws follow the visible-normal
+    density only as closely as that fit; at roughness 0.7 the steepest slopes,
+    which refract to the edge of the transmission cone, already differ by a
+    few hundred in a million draws there, and these bins stay below that."""
+    o = _transmissive(ior=1.5, roughness=roughness)
+    p, total, acc, d, wi = _density_chi2(o, [math.sqrt(1 - wz * wz), 0.0, wz])
+    assert p > 1e-4, (p, total, acc)
+    assert abs(total - acc) < 5e-3, (total, acc)
+
+
+def test_transmission_draws_reach_the_incoming_hemisphere():
+    """Leaving the medium at grazing incidence, o = -eta' i + (eta' i.h - cos t) h
+    (eta' = ior) has o.z = eta' |i.z| - (eta' i.h - cos t) |h.z|, which a tilted
+    h makes negative: some refracted draws land on the incoming side (entering,
+    both terms keep o.z < 0).  The density covers them, although the reported
+    transmission pdf is zero there: with the GGX lobe's reported pdf
+    p_r = 2 pdf_world on that side, the density (p_r + s_t) / 2 exceeds
+    pdf_world by s_t / 2 > 0."""
+    o = _transmissive(ior=1.5, roughness=0.5)
+    wz = -0.15
+    p, total, acc, d, wi = _density_chi2(o, [math.sqrt(1 - wz * wz), 0.0, wz])
+    assert p > 1e-4, (p, total, acc)
+    n = 200000
+    rng = np.random.default_rng(3)
+    u = np.zeros((n, 3), np.uint32)
+    u[:, 0] = (1 << 31) + rng.integers(0, 1 << 31, n, dtype=np.uint64).astype(np.uint32)  # lobe 1 of 2
+    u[:, 1:3] = rng.random((n, 2)).astype(np.float32).view(np.uint32)
+    q = _query(o, np.tile(wi, (n, 1)), np.tile([0, 0, 1.0], (n, 1)), u)
+    dd = q[:, 13:16]
+    side = dd[:, 2] * wi[2] > 0
+    assert side.mean() > 0.01, side.mean()
+    r = _query(o, np.tile(wi, (int(side.sum()), 1)), dd[side])
+    assert np.all(r[:, 19] > r[:, 3]), np.min(r[:, 19] - r[:, 3])
+
+
+def test_full_disney_draw_density():
+    rng = np.random.default_rng(17)
+    for k in range(3):
+        rec = scenes.random_disney(rng, full=True)
+        rec[12] = max(rec[12], 0.3)              # clearcoat on
+        rec[5] = 0.0                             # isotropic
+        rec[4] = 0.3 + 0.2 * rec[4]              # roughness in [0.3, 0.5]
+        o = _scene(rec)
+        for wz in (0.8, -0.4):
+            p, total, acc, _, _ = _density_chi2(o, [math.sqrt(1 - wz * wz), 0.0, wz], n_draws=200000)
+            assert p > 1e-4, (k, wz, p, total, acc)
