@@ -424,10 +424,28 @@ GUIDE_SWEEP = [("alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1)),
                ("all lobes, alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1, lobes="all")),
                ("all lobes, alpha 0.5, first vertex", dict(alpha=0.5, guided_depth=1, lobes="all")),
                ("all lobes, alpha 0.3, two vertices", dict(alpha=0.3, guided_depth=2, lobes="all")),
-               ("all lobes, alpha 0.3, all vertices", dict(alpha=0.3, guided_depth=0, lobes="all"))]
+               ("all lobes, alpha 0.3, all vertices", dict(alpha=0.3, guided_depth=0, lobes="all")),
+               ("albedo lobe choice (mix 0.25), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="albedo", select_mix=0.25)),
+               ("albedo lobe choice (mix 0.1), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="albedo", select_mix=0.1)),
+               ("albedo lobe choice (mix 0.25), all lobes, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobes="all", lobe_select="albedo", select_mix=0.25)),
+               ("albedo lobe choice (mix 0.25), alpha 0 (no guide)",
+                dict(alpha=0.0, guided_depth=1, lobe_select="albedo", select_mix=0.25)),
+               ("learned lobe choice (mix 0.25), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.25)),
+               ("learned lobe choice (mix 0.1), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.1)),
+               ("learned lobe choice (mix 0.25), alpha 0 (no guide)",
+                dict(alpha=0.0, guided_depth=1, lobe_select="learned", select_mix=0.25)),
+               ("learned lobe choice (mix 0.4), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4)),
+               ("learned lobe choice (mix 0.25), all lobes, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobes="all", lobe_select="learned", select_mix=0.25))]
 
 
-def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128):
+def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128, only=None):
     """Guide settings on the 1080p showcase scene and on the same scene without
     its clearcoat and transmission lobes: trimmed relMSE against a ref_spp
     PT+NEE render and wall time after every 8-spp batch, for plain PT+NEE and
@@ -471,6 +489,8 @@ def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128):
         res = {"plain PT+NEE": curve(st, R, "pt_nee")}
         log("guide sweep, %s: plain %s" % (name, res["plain PT+NEE"][-1]))
         for label, cfg in GUIDE_SWEEP:
+            if only and not any(w in label for w in only):
+                continue
             c = dict(alpha=0.3, guided_depth=1, steps_per_pass=8, batch=16384, train_paths=1 << 16,
                      warmup_steps=8, learning_rate=5e-3)
             c.update(cfg)
@@ -499,6 +519,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
+    ap.add_argument("--sweep-only", default="",
+                    help="with --guide-sweep: comma-separated substrings; only settings whose label has one")
     ap.add_argument("--guide-sweep", action="store_true",
                     help="only the guide-settings sweep on the 1080p scene (with and without its clearcoat and "
                          "transmission lobes), printed as JSON")
@@ -544,7 +566,8 @@ def main():
     if args.guide_sweep:
         ctx = Context(0)
         ctx.set_stream(stream.cuda_stream)
-        print(json.dumps(bench_guide_sweep(ctx)), flush=True)
+        only = [w for w in args.sweep_only.split(",") if w] if args.sweep_only else None
+        print(json.dumps(bench_guide_sweep(ctx, only=only)), flush=True)
         return
     t0 = time.time()
     spec = scenes.icosphere_scene(level=args.level, seed=2)

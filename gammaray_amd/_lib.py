@@ -70,7 +70,7 @@ GR_JOBS_AS_GIVEN, GR_JOBS_FEWEST_FIRST = 0, 1
 class GuideParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("alpha", ctypes.c_float),
                 ("batch", _u32), ("steps_per_pass", _u32), ("train_paths", _u32), ("guided_depth", _u32),
-                ("warmup_steps", _u32), ("lobes", _u32)]
+                ("warmup_steps", _u32), ("lobes", _u32), ("lobe_select", _u32), ("select_mix", ctypes.c_float)]
 
 
 GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED = 0, 1, 2, 3
@@ -78,6 +78,7 @@ GR_INTEGRATOR_DISPLAY = 4
 GR_INTEGRATOR_BDPT_MIS = 5
 GR_INTEGRATOR_BDPT_LT = 6
 GR_GUIDE_LOBES_CONSISTENT, GR_GUIDE_LOBES_ALL = 0, 1
+GR_GUIDE_SELECT_UNIFORM, GR_GUIDE_SELECT_ALBEDO, GR_GUIDE_SELECT_LEARNED = 0, 1, 2
 
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.
 SIGNATURES = {

@@ -312,13 +312,18 @@ class Guide:
     """The learned path guide (an MLP on the matrix cores, trained online)."""
 
     def __init__(self, ctx, seed=1, learning_rate=5e-3, alpha=0.5, batch=16384, steps_per_pass=4,
-                 train_paths=1 << 16, guided_depth=0, warmup_steps=8, lobes="consistent"):
+                 train_paths=1 << 16, guided_depth=0, warmup_steps=8, lobes="consistent",
+                 lobe_select="uniform", select_mix=0.25):
         self.ctx = ctx
         sets = {"consistent": _lib.GR_GUIDE_LOBES_CONSISTENT, "all": _lib.GR_GUIDE_LOBES_ALL}
         if lobes not in sets:
             raise ValueError("guided lobe set must be one of %s" % sorted(sets))
+        choices = {"uniform": _lib.GR_GUIDE_SELECT_UNIFORM, "albedo": _lib.GR_GUIDE_SELECT_ALBEDO,
+                   "learned": _lib.GR_GUIDE_SELECT_LEARNED}
+        if lobe_select not in choices:
+            raise ValueError("lobe choice must be one of %s" % sorted(choices))
         p = _lib.GuideParams(seed, learning_rate, alpha, batch, steps_per_pass, train_paths, guided_depth,
-                             warmup_steps, sets[lobes])
+                             warmup_steps, sets[lobes], choices[lobe_select], select_mix)
         self._h = ctypes.c_void_p()
         check(lib().gr_guide_create(ctx.handle, ctypes.byref(p), ctypes.byref(self._h)), ctx.handle,
               "guide_create")

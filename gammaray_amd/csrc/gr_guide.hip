@@ -861,6 +861,11 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
     if (!(prm->alpha >= 0.0f && prm->alpha < 1.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "alpha must be in [0, 1)");
     if (prm->lobes != GR_GUIDE_LOBES_CONSISTENT && prm->lobes != GR_GUIDE_LOBES_ALL)
         return gr::set_error(ctx, GR_ERR_INVALID, "unknown guided lobe set");
+    if (prm->lobe_select != GR_GUIDE_SELECT_UNIFORM && prm->lobe_select != GR_GUIDE_SELECT_ALBEDO &&
+        prm->lobe_select != GR_GUIDE_SELECT_LEARNED)
+        return gr::set_error(ctx, GR_ERR_INVALID, "unknown lobe choice");
+    if (prm->lobe_select != GR_GUIDE_SELECT_UNIFORM && !(prm->select_mix > 0.0f && prm->select_mix <= 1.0f))
+        return gr::set_error(ctx, GR_ERR_INVALID, "select_mix must be in (0, 1]");
     if (!(prm->learning_rate > 0.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "learning rate must be positive");
     gr_guide* g = new (std::nothrow) gr_guide();
     if (!g) return gr::set_error(ctx, GR_ERR_NOMEM, "out of host memory");
@@ -909,6 +914,7 @@ void gr_guide_destroy(gr_guide* g) {
     (void)hipFree(g->work);
     (void)hipFree(g->recs_blob);
     (void)hipFree(g->vpart);
+    (void)hipFree(g->lobe_blob);
     gr_ctx* ctx = g->ctx;
     delete g;
     gr::ctx_release(ctx);

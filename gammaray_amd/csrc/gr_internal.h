@@ -273,6 +273,16 @@ struct gr_guide {
     // per-block partial sums of record values, grown on demand
     float* vpart = nullptr;
     size_t vpart_cap = 0;
+    // learned lobe choice (GR_GUIDE_SELECT_LEARNED): per (primitive, incoming
+    // bucket, lobe) fixed-point sums of the chosen lobe's contribution and
+    // their counts, and the choice probabilities made from them, for the
+    // scene they were gathered on
+    void* lobe_blob = nullptr;
+    unsigned long long* lobe_sum = nullptr;
+    uint32_t* lobe_cnt = nullptr;
+    float4* lobe_c = nullptr;
+    size_t lobe_prims = 0;
+    const gr_scene* lobe_scene = nullptr;
 };
 
 namespace gr {

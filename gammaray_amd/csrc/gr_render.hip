@@ -458,7 +458,16 @@ __device__ __forceinline__ uint32_t block_append(bool alive, uint32_t* counter) 
 
 // State of a guided trace (GR_INTEGRATOR_GUIDED).
 constexpr uint32_t kGuideNoQuery = 0xfffffffeu;  // eligible vertex, guide not sampled yet
-constexpr uint32_t kVrecFloat4 = 4;              // vertex record: raw(8) | dir, Lb.x | Lb.yz, T lum, valid
+// vertex record: raw(8) | dir, Lb.x | Lb.yz, T lum, guide-record flag |
+// primitive, lobe | bucket << 4, n c_k, lobe-record flag
+constexpr uint32_t kVrecFloat4 = 5;
+// learned lobe choice: incoming-angle buckets per primitive (4 |cos| bins,
+// outside and inside), samples per lobe before a cell's statistics are used,
+// the fixed-point scale and the clamp of one contribution
+constexpr uint32_t kLobeBuckets = 8;
+constexpr uint32_t kLobeMinCount = 64;
+constexpr float kLobeFixed = 65536.0f;
+constexpr float kLobeYMax = 1.0e4f;
 
 struct GuideArgs {
     float* raw;        // per query: 8 raw inputs
@@ -468,7 +477,12 @@ struct GuideArgs {
     float* recs;       // compacted training records
     uint32_t* rec_count;
     uint32_t rec_cap, train_stride, train_paths, drec, guided_depth, active;
-    uint32_t lobes;  // GR_GUIDE_LOBES_*: the lobe picks the guide may redirect
+    uint32_t lobes;   // GR_GUIDE_LOBES_*: the lobe picks the guide may redirect
+    uint32_t select;  // GR_GUIDE_SELECT_*: how the guided integrator picks a lobe
+    float select_mix; // share of the uniform choice in the lobe-choice probabilities
+    unsigned long long* lobe_sum;  // learned lobe choice: per (prim, bucket, lobe)
+    uint32_t* lobe_cnt;
+    const float4* lobe_c;          // per (prim, bucket): choice probabilities, x < 0: none yet
     float alpha;
     float lo[3], inv_ext[3];
 };
@@ -524,8 +538,81 @@ __device__ __forceinline__ uint32_t guided_lobes(const GpuMaterial& m, uint32_t 
     return c;
 }
 
+// Lobe-choice probabilities of the guided integrator.  The reference picks
+// each of the n lobes with probability 1/n; picking lobe k with probability
+// c_k > 0 instead and dividing that lobe's sample by n c_k keeps each lobe's
+// contribution, and so the plain estimator's mean, for any such c.
+//
+// GR_GUIDE_SELECT_ALBEDO: c_k = mix / n + (1 - mix) a_k / sum a, with a_k the
+// luminance of lobe k's colour times its Fresnel factor at the incoming
+// angle -- the share of the energy each lobe returns -- so smooth glassy
+// surfaces, where one lobe carries almost everything, spend their samples on
+// it.  GR_GUIDE_SELECT_LEARNED: the same with a_k the mean contribution of
+// lobe k's samples on earlier batches (k_lobe_stats, k_lobe_table).  mix > 0
+// keeps every c_k positive.
+__device__ __forceinline__ float lobe_albedo(const GpuLobe& l, vec3 il) {
+    const float c = fabsf(il.z);
+    switch (l.type) {
+        case kLobeDisneyDiffuse: return dev::luminance(dev::lv3(l, 0) + dev::lv3(l, 3));
+        case kLobeDisneySpecular:
+        case kLobeClearcoat: return dev::luminance(mul(dev::refl_colour(l), dev::refl_fresnel(l, c)));
+        case kLobeSpecularTrans: return dev::luminance(dev::lv3(l, 3)) * (1.0f - dev::dielectric(il.z, l.v[2]));
+        case kLobePerfectDiffuse: return 0.5f;
+        default: return 1.0f;
+    }
+}
+
+struct LobeChoice {
+    float c[4];
+    uint32_t pick, bucket;
+};
+// Incoming-angle bucket of the learned lobe choice: 4 bins of |cos theta_i|,
+// outside (il.z > 0) and inside.
+__device__ __forceinline__ uint32_t lobe_bucket(vec3 il) {
+    const uint32_t b = (uint32_t)fminf(fabsf(il.z) * 4.0f, 3.0f);
+    return il.z > 0.0f ? b : 4u + b;
+}
+__device__ __forceinline__ LobeChoice choose_lobe(const GpuMaterial& m, vec3 il, const GuideArgs& g, uint32_t u,
+                                                  uint32_t prim) {
+    LobeChoice ch;
+    const uint32_t n = m.nlobes;
+    for (uint32_t k = 0; k < 4; ++k) ch.c[k] = k < n ? 1.0f / (float)n : 0.0f;
+    ch.pick = (uint32_t)(((uint64_t)u * n) >> 32);  // the reference's choice
+    ch.bucket = lobe_bucket(il);
+    if (g.select == GR_GUIDE_SELECT_UNIFORM || n < 2) return ch;
+    bool learned = false;
+    if (g.select == GR_GUIDE_SELECT_LEARNED) {
+        const float4 c = g.lobe_c[(size_t)prim * kLobeBuckets + ch.bucket];
+        if (c.x >= 0.0f) {
+            ch.c[0] = c.x, ch.c[1] = c.y, ch.c[2] = c.z, ch.c[3] = c.w;
+            learned = true;
+        }
+    }
+    if (!learned) {
+        float a[4], sum = 0.0f;
+        for (uint32_t k = 0; k < n; ++k) {
+            a[k] = fmaxf(lobe_albedo(m.lobes[k], il), 0.0f);
+            sum += a[k];
+        }
+        if (!(sum > 0.0f) || !isfinite(sum)) return ch;
+        const float floor_c = g.select_mix / (float)n;
+        for (uint32_t k = 0; k < n; ++k) ch.c[k] = floor_c + (1.0f - g.select_mix) * (a[k] / sum);
+    }
+    const float x = u01(u);
+    float acc = 0.0f;
+    ch.pick = n - 1;
+    for (uint32_t k = 0; k + 1 < n; ++k) {
+        acc += ch.c[k];
+        if (x < acc) {
+            ch.pick = k;
+            break;
+        }
+    }
+    return ch;
+}
+
 // Weight of the guided estimator at direction wo for a pick in the guided
-// lobe set G (|G| of the material's n lobes), and the reported BSDF pdf there.
+// lobe set G of the material's n lobes, and the reported BSDF pdf there.
 //
 // sample_world's draw of lobe k at wo reports radiance F_k(wo) and pdf P(wo):
 // P is the mean of the n reported lobe pdfs, and F_k is lobe k's f plus, when
@@ -533,10 +620,11 @@ __device__ __forceinline__ uint32_t guided_lobes(const GpuMaterial& m, uint32_t 
 // (src/material/material.rs:177-211).  With s_k the density lobe k's draws
 // realise (lobe_density), the picks in G contribute
 //   (1/n) sum_{k in G} int s_k F_k cos / P dwo.
-// The guided vertex enters G with the reference's own pick (probability
-// |G|/n) and draws wo from D = alpha q + (1 - alpha) S_G / |G| (q: the guide's
-// mixture, S_G: the sum of s_k over G), so the weight
-//   sum_{k in G} s_k F_k cos / (P |G| D)
+// The guided vertex enters G with its lobe choice (probability c_G, the sum
+// of the choice probabilities c_k over G) and draws wo from
+// D = alpha q + (1 - alpha) sum_{k in G} (c_k / c_G) s_k (q: the guide's
+// mixture), so the weight
+//   sum_{k in G} s_k F_k cos / (n P c_G D)
 // has the same mean.  A pick whose lobe draw failed (wo zero) weighs 0, as in
 // sample_world.
 struct GuidedWeight {
@@ -544,7 +632,7 @@ struct GuidedWeight {
     float pdf;  // P(wo), for the next vertex's emission MIS weight
 };
 __device__ __forceinline__ GuidedWeight guided_weight(const GpuMaterial& m, vec3 wi, vec3 wo, const Frame& f,
-                                                      uint32_t set, float alpha, float q) {
+                                                      uint32_t set, float alpha, float q, const float* c) {
     GuidedWeight r;
     r.g = mk(0.0f, 0.0f, 0.0f);
     r.pdf = 0.0f;
@@ -552,17 +640,17 @@ __device__ __forceinline__ GuidedWeight guided_weight(const GpuMaterial& m, vec3
     const vec3 il = dev::to_local(wi, f), ol = dev::to_local(wo, f);
     const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
     vec3 f_sum = mk(0.0f, 0.0f, 0.0f), own = f_sum;  // own: sum over G of s_k f_k where F_k = f_k
-    float s_all = 0.0f, s_g = 0.0f, s_take = 0.0f;    // s_take: sum over G of s_k where F_k = f_sum
-    uint32_t ng = 0;
+    float s_all = 0.0f, s_take = 0.0f;                // s_take: sum over G of s_k where F_k = f_sum
+    float c_g = 0.0f, cs_g = 0.0f;                    // sums over G of c_k and c_k s_k
     for (uint32_t k = 0; k < m.nlobes; ++k) {
         const GpuLobe& l = m.lobes[k];
         const vec3 fk = dev::lobe_f(l, il, ol, true);
         f_sum = f_sum + fk;
         if (il.z != 0.0f) s_all += dev::lobe_pdf(l, il, ol);
         if (!lobe_guided(l, set)) continue;
-        ++ng;
         const float sk = dev::lobe_density(l, il, ol);
-        s_g += sk;
+        c_g += c[k];
+        cs_g += c[k] * sk;
         const uint32_t kind = dev::lobe_kind(l);
         if ((refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission)))
             s_take += sk;
@@ -570,9 +658,10 @@ __device__ __forceinline__ GuidedWeight guided_weight(const GpuMaterial& m, vec3
             own = own + fk * sk;
     }
     r.pdf = s_all / (float)m.nlobes;
-    const float dens = alpha * q + (1.0f - alpha) * (s_g / (float)ng);
+    const float dens = alpha * q + (1.0f - alpha) * (cs_g / c_g);
     if (r.pdf > 0.0f && dens > 0.0f)
-        r.g = (f_sum * s_take + own) * (fabsf(dot(f.n, wo)) / (r.pdf * ((float)ng * dens)));
+        r.g = (f_sum * s_take + own) *
+              (fabsf(dot(f.n, wo)) / (((float)m.nlobes * r.pdf) * (c_g * dens)));
     return r;
 }
 
@@ -686,7 +775,7 @@ struct ShadeOut {
 // One bounce of path `path` (state slot p: T, L, aux) at hit h of ray (o, d)
 // in scene (sv, sh).  Shared by the per-film wavefront and the batch queue so
 // both compute the same arithmetic.
-template <bool NEE, bool GUIDED>
+template <bool NEE, bool GUIDED, bool CHOICE>
 __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView& sv, const ShadeView& sh,
                                            uint64_t seed, uint32_t max_depth, uint64_t path, uint32_t p,
                                            uint32_t depth, const gr_hit& h, vec3 o, vec3 d, ShadeOut& out) {
@@ -755,13 +844,26 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                     }
                     const vec3 T_in = T;
                     vec3 out_dir;
-                    // the lobe the reference's sample_world picks with this draw
-                    const uint32_t pick = (uint32_t)(((uint64_t)r0.x * m.nlobes) >> 32);
+                    // the lobe the reference's sample_world picks with this draw, or the
+                    // guided integrator's lobe choice (choose_lobe)
+                    LobeChoice ch{};
+                    if (CHOICE) {
+                        ch = choose_lobe(m, dev::to_local(wi, f), a.gd, r0.x, h.prim);
+                    } else {
+                        ch.pick = (uint32_t)(((uint64_t)r0.x * m.nlobes) >> 32);
+                    }
+                    const uint32_t pick = ch.pick;
                     const bool split = GUIDED && gs != kNone && lobe_guided(m.lobes[pick], a.gd.lobes);
                     if (!split) {
-                        const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                        const dev::MatSample smp =
+                            (CHOICE && a.gd.select != GR_GUIDE_SELECT_UNIFORM)
+                                ? dev::sample_world_lobe(m, wi, f, true, pick, u01(r0.y), u01(r0.z))
+                                : dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
                         note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
-                        T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
+                        float w = fabsf(dot(f.n, smp.outgoing)) / smp.pdf;
+                        if (CHOICE && a.gd.select != GR_GUIDE_SELECT_UNIFORM)
+                            w = w / ((float)m.nlobes * ch.c[pick]);
+                        T = mul(T, smp.radiance * w);
                         a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
                         out_dir = smp.outgoing;
                     } else {
@@ -769,22 +871,27 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                         if (mix && u01(rg.x) < alpha) {
                             out_dir = mixture_sample(mix, u01(rg.y), u01(rg.z), u01(rg.w));
                         } else {
-                            out_dir = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z)).outgoing;
+                            out_dir = dev::sample_world_lobe(m, wi, f, true, pick, u01(r0.y), u01(r0.z)).outgoing;
                         }
                         const float q = mix ? mixture_pdf(mix, out_dir) : 0.0f;
-                        const GuidedWeight gw = guided_weight(m, wi, out_dir, f, a.gd.lobes, alpha, q);
+                        const GuidedWeight gw = guided_weight(m, wi, out_dir, f, a.gd.lobes, alpha, q, ch.c);
                         note_stat(a.stats, kStatNonfiniteBsdf, !finite3(gw.g));
                         T = mul(T, gw.g);
                         a.rb.aux[p] = make_float4(gw.pdf, 0.0f, 0.0f, 0.0f);
                     }
-                    if (split) {
-                        // training record: the direction and the radiance so far (this
-                        // vertex's NEE is added when its shadow ray resolves)
+                    // training records: the direction and the radiance so far (this
+                    // vertex's NEE is added when its shadow ray resolves) for the guide
+                    // at split vertices, and for the learned lobe choice the primitive,
+                    // incoming bucket, lobe and n c_k at every vertex with a choice
+                    const bool lobe_rec = CHOICE && a.gd.select == GR_GUIDE_SELECT_LEARNED && m.nlobes > 1;
+                    if (split || lobe_rec) {
                         const uint32_t vs = vrec_slot(a.gd, p, depth);
                         if (vs != kNone) {
                             float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
                             vr[2] = make_float4(out_dir.x, out_dir.y, out_dir.z, L.x);
-                            vr[3] = make_float4(L.y, L.z, dev::luminance(T_in), 1.0f);
+                            vr[3] = make_float4(L.y, L.z, dev::luminance(T_in), split ? 1.0f : 0.0f);
+                            vr[4] = make_float4(__uint_as_float(h.prim), __uint_as_float(pick | (ch.bucket << 4)),
+                                                (float)m.nlobes * ch.c[pick], lobe_rec ? 1.0f : 0.0f);
                         }
                     }
                     new_o = hp + out_dir * kRayPush;
@@ -807,7 +914,9 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
     out.hp = hp;
 }
 
-template <bool NEE, bool GUIDED>
+// GUIDED: the guide's mixture may be sampled at this depth (the guide's
+// queries ran); CHOICE: the guided integrator's lobe choice and records.
+template <bool NEE, bool GUIDED, bool CHOICE>
 __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = a.rb.counts[cur];
@@ -819,7 +928,7 @@ __global__ __launch_bounds__(256) void k_shade(RenderArgs a, uint32_t depth, uin
         const uint4 hv = a.rb.hits[i];
         const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
         const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
-        shade_path<NEE, GUIDED>(a, a.sv, a.sh, a.seed, a.max_depth, a.path_base + p, p, depth, h, o, d, out);
+        shade_path<NEE, GUIDED, CHOICE>(a, a.sv, a.sh, a.seed, a.max_depth, a.path_base + p, p, depth, h, o, d, out);
     }
     const uint32_t nxt = cur ^ 1u;
     const uint32_t slot = block_append(out.alive, &a.rb.counts[nxt]);
@@ -856,7 +965,7 @@ __global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a, uint32_t dep
         const uint32_t vs = vrec_slot(a.gd, p, depth);
         if (vs != kNone) {
             float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
-            if (vr[3].w != 0.0f) {
+            if (vr[3].w != 0.0f || vr[4].w != 0.0f) {
                 vr[2].w = vr[2].w + c.x;
                 vr[3].x = vr[3].x + c.y;
                 vr[3].y = vr[3].y + c.z;
@@ -989,6 +1098,62 @@ __global__ __launch_bounds__(256) void k_guide_rec_write(RenderArgs a, uint32_t 
         o[8] = r2.x, o[9] = r2.y, o[10] = r2.z, o[11] = v;
     }
     if (had) a.gd.vrec[(size_t)s * kVrecFloat4 + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Learned lobe choice, after each traced batch: every vertex record with a
+// lobe choice adds |Y| n c_k -- the radiance the chosen lobe's sample brought
+// (the path's radiance gathered after the vertex over the throughput it
+// arrived with), without the choice's 1 / (n c_k) -- to its (primitive,
+// incoming bucket, lobe) cell, in 1/65536 fixed point so the sums do not
+// depend on the order of the additions; then clears the record.
+__global__ __launch_bounds__(256) void k_lobe_stats(RenderArgs a, uint32_t nslots, uint32_t nprims) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    float4* vr = a.gd.vrec + (size_t)s * kVrecFloat4;
+    const float4 r4 = vr[4];
+    if (r4.w == 0.0f) return;
+    vr[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 r2 = vr[2], r3 = vr[3];
+    const uint32_t p = (s / a.gd.drec) * a.gd.train_stride;
+    const float4 L = p < a.npaths ? a.rb.L[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const vec3 dl = mk(L.x - r2.w, L.y - r3.x, L.z - r3.y);
+    float y = r3.z > 0.0f ? fmaxf(dev::luminance(dl), 0.0f) / r3.z * r4.z : 0.0f;
+    if (!(y >= 0.0f)) y = 0.0f;
+    y = fminf(y, kLobeYMax);
+    const uint32_t prim = __float_as_uint(r4.x), code = __float_as_uint(r4.y);
+    if (prim >= nprims || (code >> 4) >= kLobeBuckets || (code & 15u) >= 4u) return;  // not this scene's
+    const size_t cell = ((size_t)prim * kLobeBuckets + (code >> 4)) * 4 + (code & 15u);
+    atomicAdd(a.gd.lobe_sum + cell, (unsigned long long)(y * kLobeFixed));
+    atomicAdd(a.gd.lobe_cnt + cell, 1u);
+}
+
+// Choice probabilities per (primitive, incoming bucket) from the sums:
+// c_k = mix / n + (1 - mix) m_k / sum m with m_k the mean of lobe k's
+// contributions, once every lobe of the cell has kLobeMinCount samples
+// (x = -1 until then: choose_lobe falls back to the albedo rule).
+__global__ __launch_bounds__(256) void k_lobe_table(const unsigned long long* sum, const uint32_t* cnt,
+                                                    const GpuMaterial* mats, uint32_t ncells, float mix,
+                                                    float4* c) {
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncells) return;
+    const uint32_t n = mats[cell / kLobeBuckets].nlobes;
+    float4 out = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+    if (n >= 2 && n <= 4) {
+        float mk_[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
+        bool ok = true;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t nk = cnt[(size_t)cell * 4 + k];
+            if (nk < kLobeMinCount) ok = false;
+            mk_[k] = nk ? (float)((double)sum[(size_t)cell * 4 + k] / (double)nk) : 0.0f;
+            tot += mk_[k];
+        }
+        if (ok && tot > 0.0f) {
+            float cc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t k = 0; k < n; ++k) cc[k] = mix / (float)n + (1.0f - mix) * (mk_[k] / tot);
+            out = make_float4(cc[0], cc[1], cc[2], cc[3]);
+        }
+    }
+    c[cell] = out;
 }
 
 __global__ __launch_bounds__(256) void k_export_samples(RenderArgs a, float4* out) {
@@ -1982,6 +2147,35 @@ namespace {
 // Guided-trace buffers of a film: per path a query (raw inputs, mixture) and
 // a slot, plus the training vertex records (zeroed once; each batch's
 // finalize clears what it read).
+// The learned lobe choice's sums, counts and probabilities for scene sc,
+// zeroed (probabilities unset) when the guide comes to a scene other than the
+// one they were gathered on.
+hipError_t reserve_lobe_table(gr_guide* g, const gr_scene* sc, hipStream_t s) {
+    const size_t np = sc->prims.size();
+    if (g->lobe_scene == sc && g->lobe_prims == np) return hipSuccess;
+    const size_t cells = np * kLobeBuckets;
+    const size_t b_sum = ((cells * 4 * 8) + 255) & ~(size_t)255, b_cnt = ((cells * 4 * 4) + 255) & ~(size_t)255;
+    const size_t b_c = cells * sizeof(float4);
+    if (np > g->lobe_prims || !g->lobe_blob) {
+        (void)hipFree(g->lobe_blob);
+        g->lobe_blob = nullptr;
+        g->lobe_prims = 0;
+        g->lobe_scene = nullptr;
+        hipError_t e = hipMalloc(&g->lobe_blob, b_sum + b_cnt + b_c);
+        if (e != hipSuccess) return e;
+    }
+    char* b = static_cast<char*>(g->lobe_blob);
+    g->lobe_sum = reinterpret_cast<unsigned long long*>(b);
+    g->lobe_cnt = reinterpret_cast<uint32_t*>(b + b_sum);
+    g->lobe_c = reinterpret_cast<float4*>(b + b_sum + b_cnt);
+    hipError_t e = hipMemsetAsync(b, 0, b_sum + b_cnt, s);
+    if (e == hipSuccess) e = hipMemsetAsync(g->lobe_c, 0xff, b_c, s);  // NaN: no probabilities yet
+    if (e != hipSuccess) return e;
+    g->lobe_prims = np;
+    g->lobe_scene = sc;
+    return hipSuccess;
+}
+
 hipError_t reserve_guide(RenderBuffers& rb, size_t paths, size_t vrecs) {
     if (rb.guide_capacity >= paths && rb.guide_vrecs >= vrecs) return hipSuccess;
     paths = std::max(paths, rb.guide_capacity);
@@ -2066,6 +2260,15 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         a.gd.drec = drec;
         a.gd.guided_depth = gp.guided_depth;
         a.gd.lobes = gp.lobes;
+        a.gd.select = gp.lobe_select;
+        a.gd.select_mix = gp.select_mix;
+        if (gp.lobe_select == GR_GUIDE_SELECT_LEARNED) {
+            e = reserve_lobe_table(guide, sc, s);
+            if (e != hipSuccess) return e;
+            a.gd.lobe_sum = guide->lobe_sum;
+            a.gd.lobe_cnt = guide->lobe_cnt;
+            a.gd.lobe_c = guide->lobe_c;
+        }
         a.gd.alpha = gp.alpha;
         for (int k = 0; k < 3; ++k) {
             const float ext = sc->view.root_hi[k] - sc->view.root_lo[k];
@@ -2127,14 +2330,19 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, cur ^ 1u);
             if (nee) {
                 hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 2u);
-                if (guided) {
+                if (guided && (a.gd.guided_depth == 0 || depth < a.gd.guided_depth)) {
                     hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, 3u);
                     hipLaunchKernelGGL(k_guide_query, blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
                     if (a.gd.active) e = guide_infer(guide, a.gd.raw, a.npaths, film->rb.counts + 3, a.gd.mix, s);
                     if (e != hipSuccess) break;
-                    hipLaunchKernelGGL((k_shade<true, true>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                    hipLaunchKernelGGL((k_shade<true, true, true>), blocks_for(a.npaths), dim3(256), 0, s, a, depth,
+                                       cur);
+                } else if (guided) {
+                    // beyond the guided depth: the lobe choice only, no queries
+                    hipLaunchKernelGGL((k_shade<true, false, true>), blocks_for(a.npaths), dim3(256), 0, s, a, depth,
+                                       cur);
                 } else {
-                    hipLaunchKernelGGL((k_shade<true, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                    hipLaunchKernelGGL((k_shade<true, false, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
                 }
                 e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
                                     film->rb.counts + 2, film->rb.vis, GR_OCCLUSION_EXACT, nullptr, s);
@@ -2144,7 +2352,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
                 else
                     hipLaunchKernelGGL((k_shadow_accum<false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth);
             } else {
-                hipLaunchKernelGGL((k_shade<false, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
+                hipLaunchKernelGGL((k_shade<false, false, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
             }
             e = hipGetLastError();
             cur ^= 1u;
@@ -2152,9 +2360,15 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         if (e != hipSuccess) break;
         if (guided) {
             // train on this batch's records before the next batch samples the guide
+            const dim3 rb = blocks_for(nslots);
+            if (a.gd.select == GR_GUIDE_SELECT_LEARNED) {
+                const uint32_t cells = (uint32_t)(guide->lobe_prims * kLobeBuckets);
+                hipLaunchKernelGGL(k_lobe_stats, rb, dim3(256), 0, s, a, nslots, (uint32_t)guide->lobe_prims);
+                hipLaunchKernelGGL(k_lobe_table, blocks_for(cells), dim3(256), 0, s, guide->lobe_sum, guide->lobe_cnt,
+                                   sc->shade.mats, cells, a.gd.select_mix, guide->lobe_c);
+            }
             e = guide_zero_stats(guide, s);
             if (e != hipSuccess) break;
-            const dim3 rb = blocks_for(nslots);
             hipLaunchKernelGGL(k_guide_rec_count, rb, dim3(256), 0, s, a, nslots, guide->rec_bcount);
             hipLaunchKernelGGL(k_guide_rec_scan, dim3(1), dim3(1024), 0, s, guide->rec_bcount, rb.x,
                                guide->rec_boff, guide->count);
@@ -2322,7 +2536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         scene = __float_as_uint(ro.w);
         m = b.meta[cur][i];
         const JobDev& J = b.jobs[m.x];
-        shade_path<NEE, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + m.y, i, m.z, h, o, d, out);
+        shade_path<NEE, false, false>(a, J.sv, J.sh, J.seed, J.max_depth, J.path_base0 + m.y, i, m.z, h, o, d, out);
         retire = !out.alive;
     }
     const uint32_t q = block_append(out.alive, &a.rb.counts[nxt]);

@@ -644,9 +644,11 @@ struct MatSample {
     uint32_t kind;
 };
 
-// Material::sample_world; the lobe index is the multiply-shift of u_lobe.
-__device__ __forceinline__ MatSample sample_world(const GpuMaterial& m, vec3 wi, const Frame& f, bool c2l,
-                                                  uint32_t u_lobe, float u1, float u2) {
+// Material::sample_world after its lobe choice: lobe r (< m.nlobes) sampled
+// with (u1, u2), the other lobes' densities and values added as the
+// reference does.
+__device__ __forceinline__ MatSample sample_world_lobe(const GpuMaterial& m, vec3 wi, const Frame& f, bool c2l,
+                                                       uint32_t r, float u1, float u2) {
     MatSample out;
     const vec3 il = to_local(wi, f);
     out.emission = light_world(m, wi, f);
@@ -657,7 +659,6 @@ __device__ __forceinline__ MatSample sample_world(const GpuMaterial& m, vec3 wi,
         return out;
     }
     const uint32_t n = m.nlobes;
-    const uint32_t r = (uint32_t)(((uint64_t)u_lobe * n) >> 32);
     const GpuLobe& lobe = m.lobes[r];
     const uint32_t kind = lobe_kind(lobe);
     const LobeSample s = lobe_sample(lobe, il, c2l, u1, u2);
@@ -691,6 +692,13 @@ __device__ __forceinline__ MatSample sample_world(const GpuMaterial& m, vec3 wi,
     out.pdf = pdf;
     out.kind = kind;
     return out;
+}
+
+// Material::sample_world; the lobe index is the multiply-shift of u_lobe.
+__device__ __forceinline__ MatSample sample_world(const GpuMaterial& m, vec3 wi, const Frame& f, bool c2l,
+                                                  uint32_t u_lobe, float u1, float u2) {
+    const uint32_t r = (uint32_t)(((uint64_t)u_lobe * m.nlobes) >> 32);
+    return sample_world_lobe(m, wi, f, c2l, r, u1, u2);
 }
 
 }  // namespace dev

@@ -344,6 +344,8 @@ typedef struct {
     uint32_t guided_depth;    /* vertices at depth < guided_depth are guided; 0 = all */
     uint32_t warmup_steps;    /* Adam steps taken before the guide is sampled */
     uint32_t lobes;           /* GR_GUIDE_LOBES_*: the lobe picks the guide may redirect */
+    uint32_t lobe_select;     /* GR_GUIDE_SELECT_*: how a guided trace picks a vertex's lobe */
+    float select_mix;         /* (0, 1]: share of the uniform choice in the lobe-choice probabilities */
 } gr_guide_params;
 /* Guided lobe sets.  CONSISTENT: Disney diffuse, GGX reflection and perfect
  * diffuse, whose reported pdfs are their sampling densities.  ALL: every lobe
@@ -353,6 +355,18 @@ typedef struct {
  * vector unnormalised). */
 #define GR_GUIDE_LOBES_CONSISTENT 0u
 #define GR_GUIDE_LOBES_ALL 1u
+/* Lobe choice at every vertex of a guided trace.  UNIFORM: the reference's
+ * (each of the n lobes with probability 1/n).  ALBEDO: lobe k with probability
+ * c_k = select_mix / n + (1 - select_mix) a_k / sum a, a_k the luminance of
+ * its colour times its Fresnel factor at the incoming angle.  LEARNED: the
+ * same with a_k the mean contribution lobe k's samples brought on earlier
+ * batches of this guide and scene, per primitive and incoming-angle bucket
+ * (ALBEDO until every lobe there has 64 samples).  A chosen lobe's sample is
+ * divided by n c_k, so each lobe's contribution -- and the plain mean -- is
+ * kept. */
+#define GR_GUIDE_SELECT_UNIFORM 0u
+#define GR_GUIDE_SELECT_ALBEDO 1u
+#define GR_GUIDE_SELECT_LEARNED 2u
 #define GR_GUIDE_PARAMS 27936  /* fp32 parameters: W1[208][64] b1 W2[64][64] b2 W3 b3 W4 b4 W5[64][32] b5 */
 #define GR_GUIDE_MIX_FLOATS 40 /* per query: 8 lobes x (weight, kappa, mean x, y, z) */
 #define GR_GUIDE_RECORD_FLOATS 12 /* training record: 8 raw inputs, direction (3), value */

@@ -221,23 +221,32 @@ def _sample_radiance(stage, film, integrator, spp, **kw):
     return buf.cpu().numpy()[:, :3].reshape(spp, H * W, 3)
 
 
-@pytest.mark.parametrize("variant,lobes", [("consistent", "consistent"), ("clearcoat", "consistent"),
-                                           ("transmission", "consistent"), ("clearcoat", "all"),
-                                           ("transmission", "all")])
-def test_guided_render_has_the_plain_mean(variant, lobes, gpu_ctx):
+@pytest.mark.parametrize("variant,lobes,select", [("consistent", "consistent", "uniform"),
+                                                  ("clearcoat", "consistent", "uniform"),
+                                                  ("transmission", "consistent", "uniform"),
+                                                  ("clearcoat", "all", "uniform"),
+                                                  ("transmission", "all", "uniform"),
+                                                  ("transmission", "consistent", "albedo"),
+                                                  ("clearcoat", "all", "albedo"),
+                                                  ("transmission", "all", "albedo"),
+                                                  ("transmission", "consistent", "learned"),
+                                                  ("clearcoat", "all", "learned")])
+def test_guided_render_has_the_plain_mean(variant, lobes, select, gpu_ctx):
     """Guided and plain PT+NEE per-pixel means agree within 3 sigma (different
     seeds), with the guide sampled after warm-up -- also where clearcoat and
     transmission lobes report pdfs that are not their sampling densities, and
     with those lobes' picks guided too (lobes="all": weighted with the
     densities their samplers realise and the radiance sample_world reports
-    for each pick, the extra lobe values of transmitted picks included)."""
-    print("case guided unbiased", variant, lobes, flush=True)
+    for each pick, the extra lobe values of transmitted picks included), and
+    with the lobe chosen by albedo rather than uniformly (select="albedo")."""
+    print("case guided unbiased", variant, lobes, select, flush=True)
     spec = {"consistent": lambda: scenes.box_scene(consistent=True),
             "clearcoat": lambda: scenes.box_scene(),
             "transmission": lambda: scenes.box_scene(seed=2, full=True)}[variant]()
     W, H, spp = 32, 32, 256
     stage = Stage(spec, ctx=gpu_ctx)
-    g = Guide(gpu_ctx, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=8, alpha=0.5, lobes=lobes)
+    g = Guide(gpu_ctx, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=8, alpha=0.5, lobes=lobes,
+              lobe_select=select)
     film = Film(gpu_ctx, W, H)
     a = _sample_radiance(stage, film, "guided", spp, max_depth=6, seed=31, batch=8, guide=g)
     assert g.steps() == 4 * spp // 8
@@ -245,10 +254,10 @@ def test_guided_render_has_the_plain_mean(variant, lobes, gpu_ctx):
     b = _sample_radiance(stage, film2, "pt_nee", spp, max_depth=6, seed=32)
     assert np.isfinite(a).all()
     # iterations traced with the guide sampled
-    assert_three_sigma(a[32:].sum(2), b[32:].sum(2), "guided %s %s" % (variant, lobes))
+    assert_three_sigma(a[32:].sum(2), b[32:].sum(2), "guided %s %s %s" % (variant, lobes, select))
 
 
-def test_guide_rejects_an_unknown_lobe_set(gpu_ctx):
+def test_guide_rejects_unknown_lobe_settings(gpu_ctx):
     from gammaray_amd import _lib
     from gammaray_amd.api import lib
     import ctypes
@@ -258,21 +267,28 @@ def test_guide_rejects_an_unknown_lobe_set(gpu_ctx):
     assert not h.value
     with pytest.raises(ValueError):
         Guide(gpu_ctx, lobes="clearcoat")
+    p = _lib.GuideParams(1, 5e-3, 0.5, 4096, 4, 4096, 0, 8, 0, 3, 0.25)
+    assert lib().gr_guide_create(gpu_ctx.handle, ctypes.byref(p), ctypes.byref(h)) != 0
+    p = _lib.GuideParams(1, 5e-3, 0.5, 4096, 4, 4096, 0, 8, 0, 1, 0.0)
+    assert lib().gr_guide_create(gpu_ctx.handle, ctypes.byref(p), ctypes.byref(h)) != 0
+    assert not h.value
 
 
-def test_guided_render_is_bit_reproducible(gpu_ctx):
+@pytest.mark.parametrize("select", ["uniform", "learned"])
+def test_guided_render_is_bit_reproducible(select, gpu_ctx):
     """Training records are compacted in slot order and every gradient sum is
-    taken in a fixed order (no float atomics), so a guided render -- the
-    records, the Adam steps trained on them, the guided samples and the film
+    taken in a fixed order (no float atomics), and the learned lobe choice sums
+    its statistics in fixed point, so a guided render -- the records, the Adam
+    steps trained on them, the lobe choices, the guided samples and the film
     -- repeats bit for bit."""
-    print("case guided reproducible", flush=True)
-    spec = scenes.indirect_room_scene()
+    print("case guided reproducible", select, flush=True)
+    spec = scenes.indirect_room_scene() if select == "uniform" else scenes.box_scene(seed=2, full=True)
     W, H = 48, 40
     stage = Stage(spec, ctx=gpu_ctx)
     out = []
     for _ in range(2):
         g = Guide(gpu_ctx, seed=5, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=4, alpha=0.5,
-                  guided_depth=2)
+                  guided_depth=2, lobe_select=select)
         f = Film(gpu_ctx, W, H)
         trace(stage, f, "guided", spp=24, max_depth=6, seed=61, batch=4, guide=g)
         out.append((f.readback(), g.get_params(), g.last_records().cpu().numpy(), g.steps()))
