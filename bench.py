@@ -330,7 +330,25 @@ def guide_inference_rate(ctx, guide, n=1 << 22, reps=5):
             "frac": round(tflops / FP16_DENSE_PEAK_TFLOPS, 4), "bound": "mfma", "dtype": "fp16 x fp16 -> fp32"}
 
 
-def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
+GUIDED_CONFIG = dict(alpha=0.3, guided_depth=1, steps_per_pass=4, batch=16384, train_paths=1 << 19, warmup_steps=8,
+                     learning_rate=5e-3, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)
+
+
+def _parse_overrides(text):
+    """'k=v,k=v' -> dict with int, float or string values."""
+    out = {}
+    for item in filter(None, text.split(",")):
+        k, v = item.split("=")
+        for cast in (int, float, str):
+            try:
+                out[k] = cast(v)
+                break
+            except ValueError:
+                pass
+    return out
+
+
+def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64, overrides=None):
     """Learned path guide on the 1080p showcase scene: wall time for guided
     rendering (online training included) to reach the relative MSE that plain
     PT+NEE reaches with 64 spp, against a ref_spp PT+NEE render."""
@@ -362,10 +380,10 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64):
     e_plain = _rmse(img, R)
     r_plain = _rel_mse_trimmed(img, R)
     plain.close()
-    # the guide samples the first vertex only (the scene's error comes from long paths, which a guide
-    # at later vertices does not reduce); batches of 8 spp like the plain render, 8 Adam steps each
-    cfg = dict(alpha=0.3, guided_depth=1, steps_per_pass=8, batch=16384, train_paths=1 << 16, warmup_steps=8,
-               learning_rate=5e-3)
+    # batches of 8 spp like the plain render; the guide's mixture at the first vertex, the learned
+    # lobe choice and the roulette at every vertex (GUIDED_CONFIG)
+    cfg = dict(GUIDED_CONFIG)
+    cfg.update(overrides or {})
     g = Guide(ctx, **cfg)
     infer = guide_inference_rate(ctx, g)
     f = Film(ctx, W, H)
@@ -442,7 +460,33 @@ GUIDE_SWEEP = [("alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1)),
                ("learned lobe choice (mix 0.4), alpha 0.3, first vertex",
                 dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4)),
                ("learned lobe choice (mix 0.25), all lobes, alpha 0.3, first vertex",
-                dict(alpha=0.3, guided_depth=1, lobes="all", lobe_select="learned", select_mix=0.25))]
+                dict(alpha=0.3, guided_depth=1, lobes="all", lobe_select="learned", select_mix=0.25)),
+               ("learned lobe choice (mix 0.5), alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.5)),
+               ("learned lobe choice (mix 0.4), alpha 0.3, first vertex, 4 Adam steps per batch",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, steps_per_pass=4)),
+               ("learned lobe choice (mix 0.4), alpha 0.3, first vertex, 262144 training paths",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, train_paths=1 << 18)),
+               ("learned lobe choice (mix 0.4), alpha 0.3, two vertices",
+                dict(alpha=0.3, guided_depth=2, lobe_select="learned", select_mix=0.4)),
+               ("learned lobe choice (mix 0.4), roulette 0.1, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)),
+               ("learned lobe choice (mix 0.4), roulette 0.25, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.25)),
+               ("learned lobe choice (mix 0.4), roulette 0.5, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.5)),
+               ("learned lobe choice (mix 0.4), roulette 0.05, alpha 0.3, first vertex, 4 Adam steps per batch",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.05,
+                     steps_per_pass=4)),
+               ("learned lobe choice (mix 0.4), roulette 0.1, alpha 0.3, first vertex, 4 Adam steps per batch",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.1,
+                     steps_per_pass=4)),
+               ("learned lobe choice (mix 0.4), roulette 0.15, alpha 0.3, first vertex, 4 Adam steps per batch",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.15,
+                     steps_per_pass=4)),
+               ("learned lobe choice (mix 0.25), roulette 0.1, alpha 0.3, first vertex, 4 Adam steps per batch",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.25, rr_threshold=0.1,
+                     steps_per_pass=4))]
 
 
 def bench_guide_sweep(ctx, ref_spp=2048, W=1920, H=1080, spp=128, only=None):
@@ -519,6 +563,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
+    ap.add_argument("--guided-only", action="store_true",
+                    help="only the guided-vs-plain comparison on the 1080p scene, printed as JSON")
+    ap.add_argument("--guided-config", default="",
+                    help="k=v overrides of the guide settings of the guided comparison, e.g. train_paths=262144")
     ap.add_argument("--sweep-only", default="",
                     help="with --guide-sweep: comma-separated substrings; only settings whose label has one")
     ap.add_argument("--guide-sweep", action="store_true",
@@ -545,7 +593,7 @@ def main():
         sys.exit(2)
 
     if not args.no_cpu_baseline and not args.guide_inference_only and not args.traffic_probe \
-            and not args.guide_sweep:
+            and not args.guide_sweep and not args.guided_only:
         use_native_oracle()
 
     import torch
@@ -562,6 +610,12 @@ def main():
         ctx = Context(0)
         ctx.set_stream(stream.cuda_stream)
         print(json.dumps(guide_inference_rate(ctx, Guide(ctx))), flush=True)
+        return
+    if args.guided_only:
+        ctx = Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        print(json.dumps(bench_guided(ctx, ref_spp=args.ref_spp, overrides=_parse_overrides(args.guided_config))),
+              flush=True)
         return
     if args.guide_sweep:
         ctx = Context(0)
@@ -697,7 +751,8 @@ def main():
     if not args.no_paths:
         paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads, only=only or None)
     batch = None if args.no_batch else bench_batch(ctx, threads)
-    guided = None if args.no_guided else bench_guided(ctx, ref_spp=args.ref_spp)
+    guided = None if args.no_guided else bench_guided(ctx, ref_spp=args.ref_spp,
+                                                      overrides=_parse_overrides(args.guided_config))
 
     out = {
         "metric": "Grays/s (closest-hit + shadow rays, incoherent, 1 GPU)",

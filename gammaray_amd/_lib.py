@@ -70,7 +70,8 @@ GR_JOBS_AS_GIVEN, GR_JOBS_FEWEST_FIRST = 0, 1
 class GuideParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("alpha", ctypes.c_float),
                 ("batch", _u32), ("steps_per_pass", _u32), ("train_paths", _u32), ("guided_depth", _u32),
-                ("warmup_steps", _u32), ("lobes", _u32), ("lobe_select", _u32), ("select_mix", ctypes.c_float)]
+                ("warmup_steps", _u32), ("lobes", _u32), ("lobe_select", _u32), ("select_mix", ctypes.c_float),
+                ("rr_threshold", ctypes.c_float)]
 
 
 GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED = 0, 1, 2, 3

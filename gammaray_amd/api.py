@@ -313,7 +313,7 @@ class Guide:
 
     def __init__(self, ctx, seed=1, learning_rate=5e-3, alpha=0.5, batch=16384, steps_per_pass=4,
                  train_paths=1 << 16, guided_depth=0, warmup_steps=8, lobes="consistent",
-                 lobe_select="uniform", select_mix=0.25):
+                 lobe_select="uniform", select_mix=0.25, rr_threshold=0.0):
         self.ctx = ctx
         sets = {"consistent": _lib.GR_GUIDE_LOBES_CONSISTENT, "all": _lib.GR_GUIDE_LOBES_ALL}
         if lobes not in sets:
@@ -323,7 +323,7 @@ class Guide:
         if lobe_select not in choices:
             raise ValueError("lobe choice must be one of %s" % sorted(choices))
         p = _lib.GuideParams(seed, learning_rate, alpha, batch, steps_per_pass, train_paths, guided_depth,
-                             warmup_steps, sets[lobes], choices[lobe_select], select_mix)
+                             warmup_steps, sets[lobes], choices[lobe_select], select_mix, rr_threshold)
         self._h = ctypes.c_void_p()
         check(lib().gr_guide_create(ctx.handle, ctypes.byref(p), ctypes.byref(self._h)), ctx.handle,
               "guide_create")

@@ -866,6 +866,8 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
         return gr::set_error(ctx, GR_ERR_INVALID, "unknown lobe choice");
     if (prm->lobe_select != GR_GUIDE_SELECT_UNIFORM && !(prm->select_mix > 0.0f && prm->select_mix <= 1.0f))
         return gr::set_error(ctx, GR_ERR_INVALID, "select_mix must be in (0, 1]");
+    if (!(prm->rr_threshold >= 0.0f && prm->rr_threshold < 1.0e30f))
+        return gr::set_error(ctx, GR_ERR_INVALID, "rr_threshold must be finite and >= 0");
     if (!(prm->learning_rate > 0.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "learning rate must be positive");
     gr_guide* g = new (std::nothrow) gr_guide();
     if (!g) return gr::set_error(ctx, GR_ERR_NOMEM, "out of host memory");

@@ -281,6 +281,7 @@ struct gr_guide {
     unsigned long long* lobe_sum = nullptr;
     uint32_t* lobe_cnt = nullptr;
     float4* lobe_c = nullptr;
+    float* lobe_m = nullptr;
     size_t lobe_prims = 0;
     const gr_scene* lobe_scene = nullptr;
 };

@@ -239,6 +239,8 @@ enum { kStreamCamera = 0, kStreamLight = 1, kStreamFilm = 2 };
 // block 2 = 8..11.
 enum { kSlotLobe = 0, kSlotU1 = 1, kSlotU2 = 2, kSlotRR = 3, kSlotLight = 4, kSlotLightTri = 5,
        kSlotLightU1 = 6, kSlotLightU2 = 7, kSlotLightDir1 = 8, kSlotLightDir2 = 9,
+       // guided integrator, camera stream: its Russian roulette (block 2)
+       kSlotGuideRR = 10,
        // guided vertices, block 3: technique choice, mixture lobe, vMF cosine, azimuth
        kSlotGuideSel = 12, kSlotGuideLobe = 13, kSlotGuideU1 = 14, kSlotGuideU2 = 15 };
 
@@ -462,9 +464,13 @@ constexpr uint32_t kGuideNoQuery = 0xfffffffeu;  // eligible vertex, guide not s
 // primitive, lobe | bucket << 4, n c_k, lobe-record flag
 constexpr uint32_t kVrecFloat4 = 5;
 // learned lobe choice: incoming-angle buckets per primitive (4 |cos| bins,
-// outside and inside), samples per lobe before a cell's statistics are used,
-// the fixed-point scale and the clamp of one contribution
+// outside and inside), each with a cell for the whole primitive and one per
+// octant of the shading normal (on a sphere, the part of the surface); the
+// samples per lobe before a cell's statistics are used, the fixed-point scale
+// and the clamp of one contribution
 constexpr uint32_t kLobeBuckets = 8;
+constexpr uint32_t kLobeSub = 9;
+constexpr uint32_t kLobeCells = kLobeBuckets * kLobeSub;  // per primitive
 constexpr uint32_t kLobeMinCount = 64;
 constexpr float kLobeFixed = 65536.0f;
 constexpr float kLobeYMax = 1.0e4f;
@@ -480,9 +486,12 @@ struct GuideArgs {
     uint32_t lobes;   // GR_GUIDE_LOBES_*: the lobe picks the guide may redirect
     uint32_t select;  // GR_GUIDE_SELECT_*: how the guided integrator picks a lobe
     float select_mix; // share of the uniform choice in the lobe-choice probabilities
+    float rr;         // throughput luminance below which paths play Russian roulette (0: off)
     unsigned long long* lobe_sum;  // learned lobe choice: per (prim, bucket, lobe)
     uint32_t* lobe_cnt;
-    const float4* lobe_c;          // per (prim, bucket): choice probabilities, x < 0: none yet
+    const float4* lobe_c;          // per cell: choice probabilities, x < 0 (or NaN): none yet
+    const float* lobe_m;           // per cell: mean radiance a continuation brought (the lobes'
+                                   // mean contribution), with lobe_c
     float alpha;
     float lo[3], inv_ext[3];
 };
@@ -564,7 +573,8 @@ __device__ __forceinline__ float lobe_albedo(const GpuLobe& l, vec3 il) {
 
 struct LobeChoice {
     float c[4];
-    uint32_t pick, bucket;
+    float m;  // the cell's mean continuation radiance (lobe_m), -1 without statistics
+    uint32_t pick, bucket, octant;
 };
 // Incoming-angle bucket of the learned lobe choice: 4 bins of |cos theta_i|,
 // outside (il.z > 0) and inside.
@@ -572,19 +582,26 @@ __device__ __forceinline__ uint32_t lobe_bucket(vec3 il) {
     const uint32_t b = (uint32_t)fminf(fabsf(il.z) * 4.0f, 3.0f);
     return il.z > 0.0f ? b : 4u + b;
 }
-__device__ __forceinline__ LobeChoice choose_lobe(const GpuMaterial& m, vec3 il, const GuideArgs& g, uint32_t u,
-                                                  uint32_t prim) {
+__device__ __forceinline__ uint32_t normal_octant(vec3 n) {
+    return (n.x > 0.0f ? 1u : 0u) | (n.y > 0.0f ? 2u : 0u) | (n.z > 0.0f ? 4u : 0u);
+}
+__device__ __forceinline__ LobeChoice choose_lobe(const GpuMaterial& m, vec3 il, vec3 n_world, const GuideArgs& g,
+                                                  uint32_t u, uint32_t prim) {
     LobeChoice ch;
     const uint32_t n = m.nlobes;
     for (uint32_t k = 0; k < 4; ++k) ch.c[k] = k < n ? 1.0f / (float)n : 0.0f;
     ch.pick = (uint32_t)(((uint64_t)u * n) >> 32);  // the reference's choice
     ch.bucket = lobe_bucket(il);
+    ch.octant = normal_octant(n_world);
+    ch.m = -1.0f;
     if (g.select == GR_GUIDE_SELECT_UNIFORM || n < 2) return ch;
     bool learned = false;
     if (g.select == GR_GUIDE_SELECT_LEARNED) {
-        const float4 c = g.lobe_c[(size_t)prim * kLobeBuckets + ch.bucket];
+        const size_t cell = ((size_t)prim * kLobeBuckets + ch.bucket) * kLobeSub + 1 + ch.octant;
+        const float4 c = g.lobe_c[cell];
         if (c.x >= 0.0f) {
             ch.c[0] = c.x, ch.c[1] = c.y, ch.c[2] = c.z, ch.c[3] = c.w;
+            ch.m = g.lobe_m[cell];
             learned = true;
         }
     }
@@ -611,58 +628,60 @@ __device__ __forceinline__ LobeChoice choose_lobe(const GpuMaterial& m, vec3 il,
     return ch;
 }
 
-// Weight of the guided estimator at direction wo for a pick in the guided
-// lobe set G of the material's n lobes, and the reported BSDF pdf there.
+// The guided integrator's vertex at direction wo, in one pass over the
+// material's lobes.
 //
 // sample_world's draw of lobe k at wo reports radiance F_k(wo) and pdf P(wo):
 // P is the mean of the n reported lobe pdfs, and F_k is lobe k's f plus, when
 // k's kind matches the reflect/transmit side of wo, every other lobe's f
-// (src/material/material.rs:177-211).  With s_k the density lobe k's draws
-// realise (lobe_density), the picks in G contribute
-//   (1/n) sum_{k in G} int s_k F_k cos / P dwo.
-// The guided vertex enters G with its lobe choice (probability c_G, the sum
-// of the choice probabilities c_k over G) and draws wo from
-// D = alpha q + (1 - alpha) sum_{k in G} (c_k / c_G) s_k (q: the guide's
-// mixture), so the weight
+// (src/material/material.rs:177-211).  A choice of lobe k with probability
+// c_k weighs F_k cos / (P n c_k).
+//
+// With s_k the density lobe k's draws realise (lobe_density), the picks in
+// the guided set G contribute (1/n) sum_{k in G} int s_k F_k cos / P dwo.  A
+// guided vertex enters G with its lobe choice (probability c_G, the sum of
+// c_k over G) and draws wo from D = alpha q + (1 - alpha) sum_{k in G}
+// (c_k / c_G) s_k (q: the guide's mixture), so the weight
 //   sum_{k in G} s_k F_k cos / (n P c_G D)
-// has the same mean.  A pick whose lobe draw failed (wo zero) weighs 0, as in
-// sample_world.
-struct GuidedWeight {
-    vec3 g;
-    float pdf;  // P(wo), for the next vertex's emission MIS weight
+// has the same mean.
+struct ChoiceEval {
+    vec3 f_sum, f_pick;  // all lobes' f; the chosen lobe's
+    vec3 own;            // sum over G of s_k f_k where F_k = f_k
+    float s_all;         // sum of the reported pdfs
+    float s_take;        // sum over G of s_k where F_k = f_sum
+    float c_g, cs_g;     // sums over G of c_k and c_k s_k
+    bool take_pick;      // F_pick = f_sum
 };
-__device__ __forceinline__ GuidedWeight guided_weight(const GpuMaterial& m, vec3 wi, vec3 wo, const Frame& f,
-                                                      uint32_t set, float alpha, float q, const float* c) {
-    GuidedWeight r;
-    r.g = mk(0.0f, 0.0f, 0.0f);
-    r.pdf = 0.0f;
-    if (dev::is_zero(wo)) return r;
-    const vec3 il = dev::to_local(wi, f), ol = dev::to_local(wo, f);
-    const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
-    vec3 f_sum = mk(0.0f, 0.0f, 0.0f), own = f_sum;  // own: sum over G of s_k f_k where F_k = f_k
-    float s_all = 0.0f, s_take = 0.0f;                // s_take: sum over G of s_k where F_k = f_sum
-    float c_g = 0.0f, cs_g = 0.0f;                    // sums over G of c_k and c_k s_k
+__device__ __forceinline__ ChoiceEval choice_eval(const GpuMaterial& m, vec3 il, vec3 ol, bool refl, uint32_t set,
+                                                  const float* c, uint32_t pick, bool in_g) {
+    ChoiceEval e;
+    e.f_sum = e.f_pick = e.own = mk(0.0f, 0.0f, 0.0f);
+    e.s_all = e.s_take = e.c_g = e.cs_g = 0.0f;
+    e.take_pick = false;
     for (uint32_t k = 0; k < m.nlobes; ++k) {
         const GpuLobe& l = m.lobes[k];
-        const vec3 fk = dev::lobe_f(l, il, ol, true);
-        f_sum = f_sum + fk;
-        if (il.z != 0.0f) s_all += dev::lobe_pdf(l, il, ol);
-        if (!lobe_guided(l, set)) continue;
-        const float sk = dev::lobe_density(l, il, ol);
-        c_g += c[k];
-        cs_g += c[k] * sk;
+        vec3 fk;
+        float pk;
+        dev::lobe_f_pdf(l, il, ol, true, true, &fk, &pk);
+        e.f_sum = e.f_sum + fk;
+        if (il.z != 0.0f) e.s_all += pk;
         const uint32_t kind = dev::lobe_kind(l);
-        if ((refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission)))
-            s_take += sk;
+        const bool take = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
+        if (k == pick) {
+            e.f_pick = fk;
+            e.take_pick = take;
+        }
+        if (!in_g || !lobe_guided(l, set)) continue;
+        const float sk = (l.type == kLobeClearcoat || l.type == kLobeSpecularTrans) ? dev::lobe_density(l, il, ol)
+                                                                                   : pk;
+        e.c_g += c[k];
+        e.cs_g += c[k] * sk;
+        if (take)
+            e.s_take += sk;
         else
-            own = own + fk * sk;
+            e.own = e.own + fk * sk;
     }
-    r.pdf = s_all / (float)m.nlobes;
-    const float dens = alpha * q + (1.0f - alpha) * (cs_g / c_g);
-    if (r.pdf > 0.0f && dens > 0.0f)
-        r.g = (f_sum * s_take + own) *
-              (fabsf(dot(f.n, wo)) / (((float)m.nlobes * r.pdf) * (c_g * dens)));
-    return r;
+    return e;
 }
 
 // Vertex-record slot of path p at this depth, or kNone when p records no
@@ -762,7 +781,7 @@ __global__ __launch_bounds__(256) void k_display(RenderArgs a) {
 // uniformly, sample it, weight F / ((1/n) sum_k pdf_k) -- is split by the
 // picked lobe.  A pick in the guide's lobe set G (lobe_guided) takes its
 // direction from the guide's mixture q with probability alpha and from the
-// picked lobe otherwise, weighted as guided_weight derives so that the picks
+// picked lobe otherwise, weighted as choice_eval derives so that the picks
 // in G keep the mean they have in the reference's estimator; any other pick
 // is the reference's sample as is.  So the guided estimator converges to the
 // plain estimator's image, including the reference's pdf quirks; the NEE and
@@ -848,36 +867,72 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                     // guided integrator's lobe choice (choose_lobe)
                     LobeChoice ch{};
                     if (CHOICE) {
-                        ch = choose_lobe(m, dev::to_local(wi, f), a.gd, r0.x, h.prim);
+                        ch = choose_lobe(m, dev::to_local(wi, f), f.n, a.gd, r0.x, h.prim);
                     } else {
                         ch.pick = (uint32_t)(((uint64_t)r0.x * m.nlobes) >> 32);
                     }
                     const uint32_t pick = ch.pick;
+                    // the pixel's scale for the guided roulette: the first vertex's
+                    // mean continuation radiance, carried in aux.z
+                    const float m_first = !CHOICE ? 0.0f : (depth == 0 ? fmaxf(ch.m, 0.0f) : aux.z);
                     const bool split = GUIDED && gs != kNone && lobe_guided(m.lobes[pick], a.gd.lobes);
-                    if (!split) {
-                        const dev::MatSample smp =
-                            (CHOICE && a.gd.select != GR_GUIDE_SELECT_UNIFORM)
-                                ? dev::sample_world_lobe(m, wi, f, true, pick, u01(r0.y), u01(r0.z))
-                                : dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
+                    if (!CHOICE) {
+                        const dev::MatSample smp = dev::sample_world(m, wi, f, true, r0.x, u01(r0.y), u01(r0.z));
                         note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
-                        float w = fabsf(dot(f.n, smp.outgoing)) / smp.pdf;
-                        if (CHOICE && a.gd.select != GR_GUIDE_SELECT_UNIFORM)
-                            w = w / ((float)m.nlobes * ch.c[pick]);
-                        T = mul(T, smp.radiance * w);
+                        T = mul(T, smp.radiance * (fabsf(dot(f.n, smp.outgoing)) / smp.pdf));
                         a.rb.aux[p] = make_float4(smp.pdf, (smp.kind & kLobeSpecular) ? 1.0f : 0.0f, 0.0f, 0.0f);
                         out_dir = smp.outgoing;
                     } else {
-                        const uint4 rg = draws(seed, path, kStreamCamera, depth, kSlotGuideSel / 4);
-                        if (mix && u01(rg.x) < alpha) {
+                        // the direction: the guide's mixture (a guided pick, probability
+                        // alpha) or the chosen lobe's own draw; then one pass over the lobes
+                        const vec3 il = dev::to_local(wi, f);
+                        bool from_guide = false;
+                        uint4 rg = make_uint4(0u, 0u, 0u, 0u);
+                        if (split) {
+                            rg = draws(seed, path, kStreamCamera, depth, kSlotGuideSel / 4);
+                            from_guide = mix && u01(rg.x) < alpha;
+                        }
+                        if (from_guide) {
                             out_dir = mixture_sample(mix, u01(rg.y), u01(rg.z), u01(rg.w));
                         } else {
-                            out_dir = dev::sample_world_lobe(m, wi, f, true, pick, u01(r0.y), u01(r0.z)).outgoing;
+                            vec3 hh = mk(0.0f, 0.0f, 0.0f);
+                            out_dir = dev::to_world(dev::lobe_draw(m.lobes[pick], il, u01(r0.y), u01(r0.z), &hh), f);
                         }
-                        const float q = mix ? mixture_pdf(mix, out_dir) : 0.0f;
-                        const GuidedWeight gw = guided_weight(m, wi, out_dir, f, a.gd.lobes, alpha, q, ch.c);
-                        note_stat(a.stats, kStatNonfiniteBsdf, !finite3(gw.g));
-                        T = mul(T, gw.g);
-                        a.rb.aux[p] = make_float4(gw.pdf, 0.0f, 0.0f, 0.0f);
+                        const GpuLobe& lp = m.lobes[pick];
+                        const float nc = (float)m.nlobes * ch.c[pick];
+                        vec3 g = mk(0.0f, 0.0f, 0.0f);
+                        float pdf_rep = 1.0f;
+                        bool spec = false;
+                        if (!dev::is_zero(out_dir)) {
+                            const vec3 ol = dev::to_local(out_dir, f);
+                            const float cos_o = fabsf(dot(f.n, out_dir));
+                            if (dev::lobe_kind(lp) & kLobeSpecular) {
+                                // a delta lobe: its own f and pdf, no other lobe (sample_world)
+                                pdf_rep = 1.0f / (float)m.nlobes;
+                                g = dev::lobe_f(lp, il, ol, true) * (cos_o / pdf_rep / nc);
+                                spec = true;
+                            } else {
+                                const bool refl = (dot(wi, f.gn) * dot(out_dir, f.gn)) > 0.0f;
+                                const ChoiceEval ev = choice_eval(m, il, ol, refl, a.gd.lobes, ch.c, pick, split);
+                                pdf_rep = ev.s_all / (float)m.nlobes;
+                                if (pdf_rep > 0.0f) {
+                                    if (!split) {
+                                        g = (ev.take_pick ? ev.f_sum : ev.f_pick) * (cos_o / pdf_rep / nc);
+                                    } else {
+                                        const float q = mix ? mixture_pdf(mix, out_dir) : 0.0f;
+                                        const float dens = alpha * q + (1.0f - alpha) * (ev.cs_g / ev.c_g);
+                                        if (dens > 0.0f)
+                                            g = (ev.f_sum * ev.s_take + ev.own) *
+                                                (cos_o / (((float)m.nlobes * pdf_rep) * (ev.c_g * dens)));
+                                    }
+                                } else {
+                                    pdf_rep = 1.0f;  // sample_world's black sample
+                                }
+                            }
+                        }
+                        note_stat(a.stats, kStatNonfiniteBsdf, !finite3(g));
+                        T = mul(T, g);
+                        a.rb.aux[p] = make_float4(pdf_rep, spec ? 1.0f : 0.0f, m_first, 0.0f);
                     }
                     // training records: the direction and the radiance so far (this
                     // vertex's NEE is added when its shadow ray resolves) for the guide
@@ -890,12 +945,29 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                             float4* vr = a.gd.vrec + (size_t)vs * kVrecFloat4;
                             vr[2] = make_float4(out_dir.x, out_dir.y, out_dir.z, L.x);
                             vr[3] = make_float4(L.y, L.z, dev::luminance(T_in), split ? 1.0f : 0.0f);
-                            vr[4] = make_float4(__uint_as_float(h.prim), __uint_as_float(pick | (ch.bucket << 4)),
+                            vr[4] = make_float4(__uint_as_float(h.prim),
+                                                __uint_as_float(pick | (ch.bucket << 4) | (ch.octant << 8)),
                                                 (float)m.nlobes * ch.c[pick], lobe_rec ? 1.0f : 0.0f);
                         }
                     }
                     new_o = hp + out_dir * kRayPush;
                     new_d = out_dir;
+                    if (CHOICE && a.gd.rr > 0.0f && depth >= 1) {
+                        // the guided integrator's roulette: a continuation whose expected
+                        // share of its pixel -- throughput times this vertex's mean
+                        // continuation radiance over the first vertex's -- is below rr
+                        // goes on with probability share / rr, its throughput divided by
+                        // that (unbiased); without those statistics the share is the
+                        // throughput's luminance
+                        const float share = (m_first > 0.0f && ch.m >= 0.0f)
+                                                ? dev::luminance(T_in) * (ch.m / m_first)
+                                                : dev::luminance(T);
+                        if (share < a.gd.rr) {
+                            const float q = share / a.gd.rr;
+                            const uint4 rq = draws(seed, path, kStreamCamera, depth, kSlotGuideRR / 4);
+                            T = (u01(rq.z) < q) ? T / q : mk(0.0f, 0.0f, 0.0f);
+                        }
+                    }
                     if (depth > 10 || dev::nearly_zero3(T)) russian_roulette(T, depth, u01(r0.w));
                     alive = !dev::is_zero(T);
                 }
@@ -1121,39 +1193,50 @@ __global__ __launch_bounds__(256) void k_lobe_stats(RenderArgs a, uint32_t nslot
     if (!(y >= 0.0f)) y = 0.0f;
     y = fminf(y, kLobeYMax);
     const uint32_t prim = __float_as_uint(r4.x), code = __float_as_uint(r4.y);
-    if (prim >= nprims || (code >> 4) >= kLobeBuckets || (code & 15u) >= 4u) return;  // not this scene's
-    const size_t cell = ((size_t)prim * kLobeBuckets + (code >> 4)) * 4 + (code & 15u);
-    atomicAdd(a.gd.lobe_sum + cell, (unsigned long long)(y * kLobeFixed));
-    atomicAdd(a.gd.lobe_cnt + cell, 1u);
+    const uint32_t lobe = code & 15u, bucket = (code >> 4) & 15u, oct = code >> 8;
+    if (prim >= nprims || bucket >= kLobeBuckets || lobe >= 4u || oct >= 8u) return;  // not this scene's
+    const size_t base = ((size_t)prim * kLobeBuckets + bucket) * kLobeSub;
+    const unsigned long long q = (unsigned long long)(y * kLobeFixed);
+    atomicAdd(a.gd.lobe_sum + base * 4 + lobe, q);  // the primitive's cell
+    atomicAdd(a.gd.lobe_cnt + base * 4 + lobe, 1u);
+    atomicAdd(a.gd.lobe_sum + (base + 1 + oct) * 4 + lobe, q);  // the normal octant's
+    atomicAdd(a.gd.lobe_cnt + (base + 1 + oct) * 4 + lobe, 1u);
 }
 
-// Choice probabilities per (primitive, incoming bucket) from the sums:
-// c_k = mix / n + (1 - mix) m_k / sum m with m_k the mean of lobe k's
-// contributions, once every lobe of the cell has kLobeMinCount samples
-// (x = -1 until then: choose_lobe falls back to the albedo rule).
+// Choice probabilities per cell from the sums: c_k = mix / n + (1 - mix) m_k /
+// sum m with m_k the mean of lobe k's contributions, from the normal
+// octant's own sums once each lobe there has kLobeMinCount samples, else from
+// its primitive's cell once that has them (x = -1 until then: choose_lobe
+// falls back to the albedo rule).
+__device__ __forceinline__ bool lobe_probs(const unsigned long long* sum, const uint32_t* cnt, size_t cell,
+                                           uint32_t n, float mix, float4* out, float* m_out) {
+    float mk_[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t nk = cnt[cell * 4 + k];
+        if (nk < kLobeMinCount) return false;
+        mk_[k] = (float)((double)sum[cell * 4 + k] / (double)nk);
+        tot += mk_[k];
+    }
+    if (!(tot > 0.0f)) return false;
+    float cc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < n; ++k) cc[k] = mix / (float)n + (1.0f - mix) * (mk_[k] / tot);
+    *out = make_float4(cc[0], cc[1], cc[2], cc[3]);
+    *m_out = tot / ((float)n * kLobeFixed);
+    return true;
+}
 __global__ __launch_bounds__(256) void k_lobe_table(const unsigned long long* sum, const uint32_t* cnt,
                                                     const GpuMaterial* mats, uint32_t ncells, float mix,
-                                                    float4* c) {
+                                                    float4* c, float* m) {
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= ncells) return;
-    const uint32_t n = mats[cell / kLobeBuckets].nlobes;
+    const uint32_t n = mats[cell / kLobeCells].nlobes;
+    const uint32_t sub = cell % kLobeSub;
     float4 out = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
-    if (n >= 2 && n <= 4) {
-        float mk_[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
-        bool ok = true;
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t nk = cnt[(size_t)cell * 4 + k];
-            if (nk < kLobeMinCount) ok = false;
-            mk_[k] = nk ? (float)((double)sum[(size_t)cell * 4 + k] / (double)nk) : 0.0f;
-            tot += mk_[k];
-        }
-        if (ok && tot > 0.0f) {
-            float cc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (uint32_t k = 0; k < n; ++k) cc[k] = mix / (float)n + (1.0f - mix) * (mk_[k] / tot);
-            out = make_float4(cc[0], cc[1], cc[2], cc[3]);
-        }
-    }
+    float mo = -1.0f;
+    if (n >= 2 && n <= 4 && !lobe_probs(sum, cnt, cell, n, mix, &out, &mo) && sub != 0)
+        (void)lobe_probs(sum, cnt, cell - sub, n, mix, &out, &mo);
     c[cell] = out;
+    m[cell] = mo;
 }
 
 __global__ __launch_bounds__(256) void k_export_samples(RenderArgs a, float4* out) {
@@ -2153,9 +2236,9 @@ namespace {
 hipError_t reserve_lobe_table(gr_guide* g, const gr_scene* sc, hipStream_t s) {
     const size_t np = sc->prims.size();
     if (g->lobe_scene == sc && g->lobe_prims == np) return hipSuccess;
-    const size_t cells = np * kLobeBuckets;
+    const size_t cells = np * kLobeCells;
     const size_t b_sum = ((cells * 4 * 8) + 255) & ~(size_t)255, b_cnt = ((cells * 4 * 4) + 255) & ~(size_t)255;
-    const size_t b_c = cells * sizeof(float4);
+    const size_t b_c = cells * sizeof(float4) + cells * sizeof(float);
     if (np > g->lobe_prims || !g->lobe_blob) {
         (void)hipFree(g->lobe_blob);
         g->lobe_blob = nullptr;
@@ -2168,6 +2251,7 @@ hipError_t reserve_lobe_table(gr_guide* g, const gr_scene* sc, hipStream_t s) {
     g->lobe_sum = reinterpret_cast<unsigned long long*>(b);
     g->lobe_cnt = reinterpret_cast<uint32_t*>(b + b_sum);
     g->lobe_c = reinterpret_cast<float4*>(b + b_sum + b_cnt);
+    g->lobe_m = reinterpret_cast<float*>(b + b_sum + b_cnt + cells * sizeof(float4));
     hipError_t e = hipMemsetAsync(b, 0, b_sum + b_cnt, s);
     if (e == hipSuccess) e = hipMemsetAsync(g->lobe_c, 0xff, b_c, s);  // NaN: no probabilities yet
     if (e != hipSuccess) return e;
@@ -2262,12 +2346,14 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         a.gd.lobes = gp.lobes;
         a.gd.select = gp.lobe_select;
         a.gd.select_mix = gp.select_mix;
+        a.gd.rr = gp.rr_threshold;
         if (gp.lobe_select == GR_GUIDE_SELECT_LEARNED) {
             e = reserve_lobe_table(guide, sc, s);
             if (e != hipSuccess) return e;
             a.gd.lobe_sum = guide->lobe_sum;
             a.gd.lobe_cnt = guide->lobe_cnt;
             a.gd.lobe_c = guide->lobe_c;
+            a.gd.lobe_m = guide->lobe_m;
         }
         a.gd.alpha = gp.alpha;
         for (int k = 0; k < 3; ++k) {
@@ -2362,10 +2448,10 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
             // train on this batch's records before the next batch samples the guide
             const dim3 rb = blocks_for(nslots);
             if (a.gd.select == GR_GUIDE_SELECT_LEARNED) {
-                const uint32_t cells = (uint32_t)(guide->lobe_prims * kLobeBuckets);
+                const uint32_t cells = (uint32_t)(guide->lobe_prims * kLobeCells);
                 hipLaunchKernelGGL(k_lobe_stats, rb, dim3(256), 0, s, a, nslots, (uint32_t)guide->lobe_prims);
                 hipLaunchKernelGGL(k_lobe_table, blocks_for(cells), dim3(256), 0, s, guide->lobe_sum, guide->lobe_cnt,
-                                   sc->shade.mats, cells, a.gd.select_mix, guide->lobe_c);
+                                   sc->shade.mats, cells, a.gd.select_mix, guide->lobe_c, guide->lobe_m);
             }
             e = guide_zero_stats(guide, s);
             if (e != hipSuccess) break;

@@ -424,47 +424,44 @@ struct LobeSample {
     float pdf;
 };
 
-__device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool c2l, float u1, float u2) {
-    LobeSample s{mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), 0.0f};
+// The direction lobe l's sample_f draws from i with (u1, u2), zero when the
+// draw fails; *h receives the reflection lobes' sampled half vector.
+__device__ __forceinline__ vec3 lobe_draw(const GpuLobe& l, vec3 i, float u1, float u2, vec3* h) {
+    const vec3 zero = mk(0.0f, 0.0f, 0.0f);
     switch (l.type) {
         case kLobeDisneySpecular:
         case kLobeClearcoat: {
-            if (i.z == 0.0f) return s;
-            const vec3 h = l.type == kLobeClearcoat ? gtr1_sample_half(l.v[0], i, u1, u2)
-                                                    : ggx_sample_half(l.v[0], l.v[1], i, u1, u2);
-            const vec3 o = reflect(i, h);
-            if (!same_hemi(i, o)) return s;
-            s.result = lobe_f(l, i, o, c2l);
-            s.outgoing = o;
-            s.pdf = refl_pdf_h(l, i, h) / (4.0f * dot(i, h));
-            return s;
+            if (i.z == 0.0f) return zero;
+            *h = l.type == kLobeClearcoat ? gtr1_sample_half(l.v[0], i, u1, u2)
+                                          : ggx_sample_half(l.v[0], l.v[1], i, u1, u2);
+            const vec3 o = reflect(i, *h);
+            return same_hemi(i, o) ? o : zero;
         }
         case kLobeSpecularTrans: {
-            if (i.z == 0.0f) return s;
-            const vec3 h = ggx_sample_half(l.v[0], l.v[1], i, u1, u2);
+            if (i.z == 0.0f) return zero;
+            const vec3 hh = ggx_sample_half(l.v[0], l.v[1], i, u1, u2);
             const float eta = i.z > 0.0f ? 1.0f / l.v[2] : l.v[2];
-            const vec3 o = refract(i, h, eta);
-            if (is_zero(o)) return s;
-            s.result = lobe_f(l, i, o, c2l);
-            s.outgoing = o;
-            s.pdf = lobe_pdf(l, i, o);
-            return s;
+            return refract(i, hh, eta);
         }
-        case kLobePerfectMirror: {
-            const vec3 o = mk(-i.x, -i.y, i.z);
-            s.result = lobe_f(l, i, o, c2l);
-            s.outgoing = o;
-            s.pdf = 1.0f;
-            return s;
-        }
-        default: {
-            const vec3 o = cosine_hemisphere(u1, u2, i.z < 0.0f);
-            s.result = lobe_f(l, i, o, c2l);
-            s.outgoing = o;
-            s.pdf = lobe_pdf(l, i, o);
-            return s;
-        }
+        case kLobePerfectMirror: return mk(-i.x, -i.y, i.z);
+        default: return cosine_hemisphere(u1, u2, i.z < 0.0f);
     }
+}
+
+__device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool c2l, float u1, float u2) {
+    LobeSample s{mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), 0.0f};
+    vec3 h = s.outgoing;
+    const vec3 o = lobe_draw(l, i, u1, u2, &h);
+    if (is_zero(o) && l.type != kLobeDisneyDiffuse && l.type != kLobePerfectDiffuse) return s;
+    s.result = lobe_f(l, i, o, c2l);
+    s.outgoing = o;
+    switch (l.type) {
+        case kLobeDisneySpecular:
+        case kLobeClearcoat: s.pdf = refl_pdf_h(l, i, h) / (4.0f * dot(i, h)); break;
+        case kLobePerfectMirror: s.pdf = 1.0f; break;
+        default: s.pdf = lobe_pdf(l, i, o); break;
+    }
+    return s;
 }
 
 // lobe_f(l, i, o, c2l) (when want_f) and lobe_pdf(l, i, o) in one evaluation,

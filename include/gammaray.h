@@ -346,6 +346,9 @@ typedef struct {
     uint32_t lobes;           /* GR_GUIDE_LOBES_*: the lobe picks the guide may redirect */
     uint32_t lobe_select;     /* GR_GUIDE_SELECT_*: how a guided trace picks a vertex's lobe */
     float select_mix;         /* (0, 1]: share of the uniform choice in the lobe-choice probabilities */
+    float rr_threshold;       /* >= 0: from the second vertex on, a guided path whose throughput
+                                 luminance is below it continues with probability luminance /
+                                 rr_threshold, its throughput divided by that (0: no such roulette) */
 } gr_guide_params;
 /* Guided lobe sets.  CONSISTENT: Disney diffuse, GGX reflection and perfect
  * diffuse, whose reported pdfs are their sampling densities.  ALL: every lobe

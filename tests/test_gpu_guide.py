@@ -230,23 +230,28 @@ def _sample_radiance(stage, film, integrator, spp, **kw):
                                                   ("clearcoat", "all", "albedo"),
                                                   ("transmission", "all", "albedo"),
                                                   ("transmission", "consistent", "learned"),
-                                                  ("clearcoat", "all", "learned")])
+                                                  ("clearcoat", "all", "learned"),
+                                                  ("transmission", "consistent", "learned+roulette"),
+                                                  ("consistent", "consistent", "uniform+roulette")])
 def test_guided_render_has_the_plain_mean(variant, lobes, select, gpu_ctx):
     """Guided and plain PT+NEE per-pixel means agree within 3 sigma (different
     seeds), with the guide sampled after warm-up -- also where clearcoat and
     transmission lobes report pdfs that are not their sampling densities, and
     with those lobes' picks guided too (lobes="all": weighted with the
     densities their samplers realise and the radiance sample_world reports
-    for each pick, the extra lobe values of transmitted picks included), and
-    with the lobe chosen by albedo rather than uniformly (select="albedo")."""
+    for each pick, the extra lobe values of transmitted picks included), with
+    the lobe chosen by albedo or by learned statistics rather than uniformly,
+    and with the guided integrator's Russian roulette (rr_threshold 0.3, so
+    it fires often on these boxes)."""
     print("case guided unbiased", variant, lobes, select, flush=True)
     spec = {"consistent": lambda: scenes.box_scene(consistent=True),
             "clearcoat": lambda: scenes.box_scene(),
             "transmission": lambda: scenes.box_scene(seed=2, full=True)}[variant]()
     W, H, spp = 32, 32, 256
     stage = Stage(spec, ctx=gpu_ctx)
+    choice, _, rr = select.partition("+")
     g = Guide(gpu_ctx, batch=4096, steps_per_pass=4, train_paths=4096, warmup_steps=8, alpha=0.5, lobes=lobes,
-              lobe_select=select)
+              lobe_select=choice, rr_threshold=0.3 if rr else 0.0)
     film = Film(gpu_ctx, W, H)
     a = _sample_radiance(stage, film, "guided", spp, max_depth=6, seed=31, batch=8, guide=g)
     assert g.steps() == 4 * spp // 8
@@ -333,23 +338,26 @@ def _guide_loss(g, recs):
     return float(-(w * torch.log(q.clamp(min=1e-30))).mean().item()), mix
 
 
-@pytest.mark.parametrize("lobes", ["consistent", "all"])
-def test_guided_full_1080p_scene(lobes, gpu_ctx):
+@pytest.mark.parametrize("setting", ["consistent lobes", "all lobes", "bench"])
+def test_guided_full_1080p_scene(setting, gpu_ctx):
     """Neural guiding on the full 1080p scene (100k-triangle mesh, 200 spheres,
-    PT+NEE depth 8), with the bench's guide settings (alpha 0.3, first vertex,
-    8-spp batches, 8 Adam steps per batch).  Checks:
+    PT+NEE depth 8): alpha 0.3 at the first vertex, 8-spp batches, with either
+    lobe set and the reference's lobe choice, and with the bench's setting
+    (learned lobe choice, roulette 0.1, 4 Adam steps per batch).  Checks:
     * mixtures are finite and normalised;
     * training lowers the loss on the first batch's records, against an
       untrained network;
     * 16 guided spp agree with 16 plain spp (independent seeds) per 16 x 16
       tile within 3 sigma (binomial bound) and in total."""
     import torch
-    print("case guided 1080p", lobes, flush=True)
+    print("case guided 1080p", setting, flush=True)
     spec = scenes.showcase_scene()
     W, H = 1920, 1080
     stage = Stage(spec, ctx=gpu_ctx)
     kw = dict(seed=7, learning_rate=5e-3, alpha=0.3, batch=16384, steps_per_pass=8, train_paths=65536,
-              guided_depth=1, warmup_steps=8, lobes=lobes)
+              guided_depth=1, warmup_steps=8, lobes="all" if setting == "all lobes" else "consistent")
+    if setting == "bench":
+        kw.update(steps_per_pass=4, train_paths=1 << 19, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)
     g = Guide(gpu_ctx, **kw)
     fresh = Guide(gpu_ctx, **kw)  # same initial weights, never trained
     warm = Film(gpu_ctx, W, H)
@@ -371,4 +379,4 @@ def test_guided_full_1080p_scene(lobes, gpu_ctx):
                          batch=8, guide=g)
     b = _sample_radiance(stage, Film(gpu_ctx, W, H), "pt_nee", 16, max_depth=8, seed=54)
     assert np.isfinite(a).all()
-    assert_three_sigma_blocks(a.sum(2), b.sum(2), W, H, 16, "guided 1080p tiles " + lobes)
+    assert_three_sigma_blocks(a.sum(2), b.sum(2), W, H, 16, "guided 1080p tiles " + setting)
