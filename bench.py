@@ -384,29 +384,43 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64, overrides=None):
     # lobe choice and the roulette at every vertex (GUIDED_CONFIG)
     cfg = dict(GUIDED_CONFIG)
     cfg.update(overrides or {})
+
+    def run(c):
+        """8-spp batches to twice the plain spp: (curve rows, wall time to plain's RMSE, to its relMSE)."""
+        g = Guide(ctx, **c)
+        f = Film(ctx, W, H)
+        tot, rows = 0.0, []
+        for k in range(0, 2 * spp, 8):
+            tot += timed(lambda: trace(st, f, "guided", spp=8, max_depth=8, seed=9, sample_offset=k, batch=8,
+                                       guide=g))
+            img = image(f)
+            rows.append((k + 8, round(tot, 4), round(_rmse(img, R), 6), round(_rel_mse_trimmed(img, R), 6)))
+        f.close()
+        g.close()
+
+        def reach(col, target):
+            # wall time at which the curve first reaches `target` (linear between batches)
+            prev = (0.0, float("inf"))
+            for row in rows:
+                if row[col] <= target:
+                    t0, e0 = prev
+                    return t0 + (row[1] - t0) * ((e0 - target) / (e0 - row[col])
+                                                  if np.isfinite(e0) and e0 > row[col] else 1.0)
+                prev = (row[1], row[col])
+            return None
+        return rows, reach(2, e_plain), reach(3, r_plain)
+
     g = Guide(ctx, **cfg)
     infer = guide_inference_rate(ctx, g)
-    f = Film(ctx, W, H)
-    tot, curve = 0.0, []
-    for k in range(0, 2 * spp, 8):
-        tot += timed(lambda: trace(st, f, "guided", spp=8, max_depth=8, seed=9, sample_offset=k, batch=8, guide=g))
-        img = image(f)
-        curve.append((k + 8, round(tot, 4), round(_rmse(img, R), 6), round(_rel_mse_trimmed(img, R), 6)))
-
-    def reach(col, target):
-        # wall time at which the guided curve first reaches `target` (linear between batches)
-        prev = (0.0, float("inf"))
-        for row in curve:
-            if row[col] <= target:
-                t0, e0 = prev
-                return t0 + (row[1] - t0) * ((e0 - target) / (e0 - row[col]) if np.isfinite(e0) and e0 > row[col] else 1.0)
-            prev = (row[1], row[col])
-        return None
-    reached, reached_rel = reach(2, e_plain), reach(3, r_plain)
+    g.close()
+    curve, reached, reached_rel = run(cfg)
     log("guided: plain %d spp %.3f s RMSE %.5f relMSE %.5f; guided reaches the RMSE at %s s, the relMSE at %s s" % (
         spp, t_plain, e_plain, r_plain, "%.3f" % reached if reached else "--", "%.3f" % reached_rel if reached_rel else "--"))
-    f.close()
-    g.close()
+    # the same integrator with the network off: never sampled (alpha 0) and never trained, so it is
+    # never queried either; lobe choice and roulette as above
+    off = dict(cfg, alpha=0.0, steps_per_pass=0)
+    curve_off, _, reached_off = run(off)
+    log("guided without the network: reaches the relMSE at %s s" % ("%.3f" % reached_off if reached_off else "--"))
     return {"scene": "showcase 1920x1080, PT+NEE depth 8", "reference_spp": ref_spp,
             "reference_s": round(t_ref, 3),
             "error": "relMSE with the 0.1% largest pixel errors dropped (the time ratio).  The untrimmed RMSE is "
@@ -419,7 +433,9 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64, overrides=None):
             "inference": infer,
             "time_to_equal_error_s": round(reached_rel, 4) if reached_rel else None,
             "time_ratio": round(reached_rel / t_plain, 3) if reached_rel else None,
-            "time_ratio_rmse_untrimmed": round(reached / t_plain, 3) if reached else None}
+            "time_ratio_rmse_untrimmed": round(reached / t_plain, 3) if reached else None,
+            "without_network": {"config": off, "curve": curve_off,
+                                "time_ratio": round(reached_off / t_plain, 3) if reached_off else None}}
 
 
 def _without_inconsistent_lobes(spec):

@@ -1195,26 +1195,23 @@ __global__ __launch_bounds__(256) void k_lobe_stats(RenderArgs a, uint32_t nslot
     const uint32_t prim = __float_as_uint(r4.x), code = __float_as_uint(r4.y);
     const uint32_t lobe = code & 15u, bucket = (code >> 4) & 15u, oct = code >> 8;
     if (prim >= nprims || bucket >= kLobeBuckets || lobe >= 4u || oct >= 8u) return;  // not this scene's
-    const size_t base = ((size_t)prim * kLobeBuckets + bucket) * kLobeSub;
-    const unsigned long long q = (unsigned long long)(y * kLobeFixed);
-    atomicAdd(a.gd.lobe_sum + base * 4 + lobe, q);  // the primitive's cell
-    atomicAdd(a.gd.lobe_cnt + base * 4 + lobe, 1u);
-    atomicAdd(a.gd.lobe_sum + (base + 1 + oct) * 4 + lobe, q);  // the normal octant's
-    atomicAdd(a.gd.lobe_cnt + (base + 1 + oct) * 4 + lobe, 1u);
+    // the normal octant's cell (its primitive's cell is the sum of its 8 octants, k_lobe_table)
+    const size_t cell = ((size_t)prim * kLobeBuckets + bucket) * kLobeSub + 1 + oct;
+    atomicAdd(a.gd.lobe_sum + cell * 4 + lobe, (unsigned long long)(y * kLobeFixed));
+    atomicAdd(a.gd.lobe_cnt + cell * 4 + lobe, 1u);
 }
 
 // Choice probabilities per cell from the sums: c_k = mix / n + (1 - mix) m_k /
 // sum m with m_k the mean of lobe k's contributions, from the normal
 // octant's own sums once each lobe there has kLobeMinCount samples, else from
-// its primitive's cell once that has them (x = -1 until then: choose_lobe
-// falls back to the albedo rule).
-__device__ __forceinline__ bool lobe_probs(const unsigned long long* sum, const uint32_t* cnt, size_t cell,
-                                           uint32_t n, float mix, float4* out, float* m_out) {
+// its primitive's (the sums of its 8 octants) once that has them (x = -1
+// until then: choose_lobe falls back to the albedo rule).
+__device__ __forceinline__ bool lobe_probs(const unsigned long long* sum, const uint32_t* cnt, uint32_t n,
+                                           float mix, float4* out, float* m_out) {
     float mk_[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
     for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t nk = cnt[cell * 4 + k];
-        if (nk < kLobeMinCount) return false;
-        mk_[k] = (float)((double)sum[cell * 4 + k] / (double)nk);
+        if (cnt[k] < kLobeMinCount) return false;
+        mk_[k] = (float)((double)sum[k] / (double)cnt[k]);
         tot += mk_[k];
     }
     if (!(tot > 0.0f)) return false;
@@ -1233,8 +1230,17 @@ __global__ __launch_bounds__(256) void k_lobe_table(const unsigned long long* su
     const uint32_t sub = cell % kLobeSub;
     float4 out = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
     float mo = -1.0f;
-    if (n >= 2 && n <= 4 && !lobe_probs(sum, cnt, cell, n, mix, &out, &mo) && sub != 0)
-        (void)lobe_probs(sum, cnt, cell - sub, n, mix, &out, &mo);
+    if (sub != 0 && n >= 2 && n <= 4 && !lobe_probs(sum + (size_t)cell * 4, cnt + (size_t)cell * 4, n, mix, &out, &mo)) {
+        unsigned long long ps[4] = {0ull, 0ull, 0ull, 0ull};
+        uint32_t pc[4] = {0u, 0u, 0u, 0u};
+        const size_t first = (size_t)(cell - sub + 1) * 4;
+        for (uint32_t o = 0; o < 8; ++o)
+            for (uint32_t k = 0; k < n; ++k) {
+                ps[k] += sum[first + o * 4 + k];
+                pc[k] += cnt[first + o * 4 + k];
+            }
+        (void)lobe_probs(ps, pc, n, mix, &out, &mo);
+    }
     c[cell] = out;
     m[cell] = mo;
 }
