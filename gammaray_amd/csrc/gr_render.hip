@@ -1046,8 +1046,8 @@ __global__ __launch_bounds__(256) void k_shadow_accum(RenderArgs a, uint32_t dep
     }
 }
 
-// Guide inputs of the vertices the guide covers (a material with a consistent
-// lobe, depth < guided_depth): queries when the guide is sampled, vertex
+// Guide inputs of the vertices the guide covers (depth < guided_depth, a lobe
+// choice in the guided set): queries when the guide is sampled, vertex
 // records for training paths.
 __global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t depth, uint32_t cur) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1062,10 +1062,14 @@ __global__ __launch_bounds__(256) void k_guide_query(RenderArgs a, uint32_t dept
         if (h.prim != kNone && (a.gd.guided_depth == 0 || depth < a.gd.guided_depth)) {
             const GpuMaterial& m = a.sh.mats[h.prim];
             if (guided_lobes(m, a.gd.lobes) != 0) {
-                eligible = true;
                 const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
                 const Frame f = hit_frame(a.sv, a.sh, h, o, d);
-                guide_raw_inputs(o + d * h.t, -d, f.n, m.roughness, a.gd.lo, a.gd.inv_ext, x);
+                // the lobe choice the shading kernel makes with the same draw and
+                // table: only a pick in the guided set uses the mixture
+                const uint4 r0 = draws(a.seed, a.path_base + p, kStreamCamera, depth, 0);
+                const LobeChoice ch = choose_lobe(m, dev::to_local(-d, f), f.n, a.gd, r0.x, h.prim);
+                eligible = lobe_guided(m.lobes[ch.pick], a.gd.lobes);
+                if (eligible) guide_raw_inputs(o + d * h.t, -d, f.n, m.roughness, a.gd.lo, a.gd.inv_ext, x);
             }
         }
     }
@@ -2409,8 +2413,10 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         }
         for (uint32_t depth = 0; prm.integrator != GR_INTEGRATOR_DISPLAY && (prm.max_depth == 0 || depth < prm.max_depth);
              ++depth) {
-            if (prm.max_depth == 0) {
-                // Unbounded paths (the reference's loop): stop once no path is live.
+            if (prm.max_depth == 0 && depth > 0 && depth % 8 == 0) {
+                // Unbounded paths (the reference's loop): stop once no path is live,
+                // checked every 8 bounces (a bounce with no live path launches
+                // kernels that find their counts zero)
                 uint32_t live = 0;
                 e = hipMemcpyAsync(&live, film->rb.counts + cur, 4, hipMemcpyDeviceToHost, s);
                 if (e == hipSuccess) e = hipStreamSynchronize(s);
