@@ -2338,7 +2338,10 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
     if (guided) {
         const gr_guide_params& gp = guide->prm;
         const uint32_t drec = prm.max_depth == 0 ? 16u : std::min(prm.max_depth, 16u);
-        nslots = gp.train_paths * drec;
+        // a batch has at most `paths` training paths, whatever the guide asks for
+        const uint64_t train = std::min<uint64_t>(gp.train_paths, paths);
+        if (train * drec > 0x7fffffffull) return hipErrorInvalidValue;
+        nslots = (uint32_t)(train * drec);
         e = reserve_guide(film->rb, paths, nslots);
         if (e == hipSuccess) e = guide_reserve_records(guide, nslots);
         if (e != hipSuccess) return e;
@@ -2350,7 +2353,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         a.gd.recs = guide->recs;
         a.gd.rec_count = guide->count;
         a.gd.rec_cap = (uint32_t)guide->recs_cap;
-        a.gd.train_paths = gp.train_paths;
+        a.gd.train_paths = (uint32_t)train;
         a.gd.drec = drec;
         a.gd.guided_depth = gp.guided_depth;
         a.gd.lobes = gp.lobes;

@@ -380,3 +380,58 @@ def test_guided_full_1080p_scene(setting, gpu_ctx):
     b = _sample_radiance(stage, Film(gpu_ctx, W, H), "pt_nee", 16, max_depth=8, seed=54)
     assert np.isfinite(a).all()
     assert_three_sigma_blocks(a.sum(2), b.sum(2), W, H, 16, "guided 1080p tiles " + setting)
+
+
+def _rel_mse_trimmed(img, ref, trim=1e-3):
+    d = ((img - ref) ** 2 / (ref ** 2 + 1e-2)).ravel()
+    d = d[np.isfinite(d)]
+    return float(np.sort(d)[: int(len(d) * (1 - trim))].mean())
+
+
+def test_guided_gains_at_equal_time(gpu_ctx):
+    """The equal-time gain of guided rendering on the 1080p scene's geometry
+    and camera at 960 x 540: 64 plain spp and 64 guided spp (the bench's guide
+    setting, trained while rendering; both in 8-spp batches) against a 1024-spp
+    plain render, error = relative MSE with the 0.1 % largest pixel errors
+    dropped (the bench's measure).  Guided error times guided time must be
+    below 0.8 of plain's (the bench measures about 0.5 at 1080p)."""
+    import time
+    print("case guided equal time", flush=True)
+    spec = scenes.showcase_scene()
+    W, H = 960, 540
+    stage = Stage(spec, ctx=gpu_ctx)
+
+    def image(f):
+        px = f.readback()
+        return px[..., :3] / px[..., 3:4]
+
+    def timed(fn):
+        gpu_ctx.synchronize()
+        t = time.perf_counter()
+        fn()
+        gpu_ctx.synchronize()
+        return time.perf_counter() - t
+
+    ref = Film(gpu_ctx, W, H)
+    trace(stage, ref, "pt_nee", spp=1024, max_depth=8, seed=1000)
+    R = image(ref)
+    warm = Film(gpu_ctx, W, H)
+    trace(stage, warm, "pt_nee", spp=8, max_depth=8, seed=5)
+    plain = Film(gpu_ctx, W, H)
+    # both in 8-spp batches (the guide trains between batches; at 1080p plain's
+    # default batch is 8 spp too)
+    t_plain = timed(lambda: trace(stage, plain, "pt_nee", spp=64, max_depth=8, seed=7, batch=8))
+    e_plain = _rel_mse_trimmed(image(plain), R)
+    g = Guide(gpu_ctx, alpha=0.3, guided_depth=1, steps_per_pass=4, batch=16384, train_paths=1 << 19,
+              warmup_steps=8, learning_rate=5e-3, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)
+    guided = Film(gpu_ctx, W, H)
+    t_guided = 0.0
+    for k in range(0, 64, 8):
+        t_guided += timed(lambda: trace(stage, guided, "guided", spp=8, max_depth=8, seed=9, sample_offset=k, batch=8,
+                                        guide=g))
+    e_guided = _rel_mse_trimmed(image(guided), R)
+    ratio = (e_guided * t_guided) / (e_plain * t_plain)
+    print("plain %.5f in %.4f s, guided %.5f in %.4f s: error x time ratio %.3f"
+          % (e_plain, t_plain, e_guided, t_guided, ratio), flush=True)
+    assert e_guided < 0.85 * e_plain, (e_guided, e_plain)
+    assert ratio < 0.8, ratio
