@@ -234,9 +234,10 @@ def bench_batch(ctx, threads, pool=1 << 23, steps=16):
     specs, jobs = scenes.batch_workload(seed=4)
     stages = [Stage(sp, ctx=ctx) for sp in specs]
     log("batch: 8 scenes %s triangles built (%.1f s)" % ([sp.num_components for sp in specs], time.time() - t0))
-    # the batch's launches trace closest-hit rays and NEE shadow segments: tune on both
-    rays = torch.from_numpy(scenes.ray_batch(specs[-1], 1 << 21, seed=9)).cuda()
-    segs = torch.from_numpy(scenes.segment_batch(specs[-1], 1 << 21, seed=10)).cuda()
+    # the batch's launches trace bounce rays and NEE shadow segments: tune on rays of both
+    # kinds in the largest scene (leaving its surfaces, and from them to its light)
+    rays = torch.from_numpy(scenes.bounce_batch(specs[-1], 1 << 21, seed=9)).cuda()
+    segs = torch.from_numpy(scenes.light_segment_batch(specs[-1], 1 << 21, seed=10)).cuda()
     tuning = stages[-1].autotune(rays, segs)
     del rays, segs
     for st in stages:

@@ -441,6 +441,67 @@ def segment_batch(spec, n, seed, chunk=1 << 22):
     return out
 
 
+def _surface_points(spec, rng, n):
+    """n points on the scene's non-emitting mesh triangles (uniform over the
+    triangles, then uniform in each), with each triangle's unit normal turned
+    to +y, from above which the scenes' cameras and lights see them."""
+    tris = [(p.vertices[p.tris[:, 0]], p.vertices[p.tris[:, 1]], p.vertices[p.tris[:, 2]])
+            for p in spec.prims if p.kind == "mesh" and p.material[0] != 1]
+    a = np.concatenate([t[0] for t in tris]).astype(np.float64)
+    b = np.concatenate([t[1] for t in tris]).astype(np.float64)
+    c = np.concatenate([t[2] for t in tris]).astype(np.float64)
+    k = rng.integers(0, len(a), n)
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1.0
+    u[flip], v[flip] = 1.0 - u[flip], 1.0 - v[flip]
+    pts = a[k] + (b[k] - a[k]) * u[:, None] + (c[k] - a[k]) * v[:, None]
+    nrm = np.cross(b[k] - a[k], c[k] - a[k])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    nrm[nrm[:, 1] < 0] *= -1.0
+    return pts, nrm
+
+
+def bounce_batch(spec, n, seed):
+    """(n, 8) float32 rays like a path tracer's bounces off the scene's
+    surfaces: origins on its mesh triangles, lifted 1e-4 along the normal,
+    directions cosine-distributed about the normal."""
+    rng = np.random.default_rng(seed)
+    pts, nrm = _surface_points(spec, rng, n)
+    r, phi = np.sqrt(rng.random(n)), 2.0 * math.pi * rng.random(n)
+    local = np.stack([r * np.cos(phi), np.sqrt(np.maximum(0.0, 1.0 - r * r)), r * np.sin(phi)], 1)
+    # rotate +y onto the normal (Rodrigues; the normals point up, so y + 1 > 0)
+    x, y, z = nrm[:, 0:1], nrm[:, 1:2], nrm[:, 2:3]
+    k = 1.0 / (1.0 + y)
+    lx, ly, lz = local[:, 0:1], local[:, 1:2], local[:, 2:3]
+    d = np.concatenate([lx * (y + z * z * k) + ly * x - lz * x * z * k,
+                        -lx * x + ly * y - lz * z,
+                        -lx * x * z * k + ly * z + lz * (y + x * x * k)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out = np.zeros((n, 8), np.float32)
+    out[:, 0:3] = pts + 1e-4 * nrm
+    out[:, 4:7] = d
+    return out
+
+
+def light_segment_batch(spec, n, seed):
+    """(n, 8) float32 shadow segments like next-event estimation's: from points
+    on the scene's mesh triangles to uniform points of its emitting meshes."""
+    rng = np.random.default_rng(seed)
+    pts, nrm = _surface_points(spec, rng, n)
+    lights = [p for p in spec.prims if p.kind == "mesh" and p.material[0] == 1]
+    la = np.concatenate([p.vertices[p.tris[:, 0]] for p in lights]).astype(np.float64)
+    lb = np.concatenate([p.vertices[p.tris[:, 1]] for p in lights]).astype(np.float64)
+    lc = np.concatenate([p.vertices[p.tris[:, 2]] for p in lights]).astype(np.float64)
+    k = rng.integers(0, len(la), n)
+    u, v = rng.random(n), rng.random(n)
+    flip = u + v > 1.0
+    u[flip], v[flip] = 1.0 - u[flip], 1.0 - v[flip]
+    out = np.zeros((n, 8), np.float32)
+    out[:, 0:3] = pts + 1e-4 * nrm
+    out[:, 4:7] = la[k] + (lb[k] - la[k]) * u[:, None] + (lc[k] - la[k]) * v[:, None]
+    return out
+
+
 def rays6(r8):
     """Drop the padding lanes: (n, 8) -> (n, 6)."""
     r8 = np.asarray(r8)
