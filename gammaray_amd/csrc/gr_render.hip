@@ -1278,10 +1278,14 @@ constexpr int kTile = 16, kApron = 4, kSpan = kTile + 2 * kApron;
 // Film::report_samples as a gather (film.rs:78-114): pixel (x, y) visits the
 // samples of pixels y-4..y+4, x-4..x+4 in row-major order -- the order the
 // serial splat reaches it -- and applies each sample's own footprint test.
-// A sample's continuous position and footprint (the pixel range its splat
-// covers) are computed once, when the tile stages it in LDS, with the same
-// arithmetic the splat uses; each pixel then only compares its coordinates
-// with the staged bounds and weights the samples that cover it.
+// A sample's footprint (the pixel range its splat covers, at most 5 x 5) and
+// the Mitchell factors of its columns and rows are computed once, when the
+// tile stages it in LDS, with the same arithmetic the splat uses; each pixel
+// then takes its weight as the product of its column's and row's factors.
+// The visit is branch-free: a sample that does not cover the pixel (or lies
+// outside the film) adds +0 through a select, which leaves the sum's bits
+// unchanged (the sums start at +0 and never become -0), so every wave runs
+// the 81 visits at full width instead of diverging on each.
 // Light-traced splats are summed as 64-bit fixed point (units of 2^-32) so
 // that the per-pixel sums do not depend on the order in which the atomics
 // land: a batch's image is the same on every run.  Range: |sum| < 2^31 per
@@ -1290,9 +1294,16 @@ constexpr double kSplatScale = 4294967296.0;
 
 __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, float4* film, uint32_t iters,
                                                               const long long* splat, const float* ew) {
-    __shared__ float s_cd[kSpan * kSpan], s_rd[kSpan * kSpan], s_r[kSpan * kSpan], s_g[kSpan * kSpan],
-        s_b[kSpan * kSpan];
-    __shared__ int4 s_box[kSpan * kSpan];  // min_col, max_col, min_row, max_row
+    constexpr int kSpanN = kSpan * kSpan;
+    // per staged sample: radiance; its footprint as four bytes, first and
+    // last column and first and last row relative to the tile's corner less
+    // kApron + kFoot (empty outside the film); and the footprint's Mitchell
+    // factors by column and by row (stride 5: odd, so lanes reading
+    // consecutive samples hit distinct banks)
+    __shared__ float s_r[kSpanN], s_g[kSpanN], s_b[kSpanN];
+    __shared__ uint32_t s_box[kSpanN];
+    __shared__ float s_wx[kSpanN * 5], s_wy[kSpanN * 5];
+    constexpr int kFoot = 8;  // footprint columns reach 2 + 2 beyond the apron
     const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
     const int x = x0 + tx, y = y0 + ty;
@@ -1302,48 +1313,85 @@ __global__ __launch_bounds__(kTile* kTile) void k_film_gather(RenderArgs a, floa
     const float wf = (float)a.width, hf = (float)a.height;
     const int last_col = W - 1, last_row = H - 1;
     const uint32_t npx = a.width * a.height;
-    for (uint32_t it = 0; it < iters; ++it) {
+    // Each thread stages up to kPer samples per iteration; the next
+    // iteration's are fetched into registers while this one is gathered.
+    constexpr int kThreads = kTile * kTile, kPer = (kSpanN + kThreads - 1) / kThreads;
+    float4 pL[kPer];
+    float2 pst[kPer];
+    auto fetch = [&](uint32_t it) {
         const uint64_t it0 = a.ring_cap ? (a.ring_base + (uint64_t)it * npx) % a.ring_cap : (uint64_t)it * npx;
-        for (int k = threadIdx.x; k < kSpan * kSpan; k += kTile * kTile) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int k = (int)threadIdx.x + j * kThreads;
             const int sx = x0 - kApron + k % kSpan, sy = y0 - kApron + k / kSpan;
-            if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+            if (k < kSpanN && sx >= 0 && sx < W && sy >= 0 && sy < H) {
                 uint64_t q = it0 + (uint64_t)sy * W + sx;
                 if (a.ring_cap && q >= a.ring_cap) q -= a.ring_cap;  // q < npx + ring_cap <= 2 ring_cap
-                const size_t p = (size_t)q;
-                const float2 st = a.rb.st[p];
-                const float4 L = a.rb.L[p];
+                pst[j] = a.rb.st[(size_t)q];
+                pL[j] = a.rb.L[(size_t)q];
+            }
+        }
+    };
+    if (iters) fetch(0);
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int k = (int)threadIdx.x + j * kThreads;
+            if (k >= kSpanN) continue;
+            const int sx = x0 - kApron + k % kSpan, sy = y0 - kApron + k / kSpan;
+            if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                const float2 st = pst[j];
+                const float4 L = pL[j];
                 const float col_cont = 0.0f + (0.5f * (st.x + 1.0f)) * (wf - 0.0f);
                 const float row_cont = 0.0f + (0.5f * (st.y + 1.0f)) * (hf - 0.0f);
                 const float cd = col_cont - 0.5f, rd = row_cont - 0.5f;
-                s_cd[k] = cd;
-                s_rd[k] = rd;
-                s_box[k] = make_int4(clampi(f2i(ceilf(cd - 2.0f)), 0, last_col),
-                                     clampi(f2i(floorf(cd + 2.0f)), 0, last_col),
-                                     clampi(f2i(ceilf(rd - 2.0f)), 0, last_row),
-                                     clampi(f2i(floorf(rd + 2.0f)), 0, last_row));
+                const int c0 = clampi(f2i(ceilf(cd - 2.0f)), 0, last_col);
+                const int r0 = clampi(f2i(ceilf(rd - 2.0f)), 0, last_row);
+                const int c1 = clampi(f2i(floorf(cd + 2.0f)), 0, last_col);
+                const int r1 = clampi(f2i(floorf(rd + 2.0f)), 0, last_row);
+                // the footprint of a sample staged for this tile lies within
+                // kFoot of the apron (its pixel is, and the footprint spans
+                // +-4 around it; a NaN position gives column / row 0, which
+                // covers no pixel of a tile that far and is packed as empty)
+                const int bx = x0 - kApron - kFoot, by = y0 - kApron - kFoot;
+                const bool fits = c0 - bx >= 0 && c1 - bx < 256 && r0 - by >= 0 && r1 - by < 256;
+                s_box[k] = fits ? (uint32_t)(c0 - bx) | (uint32_t)(c1 - bx) << 8 | (uint32_t)(r0 - by) << 16 |
+                                      (uint32_t)(r1 - by) << 24
+                                : 0x00010001u;  // first 1, last 0: covers no pixel
+                for (int i = 0; i < 5; ++i) {
+                    s_wx[k * 5 + i] = mitchell1(((float)(c0 + i) - cd) / 2.0f);
+                    s_wy[k * 5 + i] = mitchell1(((float)(r0 + i) - rd) / 2.0f);
+                }
                 s_r[k] = L.x;
                 s_g[k] = L.y;
                 s_b[k] = L.z;
                 // counted once, by the tile that holds the sample's own pixel
                 const bool home = sx >= x0 && sx < x0 + kTile && sy >= y0 && sy < y0 + kTile;
                 note_stat(a.stats, kStatNonfiniteRadiance, home && !finite3(mk(L.x, L.y, L.z)));
+            } else {
+                s_box[k] = 0x00010001u;  // covers no pixel
             }
         }
         __syncthreads();
+        if (it + 1 < iters) fetch(it + 1);
         if (inside) {
-            for (int sy = y - kApron; sy <= y + kApron; ++sy) {
-                if (sy < 0 || sy >= H) continue;
-                for (int sx = x - kApron; sx <= x + kApron; ++sx) {
-                    if (sx < 0 || sx >= W) continue;
-                    const int k = (sy - (y0 - kApron)) * kSpan + (sx - (x0 - kApron));
-                    const int4 bx = s_box[k];
-                    if (x < bx.x || x > bx.y || y < bx.z || y > bx.w) continue;
-                    const float cd = s_cd[k], rd = s_rd[k];
-                    const float wt = mitchell1(((float)x - cd) / 2.0f) * mitchell1(((float)y - rd) / 2.0f);
-                    acc.x = acc.x + s_r[k] * wt;
-                    acc.y = acc.y + s_g[k] * wt;
-                    acc.z = acc.z + s_b[k] * wt;
-                    acc.w += wt;
+            const int kb = (y - y0) * kSpan + (x - x0);
+            // this pixel relative to the footprints' origin
+            const int px = x - (x0 - kApron - kFoot), py = y - (y0 - kApron - kFoot);
+#pragma unroll 1
+            for (int dy = 0; dy < 2 * kApron + 1; ++dy) {
+#pragma unroll
+                for (int dx = 0; dx < 2 * kApron + 1; ++dx) {
+                    const int k = kb + dy * kSpan + dx;
+                    const uint32_t bx = s_box[k];
+                    const int jx = px - (int)(bx & 0xffu), jy = py - (int)((bx >> 16) & 0xffu);
+                    const bool cov = jx >= 0 && px <= (int)((bx >> 8) & 0xffu) && jy >= 0 && py <= (int)(bx >> 24);
+                    const int ix = cov ? jx : 0, iy = cov ? jy : 0;
+                    const float wt = s_wx[k * 5 + ix] * s_wy[k * 5 + iy];
+                    acc.x = acc.x + (cov ? s_r[k] * wt : 0.0f);
+                    acc.y = acc.y + (cov ? s_g[k] * wt : 0.0f);
+                    acc.z = acc.z + (cov ? s_b[k] * wt : 0.0f);
+                    acc.w = acc.w + (cov ? wt : 0.0f);
                 }
             }
         }
