@@ -229,16 +229,22 @@ def bench_batch(ctx, threads, pool=1 << 23, steps=16):
     latency, next to the same jobs rendered one after another with gr_trace."""
     import torch
     from gammaray_amd import scenes
-    from gammaray_amd.api import Film, Stage, render_jobs, trace
+    from gammaray_amd.api import Film, Stage, autotune_jobs, render_jobs, trace
     t0 = time.time()
     specs, jobs = scenes.batch_workload(seed=4)
     stages = [Stage(sp, ctx=ctx) for sp in specs]
     log("batch: 8 scenes %s triangles built (%.1f s)" % ([sp.num_components for sp in specs], time.time() - t0))
-    # the batch's launches trace bounce rays and NEE shadow segments: tune on rays of both
-    # kinds in the largest scene (leaving its surfaces, and from them to its light)
-    rays = torch.from_numpy(scenes.bounce_batch(specs[-1], 1 << 21, seed=9)).cuda()
-    segs = torch.from_numpy(scenes.light_segment_batch(specs[-1], 1 << 21, seed=10)).cuda()
-    tuning = stages[-1].autotune(rays, segs)
+    # the batch traces bounce rays and NEE shadow segments in every scene at once: tune its
+    # multi-scene traversal on rays of both kinds in each scene (leaving its surfaces, and from
+    # them to its light), each naming its scene as the batch's own do
+    per = 1 << 18
+    rays = np.concatenate([scenes.bounce_batch(sp, per, seed=9 + k) for k, sp in enumerate(specs)])
+    segs = np.concatenate([scenes.light_segment_batch(sp, per, seed=100 + k) for k, sp in enumerate(specs)])
+    tag = np.repeat(np.arange(len(specs), dtype=np.uint32), per).view(np.float32)
+    rays[:, 3] = tag
+    segs[:, 3] = tag
+    rays, segs = torch.from_numpy(rays).cuda(), torch.from_numpy(segs).cuda()
+    tuning = autotune_jobs(ctx, stages, rays, segs)
     del rays, segs
     for st in stages:
         st.set_tuning(**tuning)

@@ -136,6 +136,7 @@ SIGNATURES = {
     "gr_scene_get_tuning": (_i, [_vp, _vp]),
     "gr_scene_autotune": (_i, [_vp, _vp, _u32, _vp, _vp]),
     "gr_scene_autotune_mixed": (_i, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _i, _vp]),
+    "gr_jobs_autotune": (_i, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp]),
 }
 
 _lib = None

@@ -431,6 +431,24 @@ def trace(stage, film, integrator="pt", spp=1, max_depth=0, light_depth=0, seed=
 JOB_ORDERS = {"as_given": _lib.GR_JOBS_AS_GIVEN, "fewest_first": _lib.GR_JOBS_FEWEST_FIRST}
 
 
+def autotune_jobs(ctx, stages, rays, segments):
+    """gr_jobs_autotune: the traversal setting for render_jobs over `stages`,
+    timed with the batch's multi-scene traversal on `rays` and `segments`
+    ((n, 8) float32 device tensors whose pad word after the origin holds the
+    stage's index as uint32 bits).  Returns the setting (pass it as
+    render_jobs' tuning)."""
+    import torch
+    n, m = rays.shape[0], segments.shape[0]
+    hits = torch.empty((max(n, 1), 4), dtype=torch.int32, device=rays.device)
+    vis = torch.empty((max(m, 1),), dtype=torch.int32, device=rays.device)
+    arr = (ctypes.c_void_p * len(stages))(*[st.handle for st in stages])
+    t = _lib.Tuning()
+    check(lib().gr_jobs_autotune(ctx.handle, arr, len(stages), ctypes.c_void_p(rays.data_ptr()), n,
+                                 ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(segments.data_ptr()), m,
+                                 ctypes.c_void_p(vis.data_ptr()), ctypes.byref(t)), ctx.handle, "jobs_autotune")
+    return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
+
+
 def render_jobs(ctx, jobs, integrator="pt_nee", path_pool=0, steps_per_poll=0, use_graph=True, tuning=None,
                 job_order="as_given", sample_ring=0):
     """Render many jobs through one shared pool of in-flight paths.  jobs: list

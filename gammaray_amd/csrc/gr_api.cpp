@@ -459,14 +459,15 @@ int gr_scene_get_tuning(gr_scene* s, gr_tuning* out) {
 }
 
 namespace {
-// Best of three timed passes (after one untimed) of the workload under setting tu.
-hipError_t time_workload(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* rays, uint32_t n, gr_hit* hits,
-                         const gr_segment* segs, uint32_t m, uint32_t* vis, int mode, hipEvent_t a, hipEvent_t b,
-                         float* best) {
-    const gr::TraceSetup ts{s->view, tu, &s->sort};
-    hipStream_t st = s->ctx->stream;
+// The fastest of 6 timed passes (after one untimed) of the workload under
+// setting tu, launched with `base` (a scene's view, or every job's scenes).
+hipError_t time_workload(const gr::TraceSetup& base, const gr::TraceTuning& tu, hipStream_t st, const gr_ray* rays,
+                         uint32_t n, gr_hit* hits, const gr_segment* segs, uint32_t m, uint32_t* vis, int mode,
+                         hipEvent_t a, hipEvent_t b, float* best) {
+    gr::TraceSetup ts = base;
+    ts.tu = tu;
     *best = 3.0e38f;
-    for (int rep = 0; rep < 7; ++rep) {  // the fastest of 6 timed runs after a warm-up
+    for (int rep = 0; rep < 7; ++rep) {
         hipError_t e = hipEventRecord(a, st);
         if (e == hipSuccess && n) e = gr::launch_intersect(ts, rays, n, nullptr, hits, nullptr, st);
         if (e == hipSuccess && m) e = gr::launch_occluded(ts, segs, m, nullptr, vis, mode, nullptr, st);
@@ -479,31 +480,24 @@ hipError_t time_workload(gr_scene* s, const gr::TraceTuning& tu, const gr_ray* r
     }
     return hipSuccess;
 }
-}  // namespace
 
-int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, const gr_segment* d_segs,
-                            uint32_t m, uint32_t* d_visible, int mode, gr_tuning* out) {
-    if (!s) return GR_ERR_INVALID;
-    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
-    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
-    if (n + (uint64_t)m == 0 || (n && (!d_rays || !d_hits)) || (m && (!d_segs || !d_visible)))
-        return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays with a hit buffer and/or segments with a result buffer%s");
-    if (mode != GR_OCCLUSION_EXACT && mode != GR_OCCLUSION_ANYHIT)
-        return fail(s->ctx, GR_ERR_INVALID, "unknown occlusion mode%s");
-    hipError_t e = gr::reserve_sort(s->sort, std::max(n, m));
+// Times candidate settings one knob at a time from the defaults and returns
+// the fastest in *out.  A candidate must win by more than 3 % to replace the
+// incumbent (run-to-run differences of the same setting reach 1-2 %, and a
+// smaller margin let them decide between equal settings).
+hipError_t search_tuning(const gr::TraceSetup& base, hipStream_t st, const gr_ray* d_rays, uint32_t n,
+                         gr_hit* d_hits, const gr_segment* d_segs, uint32_t m, uint32_t* d_visible, int mode,
+                         gr::TraceTuning* out) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (e == hipSuccess) e = hipEventCreate(&a);
+    hipError_t e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
     gr::TraceTuning best{};  // the defaults: sort off, XCD ranges, 8 LDS entries, occupancy limit,
                             // refill at 32 (40 for visibility), 128-ray chunks
     float best_ms = 0.0f;
     auto timed = [&](const gr::TraceTuning& tu, float* ms) {
-        return time_workload(s, tu, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
+        return time_workload(base, tu, st, d_rays, n, d_hits, d_segs, m, d_visible, mode, a, b, ms);
     };
     if (e == hipSuccess) e = timed(best, &best_ms);
-    // One knob at a time; a candidate must win by more than 3 % to replace the
-    // incumbent (run-to-run differences of the same setting reach 1-2 %, and a
-    // smaller margin let them decide between equal settings).
     auto consider = [&](gr::TraceTuning cand) {
         if (e != hipSuccess) return;
         float ms = 0.0f;
@@ -553,9 +547,83 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
     }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
+    if (e == hipSuccess) *out = best;
+    return e;
+}
+
+void tuning_out(const gr::TraceTuning& t, gr_tuning* out) {
+    out->sort_bits = t.sort_bits;
+    out->lds_stack = t.lds_stack;
+    out->waves_per_cu = t.waves_per_cu;
+    out->refill = t.refill;
+    out->xcd_ranges = t.xcd_ranges;
+    out->ray_chunk = t.ray_chunk;
+    out->vis_refill = t.vis_refill;
+}
+}  // namespace
+
+int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, const gr_segment* d_segs,
+                            uint32_t m, uint32_t* d_visible, int mode, gr_tuning* out) {
+    if (!s) return GR_ERR_INVALID;
+    if (!s->committed) return fail(s->ctx, GR_ERR_STATE, "scene not committed%s");
+    if (s->ctx->device == GR_HOST_ONLY) return fail(s->ctx, GR_ERR_STATE, "host-only context%s");
+    if (n + (uint64_t)m == 0 || (n && (!d_rays || !d_hits)) || (m && (!d_segs || !d_visible)))
+        return fail(s->ctx, GR_ERR_INVALID, "autotune needs rays with a hit buffer and/or segments with a result buffer%s");
+    if (mode != GR_OCCLUSION_EXACT && mode != GR_OCCLUSION_ANYHIT)
+        return fail(s->ctx, GR_ERR_INVALID, "unknown occlusion mode%s");
+    hipError_t e = gr::reserve_sort(s->sort, std::max(n, m));
+    gr::TraceTuning best{};
+    if (e == hipSuccess)
+        e = search_tuning(gr::TraceSetup{s->view, best, &s->sort}, s->ctx->stream, d_rays, n, d_hits, d_segs, m,
+                          d_visible, mode, &best);
     if (e != hipSuccess) return hip_fail(s->ctx, e, "autotune");
     s->tuning = best;
     return out ? gr_scene_get_tuning(s, out) : GR_OK;
+}
+
+int gr_jobs_autotune(gr_ctx* ctx, gr_scene* const* scenes, uint32_t nscenes, const gr_ray* d_rays, uint32_t n,
+                     gr_hit* d_hits, const gr_segment* d_segs, uint32_t m, uint32_t* d_visible, gr_tuning* out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    if (ctx->device == GR_HOST_ONLY) return fail(ctx, GR_ERR_STATE, "host-only context%s");
+    if (!scenes || nscenes == 0) return fail(ctx, GR_ERR_INVALID, "no scenes%s");
+    for (uint32_t k = 0; k < nscenes; ++k) {
+        if (!scenes[k] || !scenes[k]->committed) return fail(ctx, GR_ERR_STATE, "scene not committed%s");
+        if (scenes[k]->ctx != ctx) return fail(ctx, GR_ERR_INVALID, "scene of another context%s");
+    }
+    if (n + (uint64_t)m == 0 || (n && (!d_rays || !d_hits)) || (m && (!d_segs || !d_visible)))
+        return fail(ctx, GR_ERR_INVALID, "autotune needs rays with a hit buffer and/or segments with a result buffer%s");
+    // the scenes as the batch's traversal sees them
+    std::vector<gr::SceneRef> hs(nscenes);
+    for (uint32_t k = 0; k < nscenes; ++k) {
+        const gr::SceneView& v = scenes[k]->view;
+        hs[k].nodes = v.nodes;
+        hs[k].comps = v.comps;
+        hs[k].comp_ids = v.comp_ids;
+        hs[k].root_ref = v.root_ref;
+        hs[k].simple_boxes = v.simple_boxes;
+        for (int c = 0; c < 3; ++c) {
+            hs[k].root_lo[c] = v.root_lo[c];
+            hs[k].root_hi[c] = v.root_hi[c];
+        }
+    }
+    void* d_refs = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = gr::reserve_sort(scenes[0]->sort, std::max(n, m));
+    if (e == hipSuccess) e = hipMalloc(&d_refs, hs.size() * sizeof(gr::SceneRef));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_refs, hs.data(), hs.size() * sizeof(gr::SceneRef), hipMemcpyHostToDevice, ctx->stream);
+    gr::TraceTuning best{};
+    if (e == hipSuccess)
+        e = search_tuning(gr::TraceSetup{scenes[0]->view, best, &scenes[0]->sort, nullptr,
+                                         static_cast<const gr::SceneRef*>(d_refs), nscenes},
+                          ctx->stream, d_rays, n, d_hits, d_segs, m, d_visible, GR_OCCLUSION_EXACT, &best);
+    if (d_refs) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d_refs);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+    tuning_out(best, out);
+    return GR_OK;
 }
 
 int gr_scene_autotune(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hit* d_hits, gr_tuning* out) {

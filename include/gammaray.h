@@ -206,6 +206,15 @@ int gr_scene_autotune(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit 
 int gr_scene_autotune_mixed(gr_scene *scene, const gr_ray *d_rays, uint32_t n, gr_hit *d_hits,
                             const gr_segment *d_segs, uint32_t m, uint32_t *d_visible, int mode,
                             gr_tuning *out);
+/* The same search for gr_render_jobs (no reference counterpart): candidates
+ * are timed with the batch's multi-scene traversal on n rays and m visibility
+ * segments (exact mode) over the nscenes scenes, each ray and segment naming
+ * its scene (an index into `scenes`) in the pad word after its origin, as the
+ * batch's own do.  The setting is returned in *out (pass it as the batch's
+ * tuning); no scene's own setting changes. */
+int gr_jobs_autotune(gr_ctx *ctx, gr_scene *const *scenes, uint32_t nscenes, const gr_ray *d_rays, uint32_t n,
+                     gr_hit *d_hits, const gr_segment *d_segs, uint32_t m, uint32_t *d_visible,
+                     gr_tuning *out);
 
 /* ---- kernel timing (observability; no reference counterpart) -------------
  * When enabled, every traversal and ray-sort launch on the context is

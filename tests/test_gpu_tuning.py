@@ -160,3 +160,33 @@ def test_autotune_on_a_mixed_workload(with_rays, gpu_ctx):
     vis, _ = stage.visibility(torch.from_numpy(segs).cuda(), mode=GR_OCCLUSION_EXACT)
     ref = OracleScene(spec).visibility(scenes.rays6(segs))["visible"]
     assert np.array_equal(vis.cpu().numpy().astype(bool), ref.astype(bool))
+
+
+def test_jobs_autotune_on_every_scene(gpu_ctx):
+    """gr_jobs_autotune times the batch's multi-scene traversal on rays and
+    segments that name their scene: the setting is valid, leaves every
+    scene's own setting alone, and the multi-scene batch renders with it
+    the films the single renders give."""
+    import torch
+    from gammaray_amd.api import autotune_jobs, render_jobs
+    print("case jobs autotune", flush=True)
+    specs = [scenes.terrain_scene(2000, seed=61), scenes.terrain_scene(20000, seed=62)]
+    stages = [Stage(sp, ctx=gpu_ctx) for sp in specs]
+    before = [st.get_tuning() for st in stages]
+    per = 1 << 15
+    rays = np.concatenate([scenes.bounce_batch(sp, per, seed=70 + k) for k, sp in enumerate(specs)])
+    segs = np.concatenate([scenes.light_segment_batch(sp, per, seed=80 + k) for k, sp in enumerate(specs)])
+    tag = np.repeat(np.arange(len(specs), dtype=np.uint32), per).view(np.float32)
+    rays[:, 3] = tag
+    segs[:, 3] = tag
+    tu = autotune_jobs(gpu_ctx, stages, torch.from_numpy(rays).cuda(), torch.from_numpy(segs).cuda())
+    assert tu["lds_stack"] in (8, 16, 32) and tu["sort_bits"] in (0, 12, 15, 18, 21, 24, 30)
+    assert 1 <= tu["refill"] <= 64 and 1 <= tu["vis_refill"] <= 64 and tu["xcd_ranges"] in (0, 1)
+    assert [st.get_tuning() for st in stages] == before
+    jobs = [dict(stage=stages[k % 2], film=Film(gpu_ctx, 48, 40), spp=3, max_depth=5, seed=90 + k)
+            for k in range(4)]
+    render_jobs(gpu_ctx, jobs, "pt_nee", path_pool=1 << 14, steps_per_poll=4, use_graph=False, tuning=tu)
+    for j in jobs:
+        f = Film(gpu_ctx, 48, 40)
+        trace(j["stage"], f, "pt_nee", spp=3, max_depth=5, seed=j["seed"])
+        assert np.array_equal(f.readback().view(np.uint32), j["film"].readback().view(np.uint32))
