@@ -1,4 +1,5 @@
-"""CPU tests of the synthetic workload generators the bench tunes on."""
+"""CPU tests of the synthetic workload generators the bench tunes on, and of
+the guide settings' host-side checks."""
 import numpy as np
 
 from gammaray_amd import scenes
@@ -40,3 +41,18 @@ def test_light_segments_end_on_the_light():
     np.testing.assert_allclose(t[:, 1], lo[1], atol=1e-5)
     assert t[:, 0].std() > 0.25 * (hi[0] - lo[0]) and t[:, 2].std() > 0.25 * (hi[2] - lo[2])
     assert np.all(g[:, 1] < lo[1])  # starts below the light
+
+
+def test_guide_settings_are_checked_before_the_device():
+    """Unknown lobe sets and lobe choices are refused by the Python layer, and
+    a host-only context refuses a guide (no device to hold it)."""
+    import pytest
+    from gammaray_amd import _lib
+    from gammaray_amd.api import Context, Guide
+    ctx = Context(_lib.GR_HOST_ONLY)
+    with pytest.raises(ValueError):
+        Guide(ctx, lobes="transmission")
+    with pytest.raises(ValueError):
+        Guide(ctx, lobe_select="best")
+    with pytest.raises(_lib.GrError):
+        Guide(ctx, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)
