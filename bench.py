@@ -496,6 +496,21 @@ GUIDE_SWEEP = [("alpha 0.3, first vertex", dict(alpha=0.3, guided_depth=1)),
                 dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.1)),
                ("learned lobe choice (mix 0.4), roulette 0.25, alpha 0.3, first vertex",
                 dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.25)),
+               ("diffuse lobes guided, learned lobe choice (mix 0.4), roulette 0.1, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobes="diffuse", lobe_select="learned", select_mix=0.4,
+                     rr_threshold=0.1, steps_per_pass=4, train_paths=1 << 19)),
+               ("diffuse lobes guided, learned lobe choice (mix 0.4), roulette 0.1, alpha 0.5, first vertex",
+                dict(alpha=0.5, guided_depth=1, lobes="diffuse", lobe_select="learned", select_mix=0.4,
+                     rr_threshold=0.1, steps_per_pass=4, train_paths=1 << 19)),
+               ("diffuse lobes guided, learned lobe choice (mix 0.4), roulette 0.1, alpha 0.3, two vertices",
+                dict(alpha=0.3, guided_depth=2, lobes="diffuse", lobe_select="learned", select_mix=0.4,
+                     rr_threshold=0.1, steps_per_pass=4, train_paths=1 << 19)),
+               ("network off, learned lobe choice (mix 0.4), roulette 0.1",
+                dict(alpha=0.0, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.1,
+                     steps_per_pass=0, train_paths=1 << 19)),
+               ("consistent lobes guided, learned lobe choice (mix 0.4), roulette 0.1, alpha 0.3, first vertex",
+                dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.1,
+                     steps_per_pass=4, train_paths=1 << 19)),
                ("learned lobe choice (mix 0.4), roulette 0.5, alpha 0.3, first vertex",
                 dict(alpha=0.3, guided_depth=1, lobe_select="learned", select_mix=0.4, rr_threshold=0.5)),
                ("learned lobe choice (mix 0.4), roulette 0.05, alpha 0.3, first vertex, 4 Adam steps per batch",

@@ -78,7 +78,7 @@ GR_INTEGRATOR_PT, GR_INTEGRATOR_PT_NEE, GR_INTEGRATOR_BDPT, GR_INTEGRATOR_GUIDED
 GR_INTEGRATOR_DISPLAY = 4
 GR_INTEGRATOR_BDPT_MIS = 5
 GR_INTEGRATOR_BDPT_LT = 6
-GR_GUIDE_LOBES_CONSISTENT, GR_GUIDE_LOBES_ALL = 0, 1
+GR_GUIDE_LOBES_CONSISTENT, GR_GUIDE_LOBES_ALL, GR_GUIDE_LOBES_DIFFUSE = 0, 1, 2
 GR_GUIDE_SELECT_UNIFORM, GR_GUIDE_SELECT_ALBEDO, GR_GUIDE_SELECT_LEARNED = 0, 1, 2
 
 # name -> (restype, argtypes); every symbol declared in include/gammaray.h.

@@ -315,7 +315,8 @@ class Guide:
                  train_paths=1 << 16, guided_depth=0, warmup_steps=8, lobes="consistent",
                  lobe_select="uniform", select_mix=0.25, rr_threshold=0.0):
         self.ctx = ctx
-        sets = {"consistent": _lib.GR_GUIDE_LOBES_CONSISTENT, "all": _lib.GR_GUIDE_LOBES_ALL}
+        sets = {"consistent": _lib.GR_GUIDE_LOBES_CONSISTENT, "all": _lib.GR_GUIDE_LOBES_ALL,
+                "diffuse": _lib.GR_GUIDE_LOBES_DIFFUSE}
         if lobes not in sets:
             raise ValueError("guided lobe set must be one of %s" % sorted(sets))
         choices = {"uniform": _lib.GR_GUIDE_SELECT_UNIFORM, "albedo": _lib.GR_GUIDE_SELECT_ALBEDO,

@@ -859,7 +859,8 @@ int gr_guide_create(gr_ctx* ctx, const gr_guide_params* prm, gr_guide** out) {
     if (prm->batch == 0 || prm->batch % 16 != 0 || prm->batch > (1u << 22))
         return gr::set_error(ctx, GR_ERR_INVALID, "guide batch must be a positive multiple of 16 (<= 4M)");
     if (!(prm->alpha >= 0.0f && prm->alpha < 1.0f)) return gr::set_error(ctx, GR_ERR_INVALID, "alpha must be in [0, 1)");
-    if (prm->lobes != GR_GUIDE_LOBES_CONSISTENT && prm->lobes != GR_GUIDE_LOBES_ALL)
+    if (prm->lobes != GR_GUIDE_LOBES_CONSISTENT && prm->lobes != GR_GUIDE_LOBES_ALL &&
+        prm->lobes != GR_GUIDE_LOBES_DIFFUSE)
         return gr::set_error(ctx, GR_ERR_INVALID, "unknown guided lobe set");
     if (prm->lobe_select != GR_GUIDE_SELECT_UNIFORM && prm->lobe_select != GR_GUIDE_SELECT_ALBEDO &&
         prm->lobe_select != GR_GUIDE_SELECT_LEARNED)

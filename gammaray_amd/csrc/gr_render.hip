@@ -535,10 +535,12 @@ __device__ __forceinline__ bool lobe_consistent(const GpuLobe& l) {
 }
 
 // Whether a pick of lobe l may be redirected by the guide, for the guide's
-// lobe set (GR_GUIDE_LOBES_*): the consistent lobes, or every lobe with a
-// density (all but the mirror).
+// lobe set (GR_GUIDE_LOBES_*): the consistent lobes, every lobe with a
+// density (all but the mirror), or the diffuse lobes only.
 __device__ __forceinline__ bool lobe_guided(const GpuLobe& l, uint32_t set) {
-    return set == GR_GUIDE_LOBES_ALL ? l.type != kLobePerfectMirror : lobe_consistent(l);
+    if (set == GR_GUIDE_LOBES_ALL) return l.type != kLobePerfectMirror;
+    if (set == GR_GUIDE_LOBES_DIFFUSE) return l.type == kLobeDisneyDiffuse || l.type == kLobePerfectDiffuse;
+    return lobe_consistent(l);
 }
 
 __device__ __forceinline__ uint32_t guided_lobes(const GpuMaterial& m, uint32_t set) {

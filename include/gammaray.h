@@ -364,9 +364,10 @@ typedef struct {
  * but the mirror (a delta), clearcoat and transmission included, weighted with
  * the densities their samplers realise (the reported pdfs of those two are
  * not: the GTR1 pdf lacks cos(theta_h), the transmission pdf leaves its half
- * vector unnormalised). */
+ * vector unnormalised).  DIFFUSE: the diffuse lobes only. */
 #define GR_GUIDE_LOBES_CONSISTENT 0u
 #define GR_GUIDE_LOBES_ALL 1u
+#define GR_GUIDE_LOBES_DIFFUSE 2u /* Disney diffuse and perfect diffuse: the broad lobes */
 /* Lobe choice at every vertex of a guided trace.  UNIFORM: the reference's
  * (each of the n lobes with probability 1/n).  ALBEDO: lobe k with probability
  * c_k = select_mix / n + (1 - select_mix) a_k / sum a, a_k the luminance of

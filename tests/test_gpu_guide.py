@@ -232,6 +232,7 @@ def _sample_radiance(stage, film, integrator, spp, **kw):
                                                   ("transmission", "consistent", "learned"),
                                                   ("clearcoat", "all", "learned"),
                                                   ("transmission", "consistent", "learned+roulette"),
+                                                  ("transmission", "diffuse", "learned+roulette"),
                                                   ("consistent", "consistent", "uniform+roulette")])
 def test_guided_render_has_the_plain_mean(variant, lobes, select, gpu_ctx):
     """Guided and plain PT+NEE per-pixel means agree within 3 sigma (different
@@ -266,7 +267,7 @@ def test_guide_rejects_unknown_lobe_settings(gpu_ctx):
     from gammaray_amd import _lib
     from gammaray_amd.api import lib
     import ctypes
-    p = _lib.GuideParams(1, 5e-3, 0.5, 4096, 4, 4096, 0, 8, 2)
+    p = _lib.GuideParams(1, 5e-3, 0.5, 4096, 4, 4096, 0, 8, 3)
     h = ctypes.c_void_p()
     assert lib().gr_guide_create(gpu_ctx.handle, ctypes.byref(p), ctypes.byref(h)) != 0
     assert not h.value
