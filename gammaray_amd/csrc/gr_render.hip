@@ -2125,32 +2125,89 @@ hipError_t reserve_bdpt(RenderBuffers& rb, size_t npaths, uint32_t cmax, uint32_
 }  // namespace
 
 namespace {
+constexpr uint32_t kMaxStrategies = kBdptMaxDepth * (kBdptMaxDepth + 1);  // cmax (lmax + 1) at most
+
+// Lists the valid (strategy, path) pairs of a 256-thread block's paths,
+// strategy by strategy: the pairs of strategy s = i * ni + j (i < no, j < ni)
+// sit together in thread order, so a wave of the evaluation runs one
+// strategy over neighbouring paths -- uniform branches, neighbouring vertex
+// records, slots written side by side.  One atomic per block reserves the
+// block's run of `list` from `counter`; entry = (s0 + s) * chunk + q.  Every
+// thread of the block calls it; valid(i, j) is false for paths past the chunk.
+template <typename Valid>
+__device__ void list_by_strategy(uint32_t no, uint32_t ni, Valid valid, uint32_t q, uint32_t chunk, uint32_t s0,
+                                 uint32_t* counter, uint32_t* list) {
+    __shared__ uint32_t s_wave[4][kMaxStrategies];  // per wave, then the wave's offset within its strategy
+    __shared__ uint32_t s_tot[kMaxStrategies];      // per strategy, then the strategy's offset in the run
+    __shared__ uint32_t s_first;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ns = no * ni;
+    for (uint32_t i = 0, s = 0; i < no; ++i)
+        for (uint32_t j = 0; j < ni; ++j, ++s) {
+            const uint64_t m = __ballot(valid(i, j));
+            if (lane == 0) s_wave[wave][s] = (uint32_t)__popcll(m);
+        }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < ns; s += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < 4; ++w) {
+            const uint32_t c = s_wave[w][s];
+            s_wave[w][s] = run;
+            run += c;
+        }
+        s_tot[s] = run;
+    }
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the strategy totals, 64 at a time
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < ns; base += 64) {
+            const uint32_t s = base + lane;
+            const uint32_t v = s < ns ? s_tot[s] : 0u;
+            uint32_t incl = v;
+#pragma unroll
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(incl, d);
+                if (lane >= d) incl += u;
+            }
+            if (s < ns) s_tot[s] = carry + incl - v;
+            carry += __shfl(incl, 63);
+        }
+        if (lane == 0) s_first = carry ? atomicAdd(counter, carry) : 0u;
+    }
+    __syncthreads();
+    const uint32_t first = s_first;
+    for (uint32_t i = 0, s = 0; i < no; ++i)
+        for (uint32_t j = 0; j < ni; ++j, ++s) {
+            const bool v = valid(i, j);
+            const uint64_t m = __ballot(v);
+            if (v) list[first + s_tot[s] + s_wave[wave][s] + (uint32_t)__popcll(m & lanes_below())] = (s0 + s) * chunk + q;
+        }
+}
+
 // The strategies (c, l) of paths [p0, p1) that connect() evaluates -- c <=
-// nc, l <= nl, c + l <= BDPT_MAX_DEPTH, light sample not failed -- listed
-// path by path as s * chunk + (p - p0), s = (c - 1)(lmax + 1) + l.  Most of
-// the cmax (lmax + 1) strategies of a path are invalid (subpaths end early),
-// so evaluating only the listed ones keeps every lane of a wave busy.
+// nc, l <= nl, c + l <= BDPT_MAX_DEPTH, light sample not failed -- listed as
+// s * chunk + (p - p0), s = (c - 1)(lmax + 1) + l.  Most of the cmax (lmax + 1)
+// strategies of a path are invalid (subpaths end early), so evaluating only
+// the listed ones keeps every lane of a wave busy.  Light tracing's
+// strategies (0, l) go to a list of their own (after the connection
+// strategies' region), so that every wave of an evaluation takes one branch.
 __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t chunk = p1 - p0;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nc = 0, nl = 0, n = 0, nlt = 0;
+    uint32_t nc = 0, nl = 0, nlt = 0;
     if (q < chunk) {
         nl = b.nl[p0 + q];
         nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
-        for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) n += min(nl, kBdptMaxDepth - c) + 1;
         if (b.mis == 2 && !(nl & kLightFailed)) nlt = nl;  // light tracing: (0, l), l = 1..nl
     }
-    uint32_t at = block_reserve(n, &a.rb.counts[4]);
-    for (uint32_t c = 1; c <= nc && c <= kBdptMaxDepth; ++c) {
-        const uint32_t lm = min(nl, kBdptMaxDepth - c);
-        for (uint32_t l = 0; l <= lm; ++l) b.list[at++] = ((c - 1) * (b.lmax + 1) + l) * chunk + q;
-    }
-    if (b.mis == 2) {
-        // light tracing's strategies in a list of their own (after the connection
-        // strategies' region), so that every wave of an evaluation takes one branch
-        uint32_t lt = block_reserve(nlt, &a.rb.counts[5]);
-        for (uint32_t l = 1; l <= nlt; ++l) b.list[(size_t)b.lt_base * chunk + lt++] = (b.lt_base + l - 1) * chunk + q;
-    }
+    list_by_strategy(
+        b.cmax, b.lmax + 1,
+        [&](uint32_t i, uint32_t l) { return i + 1 <= nc && l <= min(nl, kBdptMaxDepth - (i + 1)); }, q, chunk, 0u,
+        &a.rb.counts[4], b.list);
+    if (b.mis == 2)
+        list_by_strategy(
+            1u, b.lmax, [&](uint32_t, uint32_t j) { return j + 1 <= nlt; }, q, chunk, b.lt_base, &a.rb.counts[5],
+            b.list + (size_t)b.lt_base * chunk);
 }
 
 // One lane per listed strategy (grid-stride over the list, whole blocks per
@@ -2158,7 +2215,9 @@ __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, 
 // light vertex seen by the camera -- without its visibility test, stores
 // contribution * weight in the strategy's slot, and queues the shadow segment
 // of every strategy that needs one (wave-compacted).  One instance per list
-// keeps each instance's registers to its own branch.
+// keeps each instance's registers to its own branch.  (A separate instance
+// for the reference's weight without the MIS branch measured slower: its
+// compiled form keeps a private stack object.)
 template <bool LT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_connect_eval(RenderArgs a, BdptArgs b,
                                                                                             uint32_t p0, uint32_t p1) {
