@@ -566,7 +566,7 @@ __device__ __forceinline__ float lobe_albedo(const GpuLobe& l, vec3 il) {
     switch (l.type) {
         case kLobeDisneyDiffuse: return dev::luminance(dev::lv3(l, 0) + dev::lv3(l, 3));
         case kLobeDisneySpecular:
-        case kLobeClearcoat: return dev::luminance(mul(dev::refl_colour(l), dev::refl_fresnel(l, c)));
+        case kLobeClearcoat: return dev::luminance(mul(dev::refl_colour(l, l.type), dev::refl_fresnel(l, l.type, c)));
         case kLobeSpecularTrans: return dev::luminance(dev::lv3(l, 3)) * (1.0f - dev::dielectric(il.z, l.v[2]));
         case kLobePerfectDiffuse: return 0.5f;
         default: return 1.0f;
@@ -664,7 +664,7 @@ __device__ __forceinline__ ChoiceEval choice_eval(const GpuMaterial& m, vec3 il,
         const GpuLobe& l = m.lobes[k];
         vec3 fk;
         float pk;
-        dev::lobe_f_pdf(l, il, ol, true, true, &fk, &pk);
+        dev::lobe_f_pdf(l, l.type, il, ol, true, true, &fk, &pk);
         e.f_sum = e.f_sum + fk;
         if (il.z != 0.0f) e.s_all += pk;
         const uint32_t kind = dev::lobe_kind(l);

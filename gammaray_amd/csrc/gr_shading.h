@@ -263,8 +263,8 @@ __device__ __forceinline__ vec3 gtr1_sample_half(float alpha, vec3 i, float u1, 
 }
 
 // ---- lobes (src/material/lobes.rs) -----------------------------------------
-__device__ __forceinline__ uint32_t lobe_kind(const GpuLobe& l) {
-    switch (l.type) {
+__device__ __forceinline__ uint32_t lobe_kind(uint32_t t) {
+    switch (t) {
         case kLobeDisneySpecular:
         case kLobeClearcoat: return kLobeGlossy | kLobeReflection;
         case kLobeSpecularTrans: return kLobeGlossy | kLobeTransmission;
@@ -276,29 +276,29 @@ __device__ __forceinline__ uint32_t lobe_kind(const GpuLobe& l) {
 __device__ __forceinline__ vec3 lv3(const GpuLobe& l, int k) { return mk(l.v[k], l.v[k + 1], l.v[k + 2]); }
 
 // microfacet distribution of a reflection lobe (GGX or GTR1)
-__device__ __forceinline__ float refl_d(const GpuLobe& l, vec3 h) {
-    return l.type == kLobeClearcoat ? gtr1_d(l.v[0], h) : ggx_d(l.v[0], l.v[1], h);
+__device__ __forceinline__ float refl_d(const GpuLobe& l, uint32_t t, vec3 h) {
+    return t == kLobeClearcoat ? gtr1_d(l.v[0], h) : ggx_d(l.v[0], l.v[1], h);
 }
-__device__ __forceinline__ float refl_g(const GpuLobe& l, vec3 i, vec3 o) {
-    if (l.type == kLobeClearcoat) return 1.0f / (1.0f + gtr1_lambda(i) + gtr1_lambda(o));
+__device__ __forceinline__ float refl_g(const GpuLobe& l, uint32_t t, vec3 i, vec3 o) {
+    if (t == kLobeClearcoat) return 1.0f / (1.0f + gtr1_lambda(i) + gtr1_lambda(o));
     return ggx_g(l.v[0], l.v[1], i, o);
 }
-__device__ __forceinline__ float refl_pdf_h(const GpuLobe& l, vec3 i, vec3 h) {
-    return l.type == kLobeClearcoat ? gtr1_d(l.v[0], h) : ggx_pdf(l.v[0], l.v[1], i, h);
+__device__ __forceinline__ float refl_pdf_h(const GpuLobe& l, uint32_t t, vec3 i, vec3 h) {
+    return t == kLobeClearcoat ? gtr1_d(l.v[0], h) : ggx_pdf(l.v[0], l.v[1], i, h);
 }
-__device__ __forceinline__ vec3 refl_fresnel(const GpuLobe& l, float c) {
-    if (l.type == kLobeClearcoat) return schlick(c, lv3(l, 1));
+__device__ __forceinline__ vec3 refl_fresnel(const GpuLobe& l, uint32_t t, float c) {
+    if (t == kLobeClearcoat) return schlick(c, lv3(l, 1));
     // DisneyFresnel: lerp(spec_colour * F_dielectric, Schlick(base), metallic)
     const vec3 die = lv3(l, 5) * dielectric(c, l.v[8]);
     const vec3 con = schlick(c, lv3(l, 2));
     return lerp3(die, con, l.v[9]);
 }
-__device__ __forceinline__ vec3 refl_colour(const GpuLobe& l) {
-    return l.type == kLobeClearcoat ? lv3(l, 4) : lv3(l, 10);
+__device__ __forceinline__ vec3 refl_colour(const GpuLobe& l, uint32_t t) {
+    return t == kLobeClearcoat ? lv3(l, 4) : lv3(l, 10);
 }
 
-__device__ __forceinline__ vec3 lobe_f(const GpuLobe& l, vec3 i, vec3 o, bool c2l) {
-    switch (l.type) {
+__device__ __forceinline__ vec3 lobe_f(const GpuLobe& l, uint32_t t, vec3 i, vec3 o, bool c2l) {
+    switch (t) {
         case kLobeDisneyDiffuse: {
             const float fi = schlick_w(fabsf(i.z));
             const float fo = schlick_w(fabsf(o.z));
@@ -319,10 +319,10 @@ __device__ __forceinline__ vec3 lobe_f(const GpuLobe& l, vec3 i, vec3 o, bool c2
             const vec3 hu = i + o;
             if (is_zero(hu) || ci == 0.0f || co == 0.0f) return mk(0.0f, 0.0f, 0.0f);
             const vec3 h = normalized(hu);
-            const vec3 fr = refl_fresnel(l, dot(o, h));
-            const float d = refl_d(l, h);
-            const float g = refl_g(l, i, o);
-            return mul(refl_colour(l), fr) * (d * g / (4.0f * co * ci));
+            const vec3 fr = refl_fresnel(l, t, dot(o, h));
+            const float d = refl_d(l, t, h);
+            const float g = refl_g(l, t, i, o);
+            return mul(refl_colour(l, t), fr) * (d * g / (4.0f * co * ci));
         }
         case kLobeSpecularTrans: {
             if (same_hemi(i, o)) return mk(0.0f, 0.0f, 0.0f);
@@ -348,13 +348,13 @@ __device__ __forceinline__ vec3 lobe_f(const GpuLobe& l, vec3 i, vec3 o, bool c2
     return mk(0.0f, 0.0f, 0.0f);
 }
 
-__device__ __forceinline__ float lobe_pdf(const GpuLobe& l, vec3 i, vec3 o) {
-    switch (l.type) {
+__device__ __forceinline__ float lobe_pdf(const GpuLobe& l, uint32_t t, vec3 i, vec3 o) {
+    switch (t) {
         case kLobeDisneySpecular:
         case kLobeClearcoat: {
             if (!same_hemi(i, o)) return 0.0f;
             const vec3 h = normalized(i + o);
-            return refl_pdf_h(l, i, h) / (4.0f * dot(i, h));
+            return refl_pdf_h(l, t, i, h) / (4.0f * dot(i, h));
         }
         case kLobeSpecularTrans: {
             if (same_hemi(i, o)) return 0.0f;
@@ -392,8 +392,8 @@ __device__ __forceinline__ float lobe_pdf(const GpuLobe& l, vec3 i, vec3 o) {
 //    reach.  The reported pdf leaves h unnormalised when its z > 0 and is
 //    zero in i's hemisphere.
 //  * The mirror: a delta, no density.
-__device__ __forceinline__ float lobe_density(const GpuLobe& l, vec3 i, vec3 o) {
-    switch (l.type) {
+__device__ __forceinline__ float lobe_density(const GpuLobe& l, uint32_t t, vec3 i, vec3 o) {
+    switch (t) {
         case kLobeClearcoat: {
             if (!same_hemi(i, o)) return 0.0f;
             const vec3 h = normalized(i + o);
@@ -415,7 +415,7 @@ __device__ __forceinline__ float lobe_density(const GpuLobe& l, vec3 i, vec3 o) 
             return ggx_pdf(l.v[0], l.v[1], i, h) * fabsf((eta * eta * oh) / (sd * sd));
         }
         case kLobePerfectMirror: return 0.0f;
-        default: return lobe_pdf(l, i, o);
+        default: return lobe_pdf(l, t, i, o);
     }
 }
 
@@ -426,13 +426,13 @@ struct LobeSample {
 
 // The direction lobe l's sample_f draws from i with (u1, u2), zero when the
 // draw fails; *h receives the reflection lobes' sampled half vector.
-__device__ __forceinline__ vec3 lobe_draw(const GpuLobe& l, vec3 i, float u1, float u2, vec3* h) {
+__device__ __forceinline__ vec3 lobe_draw(const GpuLobe& l, uint32_t t, vec3 i, float u1, float u2, vec3* h) {
     const vec3 zero = mk(0.0f, 0.0f, 0.0f);
-    switch (l.type) {
+    switch (t) {
         case kLobeDisneySpecular:
         case kLobeClearcoat: {
             if (i.z == 0.0f) return zero;
-            *h = l.type == kLobeClearcoat ? gtr1_sample_half(l.v[0], i, u1, u2)
+            *h = t == kLobeClearcoat ? gtr1_sample_half(l.v[0], i, u1, u2)
                                           : ggx_sample_half(l.v[0], l.v[1], i, u1, u2);
             const vec3 o = reflect(i, *h);
             return same_hemi(i, o) ? o : zero;
@@ -448,18 +448,18 @@ __device__ __forceinline__ vec3 lobe_draw(const GpuLobe& l, vec3 i, float u1, fl
     }
 }
 
-__device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool c2l, float u1, float u2) {
+__device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, uint32_t t, vec3 i, bool c2l, float u1, float u2) {
     LobeSample s{mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), 0.0f};
     vec3 h = s.outgoing;
-    const vec3 o = lobe_draw(l, i, u1, u2, &h);
-    if (is_zero(o) && l.type != kLobeDisneyDiffuse && l.type != kLobePerfectDiffuse) return s;
-    s.result = lobe_f(l, i, o, c2l);
+    const vec3 o = lobe_draw(l, t, i, u1, u2, &h);
+    if (is_zero(o) && t != kLobeDisneyDiffuse && t != kLobePerfectDiffuse) return s;
+    s.result = lobe_f(l, t, i, o, c2l);
     s.outgoing = o;
-    switch (l.type) {
+    switch (t) {
         case kLobeDisneySpecular:
-        case kLobeClearcoat: s.pdf = refl_pdf_h(l, i, h) / (4.0f * dot(i, h)); break;
+        case kLobeClearcoat: s.pdf = refl_pdf_h(l, t, i, h) / (4.0f * dot(i, h)); break;
         case kLobePerfectMirror: s.pdf = 1.0f; break;
-        default: s.pdf = lobe_pdf(l, i, o); break;
+        default: s.pdf = lobe_pdf(l, t, i, o); break;
     }
     return s;
 }
@@ -468,9 +468,9 @@ __device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool
 // the values of the two calls operation for operation, with the reflection
 // lobes' half vector, microfacet density and incoming masking term computed
 // once.
-__device__ __forceinline__ void lobe_f_pdf(const GpuLobe& l, vec3 i, vec3 o, bool c2l, bool want_f, vec3* f,
-                                           float* pdf) {
-    switch (l.type) {
+__device__ __forceinline__ void lobe_f_pdf(const GpuLobe& l, uint32_t t, vec3 i, vec3 o, bool c2l, bool want_f,
+                                           vec3* f, float* pdf) {
+    switch (t) {
         case kLobeDisneySpecular:
         case kLobeClearcoat: {
             *f = mk(0.0f, 0.0f, 0.0f);
@@ -479,16 +479,16 @@ __device__ __forceinline__ void lobe_f_pdf(const GpuLobe& l, vec3 i, vec3 o, boo
             if (!sh && !want_f) return;
             const vec3 hu = i + o;
             const vec3 h = normalized(hu);
-            const float d = refl_d(l, h);
-            const bool cc = l.type == kLobeClearcoat;
+            const float d = refl_d(l, t, h);
+            const bool cc = t == kLobeClearcoat;
             const float lam_i = cc ? gtr1_lambda(i) : ggx_lambda(l.v[0], l.v[1], i);
             if (want_f) {
                 const float ci = fabsf(i.z), co = fabsf(o.z);
                 if (!(is_zero(hu) || ci == 0.0f || co == 0.0f)) {
                     const float lam_o = cc ? gtr1_lambda(o) : ggx_lambda(l.v[0], l.v[1], o);
-                    const vec3 fr = refl_fresnel(l, dot(o, h));
+                    const vec3 fr = refl_fresnel(l, t, dot(o, h));
                     const float g = 1.0f / (1.0f + lam_i + lam_o);
-                    *f = mul(refl_colour(l), fr) * (d * g / (4.0f * co * ci));
+                    *f = mul(refl_colour(l, t), fr) * (d * g / (4.0f * co * ci));
                 }
             }
             if (sh) {
@@ -502,10 +502,48 @@ __device__ __forceinline__ void lobe_f_pdf(const GpuLobe& l, vec3 i, vec3 o, boo
             return;
         }
         default:
-            *f = want_f ? lobe_f(l, i, o, c2l) : mk(0.0f, 0.0f, 0.0f);
-            *pdf = lobe_pdf(l, i, o);
+            *f = want_f ? lobe_f(l, t, i, o, c2l) : mk(0.0f, 0.0f, 0.0f);
+            *pdf = lobe_pdf(l, t, i, o);
             return;
     }
+}
+
+// The lobe functions with the type read from the lobe (a walk by position).
+__device__ __forceinline__ uint32_t lobe_kind(const GpuLobe& l) { return lobe_kind(l.type); }
+__device__ __forceinline__ vec3 lobe_f(const GpuLobe& l, vec3 i, vec3 o, bool c2l) { return lobe_f(l, l.type, i, o, c2l); }
+__device__ __forceinline__ float lobe_pdf(const GpuLobe& l, vec3 i, vec3 o) { return lobe_pdf(l, l.type, i, o); }
+__device__ __forceinline__ float lobe_density(const GpuLobe& l, vec3 i, vec3 o) { return lobe_density(l, l.type, i, o); }
+__device__ __forceinline__ vec3 lobe_draw(const GpuLobe& l, vec3 i, float u1, float u2, vec3* h) {
+    return lobe_draw(l, l.type, i, u1, u2, h);
+}
+__device__ __forceinline__ LobeSample lobe_sample(const GpuLobe& l, vec3 i, bool c2l, float u1, float u2) {
+    return lobe_sample(l, l.type, i, c2l, u1, u2);
+}
+
+// Visits the lobes of m in position order, each with its type as a constant:
+// fn(k, lobe, type).  The builder emits at most one lobe of each type, in the
+// order (perfect diffuse | mirror | Disney diffuse), Disney specular,
+// clearcoat, transmission, so walking the types in that order meets the
+// positions in order (every sum keeps its operation order), while a wave runs
+// each type's code once for all its lanes wherever the lobe sits in each
+// lane's list.  A walk by position runs, at every position, the code of each
+// type some lane holds there.
+template <uint32_t T, typename Fn>
+__device__ __forceinline__ void visit_lobe(const GpuMaterial& m, uint32_t& k, Fn& fn) {
+    if (k < m.nlobes && m.lobes[k].type == T) {
+        fn(k, m.lobes[k], T);
+        ++k;
+    }
+}
+template <typename Fn>
+__device__ __forceinline__ void each_lobe(const GpuMaterial& m, Fn fn) {
+    uint32_t k = 0;
+    visit_lobe<kLobePerfectDiffuse>(m, k, fn);
+    visit_lobe<kLobePerfectMirror>(m, k, fn);
+    visit_lobe<kLobeDisneyDiffuse>(m, k, fn);
+    visit_lobe<kLobeDisneySpecular>(m, k, fn);
+    visit_lobe<kLobeClearcoat>(m, k, fn);
+    visit_lobe<kLobeSpecularTrans>(m, k, fn);
 }
 
 // ---- material (src/material/material.rs) ------------------------------------
@@ -518,11 +556,10 @@ __device__ __forceinline__ vec3 f_world(const GpuMaterial& m, vec3 wi, vec3 wo, 
     const vec3 il = to_local(wi, f), ol = to_local(wo, f);
     const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
     vec3 r = mk(0.0f, 0.0f, 0.0f);
-    for (uint32_t k = 0; k < m.nlobes; ++k) {
-        const uint32_t kind = lobe_kind(m.lobes[k]);
-        if ((refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission)))
-            r = r + lobe_f(m.lobes[k], il, ol, c2l);
-    }
+    each_lobe(m, [&](uint32_t, const GpuLobe& l, uint32_t t) {
+        const uint32_t kind = lobe_kind(t);
+        if ((refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission))) r = r + lobe_f(l, t, il, ol, c2l);
+    });
     return r;
 }
 
@@ -531,7 +568,7 @@ __device__ __forceinline__ float pdf_world(const GpuMaterial& m, vec3 wi, vec3 w
     const vec3 il = to_local(wi, f), ol = to_local(wo, f);
     if (il.z == 0.0f) return 0.0f;
     float p = 0.0f;
-    for (uint32_t k = 0; k < m.nlobes; ++k) p += lobe_pdf(m.lobes[k], il, ol);
+    each_lobe(m, [&](uint32_t, const GpuLobe& l, uint32_t t) { p += lobe_pdf(l, t, il, ol); });
     return p / (float)m.nlobes;
 }
 
@@ -541,7 +578,7 @@ __device__ __forceinline__ float sample_density_world(const GpuMaterial& m, vec3
     if (m.nlobes == 0) return 0.0f;
     const vec3 il = to_local(wi, f), ol = to_local(wo, f);
     float p = 0.0f;
-    for (uint32_t k = 0; k < m.nlobes; ++k) p += lobe_density(m.lobes[k], il, ol);
+    each_lobe(m, [&](uint32_t, const GpuLobe& l, uint32_t t) { p += lobe_density(l, t, il, ol); });
     return p / (float)m.nlobes;
 }
 
@@ -561,20 +598,19 @@ __device__ __forceinline__ FPdf f_pdf_world(const GpuMaterial& m, vec3 wi, vec3 
     const vec3 il = to_local(wi, f), ol = to_local(wo, f);
     const bool refl = (dot(wi, f.gn) * dot(wo, f.gn)) > 0.0f;
     float pf = 0.0f, pr = 0.0f;
-    for (uint32_t k = 0; k < m.nlobes; ++k) {
-        const GpuLobe& l = m.lobes[k];
-        const uint32_t kind = lobe_kind(l);
+    each_lobe(m, [&](uint32_t, const GpuLobe& l, uint32_t t) {
+        const uint32_t kind = lobe_kind(t);
         const bool with_f = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
-        switch (l.type) {
+        switch (t) {
             case kLobeDisneySpecular:
             case kLobeClearcoat: {
                 const vec3 hu = il + ol;
                 const bool sh = same_hemi(il, ol);
                 if (!sh && !with_f) break;
                 const vec3 h = normalized(hu);
-                const float d = refl_d(l, h);
+                const float d = refl_d(l, t, h);
                 float lam_i = 0.0f, lam_o = 0.0f;  // masking terms, GGX or GTR1
-                if (l.type == kLobeClearcoat) {
+                if (t == kLobeClearcoat) {
                     lam_i = gtr1_lambda(il);
                     lam_o = gtr1_lambda(ol);
                 } else {
@@ -584,13 +620,13 @@ __device__ __forceinline__ FPdf f_pdf_world(const GpuMaterial& m, vec3 wi, vec3 
                 if (with_f) {
                     const float ci = fabsf(il.z), co = fabsf(ol.z);
                     if (!(is_zero(hu) || ci == 0.0f || co == 0.0f)) {
-                        const vec3 fr = refl_fresnel(l, dot(ol, h));
+                        const vec3 fr = refl_fresnel(l, t, dot(ol, h));
                         const float g = 1.0f / (1.0f + lam_i + lam_o);
-                        r.f = r.f + mul(refl_colour(l), fr) * (d * g / (4.0f * co * ci));
+                        r.f = r.f + mul(refl_colour(l, t), fr) * (d * g / (4.0f * co * ci));
                     }
                 }
                 if (sh) {
-                    if (l.type == kLobeClearcoat) {
+                    if (t == kLobeClearcoat) {
                         pf += d / (4.0f * dot(il, h));
                         pr += d / (4.0f * dot(ol, h));
                     } else {
@@ -603,24 +639,24 @@ __device__ __forceinline__ FPdf f_pdf_world(const GpuMaterial& m, vec3 wi, vec3 
                 break;
             }
             case kLobeSpecularTrans:
-                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
-                pf += lobe_pdf(l, il, ol);
-                pr += lobe_pdf(l, ol, il);
+                if (with_f) r.f = r.f + lobe_f(l, t, il, ol, c2l);
+                pf += lobe_pdf(l, t, il, ol);
+                pr += lobe_pdf(l, t, ol, il);
                 break;
             case kLobePerfectMirror:
-                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
+                if (with_f) r.f = r.f + lobe_f(l, t, il, ol, c2l);
                 pf += 1.0f;
                 pr += 1.0f;
                 break;
             default:  // Disney diffuse, perfect diffuse: cosine densities
-                if (with_f) r.f = r.f + lobe_f(l, il, ol, c2l);
+                if (with_f) r.f = r.f + lobe_f(l, t, il, ol, c2l);
                 if (same_hemi(il, ol)) {
                     pf += fabsf(ol.z) * kInvPi;
                     pr += fabsf(il.z) * kInvPi;
                 }
                 break;
         }
-    }
+    });
     if (m.nlobes != 0) {
         if (il.z != 0.0f) r.pdf_fwd = pf / (float)m.nlobes;
         if (ol.z != 0.0f) r.pdf_rev = pr / (float)m.nlobes;
@@ -667,14 +703,14 @@ __device__ __forceinline__ MatSample sample_world_lobe(const GpuMaterial& m, vec
         // the sampled lobe's kind gates the other lobes' f (material.rs:203-210)
         const bool take = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
         // the other lobes' densities and (when taken) values, each summed in lobe order
-        for (uint32_t k = 0; k < n; ++k) {
-            if (k == r) continue;
+        each_lobe(m, [&](uint32_t k, const GpuLobe& l, uint32_t t) {
+            if (k == r) return;
             vec3 fk;
             float pk;
-            lobe_f_pdf(m.lobes[k], il, s.outgoing, c2l, take, &fk, &pk);
+            lobe_f_pdf(l, t, il, s.outgoing, c2l, take, &fk, &pk);
             pdf += pk;
             if (take) radiance = radiance + fk;
-        }
+        });
     }
     pdf /= (float)n;
     if (pdf == 0.0f) {
