@@ -529,19 +529,19 @@ __device__ __forceinline__ bool bad_sample(const dev::MatSample& smp) {
 // clearcoat pdf lacks the cos(theta_h) Jacobian factor (src/material/util.rs:
 // 266-308) and the transmission pdf uses an unnormalised half vector
 // (src/material/lobes.rs:337-339); the delta mirror has no density.
-__device__ __forceinline__ bool lobe_consistent(const GpuLobe& l) {
-    return l.type == kLobeDisneyDiffuse || l.type == kLobeDisneySpecular ||
-           l.type == kLobePerfectDiffuse;
+__device__ __forceinline__ bool lobe_consistent(uint32_t t) {
+    return t == kLobeDisneyDiffuse || t == kLobeDisneySpecular || t == kLobePerfectDiffuse;
 }
 
-// Whether a pick of lobe l may be redirected by the guide, for the guide's
-// lobe set (GR_GUIDE_LOBES_*): the consistent lobes, every lobe with a
-// density (all but the mirror), or the diffuse lobes only.
-__device__ __forceinline__ bool lobe_guided(const GpuLobe& l, uint32_t set) {
-    if (set == GR_GUIDE_LOBES_ALL) return l.type != kLobePerfectMirror;
-    if (set == GR_GUIDE_LOBES_DIFFUSE) return l.type == kLobeDisneyDiffuse || l.type == kLobePerfectDiffuse;
-    return lobe_consistent(l);
+// Whether a pick of a lobe of type t may be redirected by the guide, for the
+// guide's lobe set (GR_GUIDE_LOBES_*): the consistent lobes, every lobe with
+// a density (all but the mirror), or the diffuse lobes only.
+__device__ __forceinline__ bool lobe_guided(uint32_t t, uint32_t set) {
+    if (set == GR_GUIDE_LOBES_ALL) return t != kLobePerfectMirror;
+    if (set == GR_GUIDE_LOBES_DIFFUSE) return t == kLobeDisneyDiffuse || t == kLobePerfectDiffuse;
+    return lobe_consistent(t);
 }
+__device__ __forceinline__ bool lobe_guided(const GpuLobe& l, uint32_t set) { return lobe_guided(l.type, set); }
 
 __device__ __forceinline__ uint32_t guided_lobes(const GpuMaterial& m, uint32_t set) {
     uint32_t c = 0;
@@ -577,6 +577,10 @@ struct LobeChoice {
     float c[4];
     float m;  // the cell's mean continuation radiance (lobe_m), -1 without statistics
     uint32_t pick, bucket, octant;
+    // c[k] for a lane-varying k, through constant indices (keeps c in registers)
+    __device__ __forceinline__ float c_at(uint32_t k) const {
+        return k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3]));
+    }
 };
 // Incoming-angle bucket of the learned lobe choice: 4 bins of |cos theta_i|,
 // outside (il.z > 0) and inside.
@@ -607,26 +611,34 @@ __device__ __forceinline__ LobeChoice choose_lobe(const GpuMaterial& m, vec3 il,
             learned = true;
         }
     }
+    // loops over the 4 lobe slots with constant indices, so a and c stay in registers
     if (!learned) {
-        float a[4], sum = 0.0f;
-        for (uint32_t k = 0; k < n; ++k) {
-            a[k] = fmaxf(lobe_albedo(m.lobes[k], il), 0.0f);
-            sum += a[k];
-        }
+        float a[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sum = 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (k < n) {
+                a[k] = fmaxf(lobe_albedo(m.lobes[k], il), 0.0f);
+                sum += a[k];
+            }
         if (!(sum > 0.0f) || !isfinite(sum)) return ch;
         const float floor_c = g.select_mix / (float)n;
-        for (uint32_t k = 0; k < n; ++k) ch.c[k] = floor_c + (1.0f - g.select_mix) * (a[k] / sum);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (k < n) ch.c[k] = floor_c + (1.0f - g.select_mix) * (a[k] / sum);
     }
     const float x = u01(u);
     float acc = 0.0f;
     ch.pick = n - 1;
-    for (uint32_t k = 0; k + 1 < n; ++k) {
-        acc += ch.c[k];
-        if (x < acc) {
-            ch.pick = k;
-            break;
+    bool found = false;
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k)
+        if (!found && k + 1 < n) {
+            acc += ch.c[k];
+            if (x < acc) {
+                ch.pick = k;
+                found = true;
+            }
         }
-    }
     return ch;
 }
 
@@ -660,29 +672,30 @@ __device__ __forceinline__ ChoiceEval choice_eval(const GpuMaterial& m, vec3 il,
     e.f_sum = e.f_pick = e.own = mk(0.0f, 0.0f, 0.0f);
     e.s_all = e.s_take = e.c_g = e.cs_g = 0.0f;
     e.take_pick = false;
-    for (uint32_t k = 0; k < m.nlobes; ++k) {
-        const GpuLobe& l = m.lobes[k];
+    // lobe by lobe in position order (a walk by type: the lane's c_k selected
+    // with constant indices, which keeps c in registers)
+    dev::each_lobe(m, [&](uint32_t k, const GpuLobe& l, uint32_t t) {
         vec3 fk;
         float pk;
-        dev::lobe_f_pdf(l, l.type, il, ol, true, true, &fk, &pk);
+        dev::lobe_f_pdf(l, t, il, ol, true, true, &fk, &pk);
         e.f_sum = e.f_sum + fk;
         if (il.z != 0.0f) e.s_all += pk;
-        const uint32_t kind = dev::lobe_kind(l);
+        const uint32_t kind = dev::lobe_kind(t);
         const bool take = (refl && (kind & kLobeReflection)) || (!refl && (kind & kLobeTransmission));
         if (k == pick) {
             e.f_pick = fk;
             e.take_pick = take;
         }
-        if (!in_g || !lobe_guided(l, set)) continue;
-        const float sk = (l.type == kLobeClearcoat || l.type == kLobeSpecularTrans) ? dev::lobe_density(l, il, ol)
-                                                                                   : pk;
-        e.c_g += c[k];
-        e.cs_g += c[k] * sk;
+        if (!in_g || !lobe_guided(t, set)) return;
+        const float sk = (t == kLobeClearcoat || t == kLobeSpecularTrans) ? dev::lobe_density(l, t, il, ol) : pk;
+        const float ck = k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3]));
+        e.c_g += ck;
+        e.cs_g += ck * sk;
         if (take)
             e.s_take += sk;
         else
             e.own = e.own + fk * sk;
-    }
+    });
     return e;
 }
 
@@ -901,7 +914,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                             out_dir = dev::to_world(dev::lobe_draw(m.lobes[pick], il, u01(r0.y), u01(r0.z), &hh), f);
                         }
                         const GpuLobe& lp = m.lobes[pick];
-                        const float nc = (float)m.nlobes * ch.c[pick];
+                        const float nc = (float)m.nlobes * ch.c_at(pick);
                         vec3 g = mk(0.0f, 0.0f, 0.0f);
                         float pdf_rep = 1.0f;
                         bool spec = false;
@@ -949,7 +962,7 @@ __device__ __forceinline__ void shade_path(const RenderArgs& a, const SceneView&
                             vr[3] = make_float4(L.y, L.z, dev::luminance(T_in), split ? 1.0f : 0.0f);
                             vr[4] = make_float4(__uint_as_float(h.prim),
                                                 __uint_as_float(pick | (ch.bucket << 4) | (ch.octant << 8)),
-                                                (float)m.nlobes * ch.c[pick], lobe_rec ? 1.0f : 0.0f);
+                                                (float)m.nlobes * ch.c_at(pick), lobe_rec ? 1.0f : 0.0f);
                         }
                     }
                     new_o = hp + out_dir * kRayPush;
@@ -1708,15 +1721,17 @@ __device__ Connection connect_mis(const RenderArgs& a, const BdptArgs& b, uint32
     jd->c_fwd = fc.pdf_fwd;
     jd->c_rev = fc.pdf_rev;
     vec3 ce;
+    float l_fwd = 0.0f, l_rev = 0.0f;
     if (l == 1) {
         ce = dev::light_world(a.sh.mats[lv.prim], l2c, lv.f);
-        jd->l_fwd = jd->l_rev = 0.0f;
     } else {
         const dev::FPdf fl = dev::f_pdf_world(a.sh.mats[lv.prim], lv.wi, l2c, lv.f, false);
         ce = fl.f;
-        jd->l_fwd = fl.pdf_fwd;
-        jd->l_rev = fl.pdf_rev;
+        l_fwd = fl.pdf_fwd;
+        l_rev = fl.pdf_rev;
     }
+    jd->l_fwd = l_fwd;
+    jd->l_rev = l_rev;
     const float g = fabsf(dot(cv.f.n, c2l));
     const float cd = convert_density(cv.pt, lv.pt, lv.f);
     r.contrib = mul(mul(mul(cv.T, fc.f), ce), lv.T) * (g * cd);
@@ -1801,7 +1816,7 @@ __device__ __forceinline__ bool lt_project(const LtCamera& c, vec3 y, float& cd,
 // too (all k vertices from the light walk, z_1 joined to the camera, which
 // counts as connectible), and c = 0 is allowed: z_1 = y_l seen by the camera.
 __device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, uint32_t c, uint32_t l,
-                            const JoinDensities* jd = nullptr) {
+                            bool have_jd = false, JoinDensities jd = JoinDensities{}) {
     const size_t np = a.npaths;
     const BVert* cam = b.cam + p;  // camera vertex i at cam[(i - 1) * np]
     const BVert* lig = b.lig + p;  // light vertex m at lig[(m - 1) * np]
@@ -1838,17 +1853,17 @@ __device__ float mis_weight(const RenderArgs& a, const BdptArgs& b, uint32_t p, 
             const VertView yl = load_vert(lig + (size_t)(l - 1) * np);
             const GpuMaterial& mc = a.sh.mats[xc.prim];
             const vec3 d_cy = normalized(yl.pt - xc.pt), d_yc = -d_cy;
-            pf_c1 = (jd ? jd->c_fwd : dev::pdf_world(mc, xc.wi, d_cy, xc.f)) * to_area(xc.pt, yl.pt, yl.f.n);
+            pf_c1 = (have_jd ? jd.c_fwd : dev::pdf_world(mc, xc.wi, d_cy, xc.f)) * to_area(xc.pt, yl.pt, yl.f.n);
             if (l == 1) {
                 pl_c = emit_dir_pdf(yl.f, d_yc) * to_area(yl.pt, xc.pt, xc.f.n);
             } else {
                 const GpuMaterial& my = a.sh.mats[yl.prim];
-                pl_c = (jd ? jd->l_fwd : dev::pdf_world(my, yl.wi, d_yc, yl.f)) * to_area(yl.pt, xc.pt, xc.f.n);
+                pl_c = (have_jd ? jd.l_fwd : dev::pdf_world(my, yl.wi, d_yc, yl.f)) * to_area(yl.pt, xc.pt, xc.f.n);
                 const float4 qb = lig[(size_t)(l - 2) * np].b, qc = lig[(size_t)(l - 2) * np].c;
-                pf_c2 = (jd ? jd->l_rev : dev::pdf_world(my, d_yc, yl.wi, yl.f)) *
+                pf_c2 = (have_jd ? jd.l_rev : dev::pdf_world(my, d_yc, yl.wi, yl.f)) *
                         to_area(yl.pt, mk(qb.x, qb.y, qb.z), mk(qc.x, qc.y, qc.z));
             }
-            if (c >= 2) pl_c1 = (jd ? jd->c_rev : dev::pdf_world(mc, d_cy, xc.wi, xc.f)) * to_area(xc.pt, xp, xpn);
+            if (c >= 2) pl_c1 = (have_jd ? jd.c_rev : dev::pdf_world(mc, d_cy, xc.wi, xc.f)) * to_area(xc.pt, xp, xpn);
             conn_c = xc.conn;
             conn_c1 = yl.conn;
         }
@@ -2247,9 +2262,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     cn = connect(a, b, p, c, l);
                     w = 1.0f / (float)(c + l);  // weight, integrators.rs:345-354
                 } else {
-                    JoinDensities jd;
+                    JoinDensities jd{};  // set by connect_mis when l > 0
                     cn = connect_mis(a, b, p, c, l, &jd);
-                    w = dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l, l > 0 ? &jd : nullptr);
+                    w = dev::is_zero(cn.contrib) ? 0.0f : mis_weight(a, b, p, c, l, l > 0, jd);
                 }
                 cw = cn.contrib * w;
                 emit = l > 0 && cn.needs_vis;
