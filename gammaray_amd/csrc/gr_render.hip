@@ -1476,8 +1476,8 @@ struct BVert {
 struct BdptArgs {
     BVert* cam;   // [cmax][npaths]
     BVert* lig;   // [lmax][npaths]
-    uint32_t* nc;
-    uint32_t* nl;        // light vertex count; bit 31 set = the light sample failed
+    uint32_t* nc;        // camera vertex count | kLastDead
+    uint32_t* nl;        // light vertex count | kLastDead, or kLightFailed
     uint32_t* mask;      // visibility bits per path, mwords words each
     uint2* entry;        // shadow queue: (path, strategy)
     uint32_t* list;      // the chunk's valid strategies, as s * chunk + (path - p0)
@@ -1490,7 +1490,10 @@ struct BdptArgs {
 };
 
 constexpr uint32_t kBdptMaxDepth = 16;    // BDPT_MAX_DEPTH, integrators.rs:103
-constexpr uint32_t kLightFailed = 0x80000000u;
+// Per-path subpath words nc (camera) and nl (light): a vertex count and flags.
+constexpr uint32_t kVertCount = 0xffffu;        // the vertex count
+constexpr uint32_t kLightFailed = 0x80000000u;  // nl: the light sample failed (no light subpath)
+constexpr uint32_t kLastDead = 0x40000000u;     // the subpath's last vertex is not connectible
 
 __device__ __forceinline__ void store_vert(BVert* v, vec3 wi, uint32_t prim, vec3 pt, bool conn, const Frame& f,
                                            vec3 T, vec3 le, float a_fwd = 0.0f) {
@@ -1565,6 +1568,7 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
         const uint4 hv = a.rb.hits[i];
         const gr_hit h = {hv.x, hv.y, __uint_as_float(hv.z), hv.w};
         const vec3 o = ld3(a.rb.rays[cur][2 * (size_t)i]), d = ld3(a.rb.rays[cur][2 * (size_t)i + 1]);
+        bool vconn = false;  // this vertex connectible (an escaped ray's is not)
         const float4 T4 = a.rb.T[p];
         vec3 T = ld3(T4);
         const float prev_pdf = T4.w;  // solid-angle density of d at the previous vertex
@@ -1579,6 +1583,7 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
             const dev::MatSample smp = dev::sample_world(m, wi, f, CAMERA, r0.x, u01(r0.y), u01(r0.z));
             note_stat(a.stats, kStatNonfiniteBsdf, bad_sample(smp));
             const bool conn = dev::count_lobes(m, kLobeDiffuse | kLobeGlossy) != 0;
+            vconn = conn;
             float a_fwd = 0.0f;
             if (b.mis && k > 0) {
                 // area densities for the MIS weights: this vertex from its predecessor,
@@ -1611,8 +1616,11 @@ __global__ __launch_bounds__(256) void k_walk(RenderArgs a, BdptArgs b, uint32_t
             store_vert(v, -d, kNone, mk(0.0f, 0.0f, 0.0f), false, z, T, T);
         }
         a.rb.T[p] = make_float4(T.x, T.y, T.z, next_pdf);
-        if (CAMERA) b.nc[p] = k + 1;
-        else b.nl[p] = k + 1;
+        // the last walk step to reach p writes the final word: its count and
+        // whether its last vertex connects
+        const uint32_t word = (k + 1) | (vconn ? 0u : kLastDead);
+        if (CAMERA) b.nc[p] = word;
+        else b.nl[p] = word;
     }
     const uint32_t nxt = cur ^ 1u;
     const uint32_t slot = block_append(alive, &a.rb.counts[nxt]);
@@ -1941,8 +1949,9 @@ __device__ __forceinline__ unsigned long long splat_fixed(float v) {
 __global__ __launch_bounds__(256) void k_lt_splat(RenderArgs a, BdptArgs b, uint32_t p0, uint32_t p1) {
     const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= p1) return;
-    const uint32_t nl = b.nl[p];
-    if (nl & kLightFailed) return;
+    const uint32_t wl = b.nl[p];
+    if (wl & kLightFailed) return;
+    const uint32_t nl = wl & kVertCount;
     const uint32_t chunk = p1 - p0, q = p - p0;
     const LtCamera lc = lt_camera(a);
     const int W = (int)a.width, H = (int)a.height;
@@ -2025,10 +2034,11 @@ __global__ __launch_bounds__(256) void k_connect_sum(RenderArgs a, BdptArgs b, u
     const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= p1) return;
     const uint32_t chunk = p1 - p0, q = p - p0;
-    uint32_t nc = b.nc[p];
-    const uint32_t nl = b.nl[p];
+    const uint32_t wl = b.nl[p];
+    uint32_t nc = b.nc[p] & kVertCount;
+    const uint32_t nl = wl & kVertCount;
     vec3 L = mk(0.0f, 0.0f, 0.0f);
-    if (nl & kLightFailed) nc = 0;
+    if (wl & kLightFailed) nc = 0;
     for (uint32_t c = 1; c < nc + 1; ++c) {
         for (uint32_t l = 0; l < nl + 1; ++l) {
             if (c + l > kBdptMaxDepth) continue;
@@ -2210,19 +2220,41 @@ __global__ __launch_bounds__(256) void k_connect_list(RenderArgs a, BdptArgs b, 
     const uint32_t chunk = p1 - p0;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nc = 0, nl = 0, nlt = 0;
+    uint32_t c_dead = kNone, l_dead = kNone;  // a subpath's last vertex when it is not connectible
     if (q < chunk) {
-        nl = b.nl[p0 + q];
-        nc = (nl & kLightFailed) ? 0u : b.nc[p0 + q];
-        if (b.mis == 2 && !(nl & kLightFailed)) nlt = nl;  // light tracing: (0, l), l = 1..nl
+        const uint32_t wl = b.nl[p0 + q];
+        if (!(wl & kLightFailed)) {
+            const uint32_t wc = b.nc[p0 + q];
+            nc = wc & kVertCount;
+            nl = wl & kVertCount;
+            if (wc & kLastDead) c_dead = nc;
+            if (wl & kLastDead) l_dead = nl;
+            if (b.mis == 2) nlt = nl;  // light tracing: (0, l), l = 1..nl
+        }
     }
+    // A strategy joining a non-connectible vertex (an escaped ray's, one on a
+    // light) contributes zero without a visibility test: its slot is written
+    // here and it is not listed, so the evaluation's waves hold live
+    // strategies only.  Only a subpath's last vertex is checked (that is
+    // where escapes and light hits end a walk).
+    for (uint32_t c = 1; c <= nc; ++c) {
+        const uint32_t lm = min(nl, kBdptMaxDepth - c);
+        for (uint32_t l = 1; l <= lm; ++l)
+            if (c == c_dead || l == l_dead)
+                b.slots[(size_t)((c - 1) * (b.lmax + 1) + l) * chunk + q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (l_dead != kNone && l_dead <= nlt)
+        b.slots[(size_t)(b.lt_base + l_dead - 1) * chunk + q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     list_by_strategy(
         b.cmax, b.lmax + 1,
-        [&](uint32_t i, uint32_t l) { return i + 1 <= nc && l <= min(nl, kBdptMaxDepth - (i + 1)); }, q, chunk, 0u,
-        &a.rb.counts[4], b.list);
+        [&](uint32_t i, uint32_t l) {
+            return i + 1 <= nc && l <= min(nl, kBdptMaxDepth - (i + 1)) && (l == 0 || (i + 1 != c_dead && l != l_dead));
+        },
+        q, chunk, 0u, &a.rb.counts[4], b.list);
     if (b.mis == 2)
         list_by_strategy(
-            1u, b.lmax, [&](uint32_t, uint32_t j) { return j + 1 <= nlt; }, q, chunk, b.lt_base, &a.rb.counts[5],
-            b.list + (size_t)b.lt_base * chunk);
+            1u, b.lmax, [&](uint32_t, uint32_t j) { return j + 1 <= nlt && j + 1 != l_dead; }, q, chunk, b.lt_base,
+            &a.rb.counts[5], b.list + (size_t)b.lt_base * chunk);
 }
 
 // One lane per listed strategy (grid-stride over the list, whole blocks per
