@@ -355,13 +355,22 @@ def _parse_overrides(text):
     return out
 
 
-def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64, overrides=None):
-    """Learned path guide on the 1080p showcase scene: wall time for guided
-    rendering (online training included) to reach the relative MSE that plain
-    PT+NEE reaches with 64 spp, against a ref_spp PT+NEE render."""
+GUIDED_SCENES = {
+    # name: (scene builder, width, height, description)
+    "showcase": ("showcase_scene", 1920, 1080, "showcase 1920x1080, PT+NEE depth 8"),
+    "room": ("indirect_room_scene", 1440, 1440, "indirect-light room 1440x1440 (light facing the ceiling), PT+NEE depth 8"),
+}
+
+
+def bench_guided(ctx, ref_spp=4096, spp=64, overrides=None, scene="showcase"):
+    """Learned path guide: wall time for guided rendering (online training
+    included) to reach the relative MSE that plain PT+NEE reaches with 64 spp,
+    against a ref_spp PT+NEE render, on the 1080p showcase scene or on the
+    indirectly lit room; and the same with the network switched off."""
     from gammaray_amd import scenes
     from gammaray_amd.api import Film, Guide, Stage, trace
-    spec = scenes.showcase_scene()
+    builder, W, H, label = GUIDED_SCENES[scene]
+    spec = getattr(scenes, builder)()
     st = Stage(spec, ctx=ctx)
 
     def image(f):
@@ -428,7 +437,7 @@ def bench_guided(ctx, ref_spp=4096, W=1920, H=1080, spp=64, overrides=None):
     off = dict(cfg, alpha=0.0, steps_per_pass=0)
     curve_off, _, reached_off = run(off)
     log("guided without the network: reaches the relMSE at %s s" % ("%.3f" % reached_off if reached_off else "--"))
-    return {"scene": "showcase 1920x1080, PT+NEE depth 8", "reference_spp": ref_spp,
+    return {"scene": label, "reference_spp": ref_spp,
             "reference_s": round(t_ref, 3),
             "error": "relMSE with the 0.1% largest pixel errors dropped (the time ratio).  The untrimmed RMSE is "
                      "reported beside it; single-pixel fireflies of one render or the other dominate it, so its "
@@ -603,6 +612,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
     ap.add_argument("--guided-only", action="store_true",
                     help="only the guided-vs-plain comparison on the 1080p scene, printed as JSON")
+    ap.add_argument("--guided-scene", choices=sorted(GUIDED_SCENES), default="showcase",
+                    help="with --guided-only: the scene of the guided comparison")
     ap.add_argument("--guided-config", default="",
                     help="k=v overrides of the guide settings of the guided comparison, e.g. train_paths=262144")
     ap.add_argument("--sweep-only", default="",
@@ -652,7 +663,8 @@ def main():
     if args.guided_only:
         ctx = Context(0)
         ctx.set_stream(stream.cuda_stream)
-        print(json.dumps(bench_guided(ctx, ref_spp=args.ref_spp, overrides=_parse_overrides(args.guided_config))),
+        print(json.dumps(bench_guided(ctx, ref_spp=args.ref_spp, overrides=_parse_overrides(args.guided_config),
+                                      scene=args.guided_scene)),
               flush=True)
         return
     if args.guide_sweep:
