@@ -169,19 +169,31 @@ PATH_WORKLOADS = [
 ]
 
 
-def bench_paths(ctx, stream, cpu, threads, only=None):
+def bench_paths(ctx, stream, cpu, threads, only=None, tune=False):
     """Path-tracing throughput (Mpaths/s, one camera sample = one path) of the
     render workloads, GPU time by stream events around gr_trace, plus the
-    oracle's throughput on a bounded number of iterations of the same frame."""
+    oracle's throughput on a bounded number of iterations of the same frame.
+    With tune, each scene's traversal setting is first chosen by the autotuner
+    on 2^22 bounce rays and 2^22 light segments of that scene."""
     import torch
     from gammaray_amd import scenes
     from gammaray_amd.api import Film, Stage, trace
     out = {}
+    tuned = {}
     for name, factory, W, H, spp, depth, integ, ldepth, cpu_spp in PATH_WORKLOADS:
         if only and name not in only:
             continue
         spec = getattr(scenes, factory)()
         stage = Stage(spec, ctx=ctx)
+        if tune:
+            if factory not in tuned:
+                t0 = time.time()
+                rays = torch.from_numpy(scenes.bounce_batch(spec, 1 << 22, seed=11)).cuda()
+                segs = torch.from_numpy(scenes.light_segment_batch(spec, 1 << 22, seed=12)).cuda()
+                tuned[factory] = stage.autotune(rays, segs)
+                del rays, segs
+                log("%s: autotuned %s (%.1f s)" % (factory, tuned[factory], time.time() - t0))
+            stage.set_tuning(**tuned[factory])
         film = Film(ctx, W, H)
         trace(stage, film, integ, spp=1, max_depth=depth, light_depth=ldepth, seed=1)  # warm
         torch.cuda.synchronize()
@@ -194,7 +206,7 @@ def bench_paths(ctx, stream, cpu, threads, only=None):
         ms = e0.elapsed_time(e1)
         rec = {"mpaths_s": round(W * H * spp / (ms * 1e-3) / 1e6, 2), "ms": round(ms, 3), "width": W,
                "height": H, "spp": spp, "max_depth": depth, "integrator": integ,
-               "primitives": spec.num_components}
+               "primitives": spec.num_components, "tuning": stage.get_tuning()}
         if ldepth:
             rec["light_depth"] = ldepth
         if cpu:
@@ -607,6 +619,9 @@ def main():
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
     ap.add_argument("--paths", default="",
                     help="comma-separated subset of the path workloads (%s)" % ", ".join(w[0] for w in PATH_WORKLOADS))
+    ap.add_argument("--path-tuning", choices=["default", "auto"], default="auto",
+                    help="traversal setting of the path workloads: the library's default, or autotuned per scene "
+                         "on its bounce rays and light segments")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guide comparison")
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the guided comparison")
@@ -799,7 +814,8 @@ def main():
     del rays, segs, hits, vis, rays8, segs8
     paths = None
     if not args.no_paths:
-        paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads, only=only or None)
+        paths = bench_paths(ctx, stream, not args.no_cpu_baseline, threads, only=only or None,
+                            tune=args.path_tuning == "auto")
     batch = None if args.no_batch else bench_batch(ctx, threads)
     guided = None if args.no_guided else bench_guided(ctx, ref_spp=args.ref_spp,
                                                       overrides=_parse_overrides(args.guided_config))
