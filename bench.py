@@ -619,7 +619,7 @@ def main():
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing workloads")
     ap.add_argument("--paths", default="",
                     help="comma-separated subset of the path workloads (%s)" % ", ".join(w[0] for w in PATH_WORKLOADS))
-    ap.add_argument("--path-tuning", choices=["default", "auto"], default="auto",
+    ap.add_argument("--path-tuning", choices=["default", "auto"], default="default",
                     help="traversal setting of the path workloads: the library's default, or autotuned per scene "
                          "on its bounce rays and light segments")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batched workload")
