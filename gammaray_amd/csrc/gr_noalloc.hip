@@ -70,7 +70,7 @@ extern "C" int gr_intersect_closest_noalloc(gr_scene* s, const gr_ray* d_rays, u
     void* base = nullptr;
     {
         std::lock_guard<std::mutex> lock(g_mu);
-        if (s->ctx->device < kMaxDevices) base = g_ptr[s->ctx->device];
+        if (s->ctx->device >= 0 && s->ctx->device < kMaxDevices) base = g_ptr[s->ctx->device];
     }
     if (!base) return fail(s->ctx, GR_ERR_STATE, "device scratch not set up for this context's device");
     char* p = static_cast<char*>(base);
