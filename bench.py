@@ -206,7 +206,8 @@ def bench_paths(ctx, stream, cpu, threads, only=None, tune=False):
         ms = e0.elapsed_time(e1)
         rec = {"mpaths_s": round(W * H * spp / (ms * 1e-3) / 1e6, 2), "ms": round(ms, 3), "width": W,
                "height": H, "spp": spp, "max_depth": depth, "integrator": integ,
-               "primitives": spec.num_components, "tuning": stage.get_tuning()}
+               "primitives": spec.num_components,
+               "tuning": tuned.get(factory) or "the library's integrator default (refill 16, vis_refill 16, ray_chunk 64)"}
         if ldepth:
             rec["light_depth"] = ldepth
         if cpu:

@@ -443,6 +443,7 @@ int gr_scene_set_tuning(gr_scene* s, const gr_tuning* t) {
     s->tuning.xcd_ranges = t->xcd_ranges;
     s->tuning.ray_chunk = t->ray_chunk ? t->ray_chunk : gr::kDefaultRayChunk;
     s->tuning.vis_refill = t->vis_refill ? t->vis_refill : t->refill;
+    s->render_tuning = s->tuning;  // a setting given applies to every launch on the scene
     return GR_OK;
 }
 
@@ -578,6 +579,7 @@ int gr_scene_autotune_mixed(gr_scene* s, const gr_ray* d_rays, uint32_t n, gr_hi
                           d_visible, mode, &best);
     if (e != hipSuccess) return hip_fail(s->ctx, e, "autotune");
     s->tuning = best;
+    s->render_tuning = best;
     return out ? gr_scene_get_tuning(s, out) : GR_OK;
 }
 

@@ -2344,7 +2344,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     hipLaunchKernelGGL(k_camera, blocks_for(npaths), dim3(256), 0, s, a);
     uint32_t cur = 0;
     for (uint32_t k = 0; k < cmax && e == hipSuccess; ++k) {
-        e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
+        e = launch_intersect(render_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
                              reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
         hipLaunchKernelGGL(k_walk<true>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
@@ -2355,7 +2355,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
     hipLaunchKernelGGL(k_light_start, blocks_for(npaths), dim3(256), 0, s, a, b);
     cur = 0;
     for (uint32_t k = 1; k < lmax && e == hipSuccess; ++k) {
-        e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
+        e = launch_intersect(render_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), (uint32_t)npaths, cnt + cur,
                              reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, cnt, cur ^ 1u);
         hipLaunchKernelGGL(k_walk<false>, blocks_for(npaths), dim3(256), 0, s, a, b, k, cur);
@@ -2373,7 +2373,7 @@ hipError_t bdpt_batch(gr_scene* sc, RenderArgs a, gr_film* film, uint32_t cmax, 
         if (mis == 2)
             hipLaunchKernelGGL(k_connect_eval<true>, dim3(std::min<unsigned>(blocks_for((p1 - p0) * lmax).x, 8192u)),
                                dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
-        e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
+        e = launch_occluded(render_setup(sc), reinterpret_cast<const gr_segment*>(seg), (uint32_t)shadow, cnt + 2,
                             vis, GR_OCCLUSION_EXACT, nullptr, s);
         hipLaunchKernelGGL(k_vis_scatter, blocks_for(shadow), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(k_connect_sum, blocks_for(p1 - p0), dim3(256), 0, s, a, b, (uint32_t)p0, (uint32_t)p1);
@@ -2565,7 +2565,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
         hipLaunchKernelGGL(k_camera, blocks_for(a.npaths), dim3(256), 0, s, a);
         uint32_t cur = 0;
         if (prm.integrator == GR_INTEGRATOR_DISPLAY) {
-            e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[0]), a.npaths,
+            e = launch_intersect(render_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[0]), a.npaths,
                                  nullptr, reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL(k_display, blocks_for(a.npaths), dim3(256), 0, s, a);
@@ -2581,7 +2581,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
                 if (e == hipSuccess) e = hipStreamSynchronize(s);
                 if (e != hipSuccess || live == 0) break;
             }
-            e = launch_intersect(trace_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), a.npaths, film->rb.counts + cur,
+            e = launch_intersect(render_setup(sc), reinterpret_cast<const gr_ray*>(film->rb.rays[cur]), a.npaths, film->rb.counts + cur,
                              reinterpret_cast<gr_hit*>(film->rb.hits), nullptr, s);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, film->rb.counts, cur ^ 1u);
@@ -2601,7 +2601,7 @@ hipError_t render_trace(gr_scene* sc, const gr_camera& cam, const gr_trace_param
                 } else {
                     hipLaunchKernelGGL((k_shade<true, false, false>), blocks_for(a.npaths), dim3(256), 0, s, a, depth, cur);
                 }
-                e = launch_occluded(trace_setup(sc), reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
+                e = launch_occluded(render_setup(sc), reinterpret_cast<const gr_segment*>(film->rb.seg), a.npaths,
                                     film->rb.counts + 2, film->rb.vis, GR_OCCLUSION_EXACT, nullptr, s);
                 if (e != hipSuccess) break;
                 if (guided)
@@ -2927,7 +2927,7 @@ hipError_t render_jobs(gr_ctx* ctx, const gr_job* jobs, uint32_t njobs, const gr
             hs[k].root_hi[c] = v.root_hi[c];
         }
     }
-    TraceTuning tu = scenes[0]->tuning;
+    TraceTuning tu = scenes[0]->render_tuning;
     if (prm.tuning.lds_stack != 0) {
         tu.sort_bits = sort_key_bits(prm.tuning.sort_bits);
         tu.lds_stack = prm.tuning.lds_stack;

@@ -191,6 +191,11 @@ typedef struct {
     uint32_t vis_refill;   /* idle lanes (1..64) at which a visibility wave fetches new
                               segments (default 40); 0 = refill */
 } gr_tuning;
+/* A new scene's ray batches (gr_intersect_*, gr_visibility) run the defaults
+ * above; its integrators (gr_trace, gr_render_jobs without a tuning) run
+ * refill 16, vis_refill 16, ray_chunk 64, which suit their shrinking bounce
+ * queues.  gr_scene_set_tuning and gr_scene_autotune set the one setting every
+ * launch on the scene then runs; gr_scene_get_tuning reports the ray batches'. */
 int gr_scene_set_tuning(gr_scene *scene, const gr_tuning *tuning);
 int gr_scene_get_tuning(gr_scene *scene, gr_tuning *out);
 /* Times candidate settings on n rays (device memory; closest hit) and keeps the
